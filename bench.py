@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Benchmark of the res_calc edge loop on one MI355X.
+
+Workload: the synthetic airfoil quad mesh of 1200 x 600 cells (721 801 nodes,
+720 000 cells, 1 438 200 interior edges), nodes / cells / edges shuffled with
+seed 0, FP64 q = freestream * (1 + 1e-2 N(0,1)), adt ~ U[0.5, 1.5].
+
+A step is one res_calc iteration over all edges: the increments of every edge
+added into res on the device (the reference's timed region, problem.hpp:
+269-304; its out->in copy is timed separately there and is a no-op for
+res_calc, whose user function never reads res).  Inputs are resident in HBM.
+
+Prints ONE JSON line (rank 0).  --gpus above 1 is refused: this build runs
+on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# reference unique-bytes formula (scripts/visualise_paper.py:19-22):
+# x 16 B/node + q,adt 40 B/cell + res 32 B read + 32 B write per cell +
+# 2 maps x 2 x 4 B per edge
+PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def unique_bytes(num_nodes: int, num_cells: int, num_edges: int) -> int:
+    return 16 * num_nodes + 40 * num_cells + 64 * num_cells + 16 * num_edges
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=1200)
+    ap.add_argument("--ny", type=int, default=600)
+    ap.add_argument("--strategy", default="hierarchical",
+                    choices=["atomic", "global", "hierarchical"])
+    ap.add_argument("--block-size", type=int, default=512)
+    ap.add_argument("--partition", default="kway", choices=["none", "kway", "ours", "gps"])
+    ap.add_argument("--no-stage", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-atomic-ref", action="store_true")
+    return ap.parse_args()
+
+
+def build_problem(args, partition_kind):
+    from cuda_fot_amd import meshgen, res_calc_problem
+    mesh = meshgen.airfoil(args.nx, args.ny, seed=0, shuffle=7)
+    p = res_calc_problem(mesh, block_size=args.block_size)
+    if partition_kind == "gps":
+        p.reorder("gps")
+    elif partition_kind == "kway":
+        p.reorder("gps")
+        p.partition(1.001)
+        p.reorder_to_partition()
+        p.renumber_points()
+    elif partition_kind == "ours":
+        p.partition_ours(1.001, 8)
+        p.reorder_to_partition()
+        p.renumber_points()
+    return mesh, p
+
+
+def make_loop(p, strategy, block_size, stage, graph):
+    from cuda_fot_amd import DeviceArray, Loop, capi
+    strat = {"atomic": capi.STRATEGY_ATOMIC, "global": capi.STRATEGY_GLOBAL_COLOUR,
+             "hierarchical": capi.STRATEGY_HIERARCHICAL}[strategy]
+    dev = [DeviceArray.from_host(d) for d in p.point_data]
+    out = DeviceArray.from_host(p.point_data[0])
+    flags = capi.LOOP_STAGE_INPUTS if stage else 0
+    loop = Loop(capi.functor(capi.KERNEL_RES_CALC), p.maps, p.num_points, dev, out,
+                strategy=strat, block_size=block_size,
+                partition=p.partition_vector if strat == capi.STRATEGY_HIERARCHICAL else None,
+                use_graph=graph, flags=flags)
+    return loop, dev, out
+
+
+def time_loop(loop, steps, warmup):
+    from cuda_fot_amd import Timer, synchronize
+    for _ in range(warmup):
+        loop.step()
+    synchronize()
+    t = Timer()
+    synchronize()
+    t0 = time.perf_counter()
+    t.start()
+    for _ in range(steps):
+        loop.step()
+    t.stop()
+    synchronize()
+    wall = time.perf_counter() - t0
+    return wall, t.elapsed_ms() / 1e3
+
+
+def cpu_baseline(mesh, seconds):
+    """The oracle's colour-parallel loop (problem.hpp:405-437 restated in C,
+    OpenMP over all host cores) timed on a bounded number of iterations of
+    the same mesh; iteration-loop time only, colouring excluded."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from cuda_fot_amd import res_calc_problem
+    p = res_calc_problem(mesh)
+    o = O.OracleProblem(O.RES_CALC, p.maps, p.num_points, p.point_data)
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    cores = min(cores, 16)
+    # size the sample from one timed iteration so the leg takes ~`seconds`
+    per = o.loop_colour(1, cores)
+    iters = int(max(1, min(2000, seconds / max(per, 1e-6))))
+    secs = o.loop_colour(iters, cores)
+    serial = O.OracleProblem(O.RES_CALC, p.maps, p.num_points, p.point_data)
+    serial.loop_serial(2)
+    serial_per = serial.last_loop_seconds / 2
+    return {"value": mesh["num_edges"] * iters / secs, "unit": "edges/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{iters} iterations ({secs:.1f} s) of the colour-parallel OpenMP oracle "
+                      f"loop on the same {mesh['nx']}x{mesh['ny']} mesh, iteration loop only",
+            "serial_edges_per_s": mesh["num_edges"] / serial_per}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    from cuda_fot_amd import capi
+    if capi.device_count() < 1:
+        print("no HIP device", file=sys.stderr)
+        sys.exit(1)
+    t_plan = time.perf_counter()
+    mesh, p = build_problem(args, args.partition if args.strategy == "hierarchical" else "none")
+    loop, dev, out = make_loop(p, args.strategy, args.block_size, not args.no_stage,
+                               not args.no_graph)
+    plan_s = time.perf_counter() - t_plan
+    stats = loop.stats()
+    wall, ev = time_loop(loop, args.steps, args.warmup)
+    ne = mesh["num_edges"]
+    ubytes = unique_bytes(mesh["num_nodes"], mesh["num_cells"], ne)
+    achieved = ubytes * args.steps / ev
+    result = {
+        "metric": "res_calc edges/s, achieved HBM GB/s (% of peak); learned-tuned vs plain atomics",
+        "value": ne * args.steps / wall,
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"res_calc airfoil {args.nx}x{args.ny} quad cells, "
+                               f"{ne} interior edges, shuffled, seed 0",
+                   "strategy": args.strategy, "block_size": args.block_size,
+                   "partition": args.partition if args.strategy == "hierarchical" else "none",
+                   "stage_inputs": not args.no_stage, "graph": not args.no_graph,
+                   "launches_per_step": stats["num_launches_per_iter"],
+                   "block_colours": stats["num_colours"],
+                   "avg_thread_colours": round(stats["avg_thread_colours"], 3),
+                   "reuse_factor": round(stats["reuse_factor"], 4),
+                   "plan_seconds": round(plan_s, 2)},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": None,
+                     "algorithmic_bytes_per_step": ubytes,
+                     "event_ms_per_step": ev / args.steps * 1e3},
+    }
+    loop.close()
+    if args.strategy != "atomic" and not args.no_atomic_ref:
+        _, pa = build_problem(args, "none")
+        la, _, _ = make_loop(pa, "atomic", 256, False, not args.no_graph)
+        wa, eva = time_loop(la, args.steps, args.warmup)
+        la.close()
+        result["atomic_edges_per_s"] = ne * args.steps / wa
+        result["speedup_vs_atomic"] = result["value"] / result["atomic_edges_per_s"]
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(mesh, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,49 @@
+// fot_dispatch.hpp -- maps a fot_functor description to the compiled
+// functor type.
+#ifndef FOT_DISPATCH_HPP
+#define FOT_DISPATCH_HPP
+
+#include "fot_functors.hpp"
+#include "fot_internal.hpp"
+
+namespace fot {
+
+// Calls body.template operator()<F, SOA>() for the functor `f` names.
+template <class Body>
+fot_status dispatch(const fot_functor *f, int soa, Body &&body) {
+  if (!f) return fail(FOT_ERR_INVALID_ARGUMENT, "null functor");
+#define FOT_SOA(...)                                                                   \
+  return soa ? body.template operator()<__VA_ARGS__, true>()                          \
+             : body.template operator()<__VA_ARGS__, false>()
+  switch (f->kernel) {
+  case FOT_KERNEL_RES_CALC:
+    FOT_SOA(ResCalcF);
+  case FOT_KERNEL_HEX_SCATTER:
+    FOT_SOA(HexScatterF);
+  case FOT_KERNEL_MINE: {
+#define FOT_MINE_PC(MD_, P_)                                                           \
+  if (f->mesh_dim == MD_ && f->point_dim == P_) {                                      \
+    if (f->cell_dim == 1) { FOT_SOA(MineF<MD_, P_, 1>); }                              \
+    if (f->cell_dim == P_) { FOT_SOA(MineF<MD_, P_, P_>); }                            \
+  }
+    FOT_MINE_PC(2, 1)
+    FOT_MINE_PC(2, 2)
+    FOT_MINE_PC(2, 4)
+    FOT_MINE_PC(2, 8)
+    FOT_MINE_PC(4, 1)
+    FOT_MINE_PC(4, 4)
+    FOT_MINE_PC(8, 1)
+    FOT_MINE_PC(8, 4)
+#undef FOT_MINE_PC
+    return fail(FOT_ERR_UNSUPPORTED, "mine: MeshDim %u PointDim %u CellDim %u not built",
+                f->mesh_dim, f->point_dim, f->cell_dim);
+  }
+  default:
+    return fail(FOT_ERR_UNSUPPORTED, "kernel %u is not built", f->kernel);
+  }
+#undef FOT_SOA
+}
+
+} // namespace fot
+
+#endif

@@ -1,0 +1,81 @@
+// fot_internal.hpp -- shared internals of libfot (not installed).
+#ifndef FOT_INTERNAL_HPP
+#define FOT_INTERNAL_HPP
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "fot_capi.h"
+
+namespace fot {
+
+// Records the message of the last failing call on this thread and returns
+// the status, so call sites read `return fail(FOT_ERR_..., "...")`.
+fot_status fail(fot_status status, const char *fmt, ...);
+
+// 128-bit colour set, the width of the reference's Mesh::colourset_t
+// (mesh.hpp:26).
+struct Colours {
+  uint64_t w[2] = {0, 0};
+  bool test(unsigned i) const { return (w[i >> 6] >> (i & 63)) & 1u; }
+  void set(unsigned i) { w[i >> 6] |= uint64_t(1) << (i & 63); }
+  Colours operator|(const Colours &o) const {
+    Colours r;
+    r.w[0] = w[0] | o.w[0];
+    r.w[1] = w[1] | o.w[1];
+    return r;
+  }
+  Colours &operator|=(const Colours &o) {
+    w[0] |= o.w[0];
+    w[1] |= o.w[1];
+    return *this;
+  }
+  // ~occupied & used
+  static Colours available(const Colours &used, const Colours &occupied) {
+    Colours r;
+    r.w[0] = used.w[0] & ~occupied.w[0];
+    r.w[1] = used.w[1] & ~occupied.w[1];
+    return r;
+  }
+  bool none() const { return (w[0] | w[1]) == 0; }
+  // lowest set bit, 128 if none
+  unsigned first() const {
+    if (w[0]) return unsigned(__builtin_ctzll(w[0]));
+    if (w[1]) return 64u + unsigned(__builtin_ctzll(w[1]));
+    return 128u;
+  }
+};
+
+// Host side of the hierarchical plan, shaped like the reference's
+// HierarchicalColourMemory::MemoryOfOneColour (colouring.hpp:24-41).
+struct HostColour {
+  std::vector<uint32_t> points_to_be_cached;
+  std::vector<uint32_t> points_to_be_cached_offsets{0};
+  std::vector<uint32_t> cell_list[FOT_MAX_MAPPINGS]; // SOA once built
+  std::vector<uint8_t> cell_colours;
+  std::vector<uint8_t> num_cell_colours;
+  std::vector<uint32_t> block_offsets{0};
+  std::vector<uint32_t> cell_inv_permutation;
+  uint32_t shared_size = 0;
+};
+
+} // namespace fot
+
+struct fot_hplan {
+  uint32_t num_mappings = 0;
+  uint32_t mesh_dim[FOT_MAX_MAPPINGS] = {};
+  uint32_t num_cells = 0;
+  std::vector<fot::HostColour> colours;
+};
+
+namespace fot {
+fot_status build_hplan(const uint32_t *const *c2n, const uint32_t *mesh_dim,
+                       uint32_t num_mappings, uint32_t num_cells,
+                       uint32_t num_points0, const uint32_t *partition,
+                       uint32_t block_size, fot_hplan &out);
+} // namespace fot
+
+#endif

@@ -1,0 +1,189 @@
+// fot_meshgen.cpp -- synthetic meshes and data for the loop engine.
+// All randomness is SplitMix64 (Steele, Lea, Flood 2014); each stream is
+// seeded from (seed, stream id) so outputs do not depend on call order.
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "fot_internal.hpp"
+
+using fot::fail;
+
+namespace {
+
+struct SplitMix64 {
+  uint64_t s;
+  explicit SplitMix64(uint64_t seed) : s(seed) {}
+  uint64_t next() {
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return double(next() >> 11) * 0x1.0p-53; } // [0,1)
+  // Box-Muller, one normal per pair of uniforms
+  double normal() {
+    const double u1 = 1.0 - uniform(); // (0,1]
+    const double u2 = uniform();
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+  }
+  uint32_t below(uint32_t n) { return uint32_t((uint64_t(uint32_t(next() >> 32)) * n) >> 32); }
+};
+
+uint64_t stream_seed(uint64_t seed, uint64_t stream) {
+  SplitMix64 g(seed ^ (0x632be59bd9b4e019ull * (stream + 1)));
+  return g.next();
+}
+
+void permutation(uint32_t n, uint64_t seed, uint32_t *perm) {
+  std::iota(perm, perm + n, 0u);
+  SplitMix64 g(seed);
+  for (uint32_t i = n; i > 1; --i) {
+    const uint32_t j = g.below(i);
+    std::swap(perm[i - 1], perm[j]);
+  }
+}
+
+} // namespace
+
+extern "C" fot_status fot_gen_permutation(uint32_t n, uint64_t seed, uint32_t *perm) {
+  if (!perm && n) return fail(FOT_ERR_INVALID_ARGUMENT, "null perm");
+  permutation(n, seed, perm);
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_gen_uniform(uint64_t n, uint64_t seed, double lo, double hi,
+                                      double *out) {
+  if (!out && n) return fail(FOT_ERR_INVALID_ARGUMENT, "null out");
+  SplitMix64 g(seed);
+  for (uint64_t i = 0; i < n; ++i) out[i] = lo + (hi - lo) * g.uniform();
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_gen_normal(uint64_t n, uint64_t seed, double mean, double sd,
+                                     double *out) {
+  if (!out && n) return fail(FOT_ERR_INVALID_ARGUMENT, "null out");
+  SplitMix64 g(seed);
+  for (uint64_t i = 0; i < n; ++i) out[i] = mean + sd * g.normal();
+  return FOT_OK;
+}
+
+// Grid::fillEdgeList (grid.hpp:77-104): for each row but the last, a
+// (down, right) edge pair per column but the last, then the last column's
+// down edge (listed lower point first); finally the last row's right edges.
+extern "C" fot_status fot_gen_grid_edges(uint32_t N, uint32_t M, uint32_t *e) {
+  if (!e || N < 2 || M < 2) return fail(FOT_ERR_INVALID_ARGUMENT, "grid needs N,M >= 2");
+  size_t k = 0;
+  uint32_t upper = 0, lower = M;
+  for (uint32_t r = 0; r + 1 < N; ++r) {
+    for (uint32_t c = 0; c + 1 < M; ++c) {
+      e[k++] = upper;
+      e[k++] = lower;
+      e[k++] = upper;
+      e[k++] = ++upper;
+      ++lower;
+    }
+    e[k++] = lower++;
+    e[k++] = upper++;
+  }
+  for (uint32_t c = 0; c + 1 < M; ++c) {
+    e[k++] = upper;
+    e[k++] = ++upper;
+  }
+  return FOT_OK;
+}
+
+// Grid::fillCellList3D (grid.hpp:415-460): corner order (0,0,0) (0,1,0)
+// (1,1,0) (1,0,0) then the same on the layer above.
+extern "C" fot_status fot_gen_hex_grid(uint32_t N1, uint32_t N2, uint32_t N3,
+                                       uint32_t *cells) {
+  if (!cells || N1 < 2 || N2 < 2 || N3 < 2)
+    return fail(FOT_ERR_INVALID_ARGUMENT, "hex grid needs N >= 2");
+  size_t k = 0;
+  for (uint32_t l = 0; l + 1 < N3; ++l)
+    for (uint32_t r = 0; r + 1 < N1; ++r)
+      for (uint32_t c = 0; c + 1 < N2; ++c) {
+        const uint32_t bot = l * N1 * N2, top = (l + 1) * N1 * N2;
+        const uint32_t in = r * N2 + c, out = (r + 1) * N2 + c;
+        cells[k++] = bot + in;
+        cells[k++] = bot + in + 1;
+        cells[k++] = bot + out + 1;
+        cells[k++] = bot + out;
+        cells[k++] = top + in;
+        cells[k++] = top + in + 1;
+        cells[k++] = top + out + 1;
+        cells[k++] = top + out;
+      }
+  return FOT_OK;
+}
+
+// Airfoil-like structured quad mesh with interior edges only.  Edge order
+// before shuffling follows the cells row by row: for cell (i, j) the edge
+// below it (j > 0), then the edge left of it (i > 0).
+extern "C" fot_status fot_gen_airfoil(uint32_t nx, uint32_t ny, uint64_t seed,
+                                      uint32_t shuffle, uint32_t *pedge,
+                                      uint32_t *pecell, double *x, double *q,
+                                      double *adt) {
+  if (nx < 2 || ny < 2) return fail(FOT_ERR_INVALID_ARGUMENT, "airfoil needs nx,ny >= 2");
+  const uint32_t nn = (nx + 1) * (ny + 1), nc = nx * ny;
+  const uint32_t ne = nx * (ny - 1) + (nx - 1) * ny;
+  std::vector<uint32_t> node_perm(nn), cell_perm(nc), edge_perm(ne);
+  if (shuffle & 1u) permutation(nn, stream_seed(seed, 1), node_perm.data());
+  else std::iota(node_perm.begin(), node_perm.end(), 0u);
+  if (shuffle & 2u) permutation(nc, stream_seed(seed, 2), cell_perm.data());
+  else std::iota(cell_perm.begin(), cell_perm.end(), 0u);
+  if (shuffle & 4u) permutation(ne, stream_seed(seed, 3), edge_perm.data());
+  else std::iota(edge_perm.begin(), edge_perm.end(), 0u);
+  auto node = [&](uint32_t i, uint32_t j) { return node_perm[j * (nx + 1) + i]; };
+  auto cell = [&](uint32_t i, uint32_t j) { return cell_perm[j * nx + i]; };
+  uint32_t k = 0;
+  for (uint32_t j = 0; j < ny; ++j)
+    for (uint32_t i = 0; i < nx; ++i) {
+      if (j > 0) { // horizontal edge, normal +y: left node first
+        const uint32_t e = edge_perm[k++];
+        if (pedge) {
+          pedge[2 * e] = node(i, j);
+          pedge[2 * e + 1] = node(i + 1, j);
+        }
+        if (pecell) {
+          pecell[2 * e] = cell(i, j - 1);
+          pecell[2 * e + 1] = cell(i, j);
+        }
+      }
+      if (i > 0) { // vertical edge, normal +x: upper node first
+        const uint32_t e = edge_perm[k++];
+        if (pedge) {
+          pedge[2 * e] = node(i, j + 1);
+          pedge[2 * e + 1] = node(i, j);
+        }
+        if (pecell) {
+          pecell[2 * e] = cell(i - 1, j);
+          pecell[2 * e + 1] = cell(i, j);
+        }
+      }
+    }
+  if (x) {
+    SplitMix64 g(stream_seed(seed, 4));
+    for (uint32_t j = 0; j <= ny; ++j)
+      for (uint32_t i = 0; i <= nx; ++i) {
+        const uint32_t n = node(i, j);
+        const double dx = -0.2 + 0.4 * g.uniform();
+        const double dy = -0.2 + 0.4 * g.uniform();
+        x[2 * n] = double(i) + dx;
+        x[2 * n + 1] = double(j) + dy;
+      }
+  }
+  if (q) {
+    static const double freestream[4] = {1.0, 0.4, 0.0, 2.5};
+    SplitMix64 g(stream_seed(seed, 5));
+    for (uint32_t c = 0; c < nc; ++c) {
+      const uint32_t cp = cell_perm[c];
+      for (int d = 0; d < 4; ++d) q[4 * size_t(cp) + d] = freestream[d] * (1.0 + 1e-2 * g.normal());
+    }
+  }
+  if (adt) {
+    SplitMix64 g(stream_seed(seed, 6));
+    for (uint32_t c = 0; c < nc; ++c) adt[cell_perm[c]] = 0.5 + g.uniform();
+  }
+  return FOT_OK;
+}

@@ -1,0 +1,205 @@
+// fot_runtime.hip -- status codes, device memory, streams, timers and the
+// out -> in copy of libfot.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+#include "fot_internal.hpp"
+
+namespace {
+thread_local std::string g_message;
+}
+
+namespace fot {
+fot_status fail(fot_status status, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_message = buf;
+  return status;
+}
+} // namespace fot
+
+using fot::fail;
+
+#define FOT_HIP(call)                                                          \
+  do {                                                                         \
+    const hipError_t e_ = (call);                                              \
+    if (e_ != hipSuccess)                                                      \
+      return fail(e_ == hipErrorOutOfMemory ? FOT_ERR_OUT_OF_MEMORY            \
+                                            : FOT_ERR_HIP,                     \
+                  "%s: %s", #call, hipGetErrorString(e_));                     \
+  } while (0)
+
+extern "C" const char *fot_status_string(fot_status s) {
+  switch (s) {
+  case FOT_OK: return "ok";
+  case FOT_ERR_INVALID_ARGUMENT: return "invalid argument";
+  case FOT_ERR_HIP: return "HIP runtime error";
+  case FOT_ERR_OUT_OF_MEMORY: return "out of memory";
+  case FOT_ERR_COLOUR_OVERFLOW: return "colour overflow";
+  case FOT_ERR_UNSUPPORTED: return "unsupported";
+  case FOT_ERR_INVALID_INPUT: return "invalid input";
+  case FOT_ERR_NO_DEVICE: return "no device";
+  case FOT_ERR_TIMEOUT: return "device wait timed out";
+  }
+  return "unknown status";
+}
+
+extern "C" const char *fot_last_error_message(void) { return g_message.c_str(); }
+extern "C" int fot_version(void) { return 1; }
+
+extern "C" fot_status fot_device_count(int *count) {
+  if (!count) return fail(FOT_ERR_INVALID_ARGUMENT, "null count");
+  *count = 0;
+  const hipError_t e = hipGetDeviceCount(count);
+  if (e != hipSuccess) {
+    *count = 0;
+    return fail(FOT_ERR_NO_DEVICE, "hipGetDeviceCount: %s", hipGetErrorString(e));
+  }
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_set_device(int device) {
+  FOT_HIP(hipSetDevice(device));
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_malloc(void **p, size_t bytes) {
+  if (!p) return fail(FOT_ERR_INVALID_ARGUMENT, "null pointer");
+  *p = nullptr;
+  if (bytes == 0) return FOT_OK;
+  FOT_HIP(hipMalloc(p, bytes));
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_free(void *p) {
+  if (p) FOT_HIP(hipFree(p));
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_memcpy_htod(void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return FOT_OK;
+  if (!dst || !src) return fail(FOT_ERR_INVALID_ARGUMENT, "null pointer");
+  FOT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return FOT_OK;
+  if (!dst || !src) return fail(FOT_ERR_INVALID_ARGUMENT, "null pointer");
+  FOT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_memcpy_dtod(void *dst, const void *src, size_t bytes,
+                                      void *stream) {
+  if (bytes == 0) return FOT_OK;
+  if (!dst || !src) return fail(FOT_ERR_INVALID_ARGUMENT, "null pointer");
+  FOT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice,
+                         static_cast<hipStream_t>(stream)));
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_device_synchronize(void) {
+  FOT_HIP(hipDeviceSynchronize());
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_stream_create(void **stream) {
+  if (!stream) return fail(FOT_ERR_INVALID_ARGUMENT, "null stream");
+  hipStream_t s;
+  // default priority: streams of other priorities are not used anywhere
+  FOT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_stream_destroy(void *stream) {
+  if (stream) FOT_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_stream_synchronize(void *stream) {
+  FOT_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return FOT_OK;
+}
+
+// copyKernel (data_t.cu:76-93): 16-byte vectors, a scalar tail.
+__global__ __launch_bounds__(256) void fot_copy_kernel(const uint4 *__restrict__ src,
+                                                       uint4 *__restrict__ dst,
+                                                       size_t n16,
+                                                       const unsigned char *__restrict__ tail_src,
+                                                       unsigned char *__restrict__ tail_dst,
+                                                       unsigned tail) {
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n16; i += stride)
+    dst[i] = src[i];
+  if (blockIdx.x == 0 && threadIdx.x < tail) tail_dst[threadIdx.x] = tail_src[threadIdx.x];
+}
+
+extern "C" fot_status fot_copy(void *dst, const void *src, size_t bytes, void *stream) {
+  if (bytes == 0) return FOT_OK;
+  if (!dst || !src) return fail(FOT_ERR_INVALID_ARGUMENT, "null pointer");
+  if ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 15u)
+    return fot_memcpy_dtod(dst, src, bytes, stream);
+  const size_t n16 = bytes / 16;
+  const unsigned tail = unsigned(bytes % 16);
+  const unsigned blocks = unsigned(std::min<size_t>((n16 + 255) / 256 + 1, 8192));
+  hipLaunchKernelGGL(fot_copy_kernel, dim3(blocks), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), static_cast<const uint4 *>(src),
+                     static_cast<uint4 *>(dst), n16,
+                     static_cast<const unsigned char *>(src) + n16 * 16,
+                     static_cast<unsigned char *>(dst) + n16 * 16, tail);
+  FOT_HIP(hipGetLastError());
+  return FOT_OK;
+}
+
+struct fot_timer {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+
+extern "C" fot_status fot_timer_create(fot_timer **t) {
+  if (!t) return fail(FOT_ERR_INVALID_ARGUMENT, "null timer");
+  fot_timer *p = new fot_timer;
+  hipError_t e = hipEventCreate(&p->start);
+  if (e == hipSuccess) e = hipEventCreate(&p->stop);
+  if (e != hipSuccess) {
+    if (p->start) (void)hipEventDestroy(p->start);
+    delete p;
+    return fail(FOT_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(e));
+  }
+  *t = p;
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_timer_destroy(fot_timer *t) {
+  if (!t) return FOT_OK;
+  (void)hipEventDestroy(t->start);
+  (void)hipEventDestroy(t->stop);
+  delete t;
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_timer_start(fot_timer *t, void *stream) {
+  if (!t) return fail(FOT_ERR_INVALID_ARGUMENT, "null timer");
+  FOT_HIP(hipEventRecord(t->start, static_cast<hipStream_t>(stream)));
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_timer_stop(fot_timer *t, void *stream) {
+  if (!t) return fail(FOT_ERR_INVALID_ARGUMENT, "null timer");
+  FOT_HIP(hipEventRecord(t->stop, static_cast<hipStream_t>(stream)));
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_timer_elapsed_ms(fot_timer *t, float *ms) {
+  if (!t || !ms) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
+  FOT_HIP(hipEventSynchronize(t->stop));
+  FOT_HIP(hipEventElapsedTime(ms, t->start, t->stop));
+  return FOT_OK;
+}

@@ -1,0 +1,56 @@
+"""Synthetic meshes (native generators in libfot)."""
+from __future__ import annotations
+
+import numpy as np
+
+from .capi import call, ptr
+
+
+def grid_edges(N: int, M: int) -> np.ndarray:
+    """Grid::fillEdgeList (grid.hpp:77-104): (E, 2) u32 edges of an N x M grid."""
+    ne = (N - 1) * M + N * (M - 1)
+    e = np.empty((ne, 2), dtype=np.uint32)
+    call("fot_gen_grid_edges", N, M, ptr(e))
+    return e
+
+
+def hex_grid(N1: int, N2: int, N3: int) -> np.ndarray:
+    """Grid::fillCellList3D (grid.hpp:415-460): (C, 8) u32 hexes."""
+    nc = (N1 - 1) * (N2 - 1) * (N3 - 1)
+    c = np.empty((nc, 8), dtype=np.uint32)
+    call("fot_gen_hex_grid", N1, N2, N3, ptr(c))
+    return c
+
+
+def permutation(n: int, seed: int) -> np.ndarray:
+    p = np.empty(n, dtype=np.uint32)
+    call("fot_gen_permutation", n, seed, ptr(p))
+    return p
+
+
+def uniform(n: int, seed: int, lo: float = 0.0, hi: float = 1.0) -> np.ndarray:
+    out = np.empty(n, dtype=np.float64)
+    call("fot_gen_uniform", n, seed, lo, hi, ptr(out))
+    return out
+
+
+def normal(n: int, seed: int, mean: float = 0.0, sd: float = 1.0) -> np.ndarray:
+    out = np.empty(n, dtype=np.float64)
+    call("fot_gen_normal", n, seed, mean, sd, ptr(out))
+    return out
+
+
+def airfoil(nx: int, ny: int, seed: int = 0, shuffle: int = 7) -> dict:
+    """Airfoil-like quad mesh of nx x ny cells (interior edges only) with the
+    res_calc data: pedge (E,2), pecell (E,2), x (N,2), q (C,4), adt (C,)."""
+    nn, nc = (nx + 1) * (ny + 1), nx * ny
+    ne = nx * (ny - 1) + (nx - 1) * ny
+    pedge = np.empty((ne, 2), np.uint32)
+    pecell = np.empty((ne, 2), np.uint32)
+    x = np.empty((nn, 2), np.float64)
+    q = np.empty((nc, 4), np.float64)
+    adt = np.empty(nc, np.float64)
+    call("fot_gen_airfoil", nx, ny, seed, shuffle, ptr(pedge), ptr(pecell), ptr(x), ptr(q),
+         ptr(adt))
+    return {"pedge": pedge, "pecell": pecell, "x": x, "q": q, "adt": adt,
+            "num_nodes": nn, "num_cells": nc, "num_edges": ne, "nx": nx, "ny": ny}

@@ -1,0 +1,154 @@
+"""Python mirror of the reference's Problem<SOA> (problem.hpp:29-652) over
+libfot: mesh mappings, point and cell data on the host, the reorder /
+partition pre-passes (native planning in libfot), and the GPU loops.
+
+Data layout follows the reference: point data k is (num_points[k], dim)
+AOS or (dim, num_points[k]) SOA, point data 0 is the increment target; cell
+data is SOA (dim, num_cells).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import capi, plan
+from .device import DeviceArray, Loop
+
+DEFAULT_BLOCK_SIZE = 128  # problem.hpp:16
+
+
+def reorder_data(data: np.ndarray, perm: np.ndarray, soa: bool) -> np.ndarray:
+    """reorderData<SOA> (data_t.hpp:117-146): new[perm[i]] = old[i]."""
+    out = np.empty_like(data)
+    if data.ndim == 1:
+        out[perm] = data
+    elif soa:
+        out[:, perm] = data
+    else:
+        out[perm] = data
+    return out
+
+
+def reorder_data_inverse_cells(data: np.ndarray, inv: np.ndarray) -> np.ndarray:
+    """reorderDataInverse<true> on SOA cell data: new cell i = old inv[i]."""
+    return np.ascontiguousarray(data[..., inv])
+
+
+class Problem:
+    def __init__(self, maps, num_points, point_data, cell_data=(), block_size=DEFAULT_BLOCK_SIZE,
+                 soa=False):
+        self.maps = [capi.u32(m).copy() for m in maps]
+        self.num_points = [int(n) for n in num_points]
+        self.point_data = [None if d is None else np.array(d, dtype=np.float64) for d in point_data]
+        self.cell_data = [np.array(c, dtype=np.float64) for c in cell_data]
+        self.block_size = int(block_size)
+        self.soa = bool(soa)
+        self.partition_vector: np.ndarray | None = None
+        self.applied_permutation: np.ndarray | None = None
+
+    @property
+    def num_cells(self) -> int:
+        return self.maps[0].shape[0]
+
+    # ---- pre-passes --------------------------------------------------
+    def reorder(self, ordering: str = "gps", same_mapping=None) -> None:
+        """Problem::reorder (problem.hpp:439-482) with a native bandwidth
+        ordering of mapping 0's point graph (GPS-style or RCM) in place of
+        Scotch.  Mappings listed in same_mapping (default: those identical to
+        mapping 0) take mapping 0's permutation (SameMapping); the others a
+        first-touch renumbering (mesh.hpp:247-270)."""
+        if same_mapping is None:
+            same_mapping = [k for k in range(1, len(self.maps))
+                            if np.array_equal(self.maps[k], self.maps[0])]
+        ind, end = plan.graph_csr(self.maps[0], self.num_points[0])
+        perm = plan.gps_order(ind, end) if ordering == "gps" else plan.rcm_order(ind, end)
+        new0, inv = plan.mesh_reorder_cells(self.maps[0], perm)
+        self.maps[0] = new0
+        for k in range(1, len(self.maps)):
+            self.maps[k] = self.maps[k][inv]
+        self._reorder_point(0, perm)
+        for k in range(1, len(self.maps)):
+            if k in same_mapping:
+                p = perm
+            else:
+                p = plan.point_renumbering_first_touch(self.maps[k], self.num_points[k])
+            self.maps[k] = p[self.maps[k]]
+            self._reorder_point(k, p)
+        self.cell_data = [reorder_data_inverse_cells(c, inv) for c in self.cell_data]
+        self._compose(perm)
+
+    def partition(self, tolerance: float = 1.001) -> None:
+        """Problem::partition (problem.hpp:484-499) with the native k-way
+        partitioner in place of METIS."""
+        self.partition_vector = plan.partition_metis_enh(self.maps, self.num_points,
+                                                         self.block_size, tolerance)
+
+    def partition_ours(self, tolerance: float = 1.001, num_blocks_in_coarse: int = 8) -> None:
+        """Problem::partitionOurs (problem.hpp:501-515)."""
+        self.partition_vector = plan.partition_ours(self.maps, self.num_points, self.block_size,
+                                                    num_blocks_in_coarse, tolerance)
+
+    def reorder_to_partition(self) -> None:
+        """Problem::reorderToPartition (problem.hpp:517-526)."""
+        part, inv = plan.reorder_to_partition(self.partition_vector)
+        self.partition_vector = part
+        self.maps = [m[inv] for m in self.maps]
+        self.cell_data = [reorder_data_inverse_cells(c, inv) for c in self.cell_data]
+
+    def renumber_points(self) -> None:
+        """Problem::renumberPoints (problem.hpp:528-551)."""
+        for k in range(len(self.maps)):
+            perm = plan.point_renumbering_by_partition(self.maps[k], self.num_points[k],
+                                                       self.partition_vector)
+            self.maps[k] = perm[self.maps[k]]
+            self._reorder_point(k, perm)
+            if k == 0:
+                self._compose(perm)
+
+    def _reorder_point(self, k, perm):
+        if self.point_data[k] is not None:
+            self.point_data[k] = reorder_data(self.point_data[k], perm, self.soa)
+
+    def _compose(self, perm):
+        # applied_permutation[i]: current id of original point i
+        if self.applied_permutation is None:
+            self.applied_permutation = perm.astype(np.uint32).copy()
+        else:
+            self.applied_permutation = perm[self.applied_permutation]
+
+    def target_in_original_order(self) -> np.ndarray:
+        """Point data 0 indexed by the original point ids (the mapping the
+        reference's tests apply through applied_permutation, res_calc.cu:153-170)."""
+        d = self.point_data[0]
+        if self.applied_permutation is None:
+            return d.copy()
+        return d[:, self.applied_permutation] if self.soa else d[self.applied_permutation]
+
+    # ---- GPU loops ----------------------------------------------------
+    def gpu_loop(self, func: capi.Functor, strategy: int, iters: int, block_size=None,
+                 use_graph=False, flags=0, partition=True):
+        """Runs `iters` iterations on the GPU and writes point data 0 back."""
+        dev = [None if d is None else DeviceArray.from_host(d) for d in self.point_data]
+        out = DeviceArray(self.point_data[0].shape, np.float64)
+        cdev = [DeviceArray.from_host(c) for c in self.cell_data]
+        part = self.partition_vector if (partition and strategy == capi.STRATEGY_HIERARCHICAL) else None
+        loop = Loop(func, self.maps, self.num_points, dev, out, cdev, soa=self.soa,
+                    strategy=strategy, block_size=block_size or self.block_size,
+                    partition=part, use_graph=use_graph, flags=flags)
+        loop.run(iters)
+        self.point_data[0] = dev[0].download()
+        stats = loop.stats()
+        loop.close()
+        return stats
+
+
+def res_calc_problem(mesh: dict, block_size: int = DEFAULT_BLOCK_SIZE, soa: bool = False,
+                     res0=None) -> Problem:
+    """res_calc as res_calc.cu:10-40 sets it up: mappings res, x, q, adt
+    (edge->cell, edge->node, edge->cell, edge->cell)."""
+    nc, nn = mesh["num_cells"], mesh["num_nodes"]
+    res = np.zeros((nc, 4)) if res0 is None else np.array(res0, dtype=np.float64)
+    x, q, adt = mesh["x"], mesh["q"], mesh["adt"]
+    if soa:
+        res, x, q = res.T.copy(), x.T.copy(), q.T.copy()
+    maps = [mesh["pecell"], mesh["pedge"], mesh["pecell"], mesh["pecell"]]
+    return Problem(maps, [nc, nn, nc, nc], [res, x, q, adt], (), block_size, soa)

@@ -1,0 +1,402 @@
+/*
+ * fot_capi.h -- C-ABI of libfot, the MI355X-native edge-centred
+ * indirect-increment loop engine.
+ *
+ * Plain C: pointers, sizes and status codes only.  Everything here is exported
+ * by cuda_fot_amd/lib/libfot.so.  Each entry point names the reference
+ * interface of bgd54/cuda_fot it replaces (file:line); entry points with no
+ * reference counterpart say so.
+ *
+ * Conventions
+ *  - Every call returns fot_status; FOT_OK is 0.  The reference aborts the
+ *    process on any CUDA error (helper_cuda.h checkCudaErrors -> exit); this
+ *    library never exits: it returns a status and keeps a per-thread message
+ *    (fot_last_error_message).
+ *  - "device" pointers are HIP device memory; "host" pointers are CPU memory.
+ *  - Mapping arrays are u32 (the reference's MY_SIZE, Makefile:75), AOS
+ *    (mesh_dim ids per cell) unless stated.  Point data is FP64, AOS or SOA as
+ *    the `soa` flag says (data_t.hpp:104-115 index<SOA>); cell data is SOA
+ *    with stride num_cells.
+ *  - Streams are hipStream_t passed as void*; NULL is the default stream.
+ */
+#ifndef FOT_CAPI_H
+#define FOT_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FOT_MAX_MAPPINGS 8
+#define FOT_MAX_CELL_DATA 4
+
+typedef enum {
+  FOT_OK = 0,
+  FOT_ERR_INVALID_ARGUMENT = 1,
+  FOT_ERR_HIP = 2,
+  FOT_ERR_OUT_OF_MEMORY = 3,
+  /* more colours than the reference's colour sets hold (mesh.hpp:25-29:
+   * 64 global colours, 128 block / thread colours) */
+  FOT_ERR_COLOUR_OVERFLOW = 4,
+  FOT_ERR_UNSUPPORTED = 5,
+  /* malformed mesh / partition / data stream (mesh.hpp:18-21
+   * InvalidInputFile) */
+  FOT_ERR_INVALID_INPUT = 6,
+  FOT_ERR_NO_DEVICE = 7,
+  /* a bounded device-side wait ran out (work-queue kernels) */
+  FOT_ERR_TIMEOUT = 8
+} fot_status;
+
+/* ------------------------------------------------------------------ */
+/* Library, errors                                                      */
+/* ------------------------------------------------------------------ */
+/* No reference counterpart (the reference exits on error). */
+const char *fot_status_string(fot_status status);
+const char *fot_last_error_message(void);
+int fot_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Device memory, streams, timers                                       */
+/* Replace data_t / device_data_t device mirrors (data_t.hpp:10-74,     */
+/* data_t.cu:1-93) and CudaTimer (timer.hpp:70-102).                    */
+/* ------------------------------------------------------------------ */
+fot_status fot_device_count(int *count);
+fot_status fot_set_device(int device);
+fot_status fot_malloc(void **device_ptr, size_t bytes);
+fot_status fot_free(void *device_ptr);
+fot_status fot_memcpy_htod(void *device_dst, const void *host_src, size_t bytes);
+fot_status fot_memcpy_dtoh(void *host_dst, const void *device_src, size_t bytes);
+fot_status fot_memcpy_dtod(void *device_dst, const void *device_src, size_t bytes,
+                           void *stream);
+fot_status fot_device_synchronize(void);
+fot_status fot_stream_create(void **stream);
+fot_status fot_stream_destroy(void *stream);
+fot_status fot_stream_synchronize(void *stream);
+/* copyKernel (data_t.cu:76-93): out -> in after each iteration, in
+ * 16-byte vectors. */
+fot_status fot_copy(void *device_dst, const void *device_src, size_t bytes,
+                    void *stream);
+
+typedef struct fot_timer fot_timer;
+fot_status fot_timer_create(fot_timer **timer);
+fot_status fot_timer_destroy(fot_timer *timer);
+fot_status fot_timer_start(fot_timer *timer, void *stream);
+fot_status fot_timer_stop(fot_timer *timer, void *stream);
+/* waits for the stop event, returns start->stop in milliseconds */
+fot_status fot_timer_elapsed_ms(fot_timer *timer, float *ms);
+
+/* ------------------------------------------------------------------ */
+/* Kernel functors                                                       */
+/* ------------------------------------------------------------------ */
+typedef enum {
+  /* Airfoil edge flux, kernels/res_calc.hpp + res_calc_func.hpp.
+   * mappings: 0 res (dim 4, target), 1 x (dim 2), 2 q (dim 4), 3 adt (1) */
+  FOT_KERNEL_RES_CALC = 0,
+  /* synthetic increment, kernels/mine.hpp + mine_func.hpp; mesh_dim 2/4/8,
+   * point_dim = PointDim, cell_dim = CellDim (1 or PointDim) */
+  FOT_KERNEL_MINE = 1,
+  /* scatter-only hex functor behind the lulesh struct API
+   * (kernels/lulesh.hpp:7-10 shapes): 8 nodes, dim 3, 24 corner forces as
+   * cell data [i * 8 + j] */
+  FOT_KERNEL_HEX_SCATTER = 2,
+  /* face flux, kernels/mini_aero.hpp + mini_aero_func.hpp: mapping 0 flux
+   * (dim 5, target), mapping 1 cell data (dim 28), 12 face doubles */
+  FOT_KERNEL_MINI_AERO = 3,
+  /* shallow-water edge increment in FP64, kernels/volna.hpp +
+   * volna_func.hpp */
+  FOT_KERNEL_VOLNA = 4
+} fot_kernel;
+
+typedef struct {
+  uint32_t kernel;    /* fot_kernel */
+  uint32_t mesh_dim;  /* MeshDim of mapping 0 */
+  uint32_t point_dim; /* dim of the increment target */
+  uint32_t cell_dim;  /* dim of cell data 0 (mine: 1 or point_dim) */
+} fot_functor;
+
+/* Fills the functor description of a kernel with its fixed shapes
+ * (mine: pass mesh_dim/point_dim/cell_dim; others ignore them). */
+fot_status fot_functor_init(fot_functor *f, fot_kernel kernel, uint32_t mesh_dim,
+                            uint32_t point_dim, uint32_t cell_dim);
+
+/*
+ * One launch of the global-colouring step over the cells of one colour.
+ * Replaces <app>::StepGPUGlobal::call<SOA> (kernels/res_calc.hpp:164-177,
+ * kernels/mine.hpp:197-212, kernels/lulesh.hpp:71-83,
+ * kernels/mini_aero.hpp:124-137).  point_data, cell_data, cell_to_node and
+ * point_stride are DEVICE arrays (of device pointers / sizes), exactly as the
+ * reference's Problem::loopGPUCellCentred builds them (problem.hpp:158-176).
+ * cell_to_node[k] is the colour's compacted AOS list; cell data SOA with
+ * stride cell_stride.  Increments are added with plain read-modify-write.
+ */
+fot_status fot_step_gpu_global(const fot_functor *f, int soa,
+                               const void *const *point_data, void *point_data_out,
+                               void *const *cell_data,
+                               const uint32_t *const *cell_to_node,
+                               uint32_t num_cells, const uint32_t *point_stride,
+                               uint32_t cell_stride, uint32_t num_blocks,
+                               uint32_t block_size, void *stream);
+
+/*
+ * Same arguments, over ALL cells in one launch, increments added with FP64
+ * global atomics (hardware global atomic add, 64-lane coalesced map reads).
+ * No reference counterpart: the reference has no atomics; this is the launch
+ * behind Problem::loopGPUEdgeCentred.
+ */
+fot_status fot_step_gpu_atomic(const fot_functor *f, int soa,
+                               const void *const *point_data, void *point_data_out,
+                               void *const *cell_data,
+                               const uint32_t *const *cell_to_node,
+                               uint32_t num_cells, const uint32_t *point_stride,
+                               uint32_t cell_stride, uint32_t num_blocks,
+                               uint32_t block_size, void *stream);
+
+/*
+ * One launch of the hierarchical step over the blocks of one block colour.
+ * Replaces <app>::StepGPUHierarchical::call<SOA> (kernels/res_calc.hpp:257-276,
+ * kernels/mine.hpp:460-481, kernels/lulesh.hpp:140-160,
+ * kernels/mini_aero.hpp:225-245).  Arguments as the reference's
+ * Problem::loopGPUHierarchical passes them (problem.hpp:289-301): plan arrays
+ * of HierarchicalColourMemory for one colour, cell_to_node[k] SOA with stride
+ * num_cells, cell_to_node[0] holding block-local point indices.  cache_size is
+ * the dynamic LDS size in bytes.
+ */
+fot_status fot_step_gpu_hierarchical(
+    const fot_functor *f, int soa, const void *const *point_data,
+    void *point_data_out, const uint32_t *points_to_be_cached,
+    const uint32_t *points_to_be_cached_offsets, void *const *cell_data,
+    const uint32_t *const *cell_to_node, const uint8_t *num_cell_colours,
+    const uint8_t *cell_colours, const uint32_t *block_offsets,
+    uint32_t num_cells, const uint32_t *point_stride, uint32_t cell_stride,
+    uint32_t num_blocks, uint32_t block_size, uint32_t cache_size, void *stream);
+
+/* Reads (and with clear != 0 resets) the flag a hierarchical launch sets
+ * when a block's points do not fit cache_size; such blocks are skipped.  No
+ * reference counterpart (the reference writes past its shared memory). */
+fot_status fot_lds_overflow_flag(uint32_t *flag, int clear);
+
+/* ------------------------------------------------------------------ */
+/* Host planning (pure CPU, no device needed)                            */
+/* ------------------------------------------------------------------ */
+
+/* Mesh::colourCells (mesh.hpp:117-162): greedy colouring on one mapping,
+ * least-used available colour among 64.  colour_out[num_cells].
+ * FOT_ERR_COLOUR_OVERFLOW if 64 colours do not suffice. */
+fot_status fot_colour_cells(const uint32_t *cell_to_node, uint32_t num_cells,
+                            uint32_t mesh_dim, uint32_t num_points,
+                            uint8_t *colour_out, uint32_t *num_colours);
+
+/* colourCellsWithOrdering (colouring.cu:94-102): in-block colouring of
+ * cells given block-local point ids; smallest-last order then first fit. */
+fot_status fot_colour_cells_with_ordering(const uint32_t *mapping,
+                                          uint32_t num_cells,
+                                          uint32_t mapping_dim,
+                                          uint8_t *colours_out,
+                                          uint32_t *num_colours);
+
+/* HierarchicalColourMemory<SOA> (colouring.hpp:22-404), host part. */
+typedef struct fot_hplan fot_hplan;
+fot_status fot_hplan_build(const uint32_t *const *cell_to_node,
+                           const uint32_t *mesh_dim, uint32_t num_mappings,
+                           uint32_t num_cells, uint32_t num_points0,
+                           const uint32_t *partition /* nullable */,
+                           uint32_t block_size, fot_hplan **plan);
+void fot_hplan_free(fot_hplan *plan);
+
+typedef struct {
+  uint32_t num_colours; /* block colours */
+  uint32_t num_blocks;
+  uint32_t num_cells;
+  uint32_t num_cached;  /* sum over blocks of points_to_be_cached */
+  uint32_t max_cached;  /* largest per-block cache */
+  uint32_t max_block;   /* largest block (cells) */
+  uint32_t max_thread_colours;
+  double avg_thread_colours;
+} fot_hplan_info;
+fot_status fot_hplan_info_get(const fot_hplan *plan, fot_hplan_info *info);
+
+/* One block colour of the plan (MemoryOfOneColour, colouring.hpp:24-41).
+ * Pointers stay owned by the plan.  cell_list[k] is SOA with stride
+ * num_cells (cellListSOA, colouring.hpp:300-308); cell_inv_permutation[i] is
+ * the original cell at plan position i of this colour (copyCellWeights,
+ * colouring.hpp:310-335). */
+typedef struct {
+  uint32_t num_blocks, num_cells, num_cached, shared_size;
+  const uint32_t *points_to_be_cached;
+  const uint32_t *points_to_be_cached_offsets; /* num_blocks + 1 */
+  const uint32_t *cell_list[FOT_MAX_MAPPINGS];
+  const uint8_t *cell_colours;                 /* num_cells */
+  const uint8_t *num_cell_colours;             /* num_blocks */
+  const uint32_t *block_offsets;               /* num_blocks + 1 */
+  const uint32_t *cell_inv_permutation;        /* num_cells */
+} fot_hplan_colour;
+fot_status fot_hplan_colour_get(const fot_hplan *plan, uint32_t colour,
+                                fot_hplan_colour *out);
+
+/* Mesh::reorderToPartition (mesh.hpp:227-245): sorts partition in place,
+ * writes the inverse cell permutation (new cell i = old inv_perm[i]). */
+fot_status fot_reorder_to_partition(uint32_t *partition, uint32_t num_cells,
+                                    uint32_t *inv_perm);
+
+/* Mesh::getPointToPartition + getPointRenumberingPermutation2
+ * (mesh.hpp:346-388): point permutation old -> new. */
+fot_status fot_point_renumbering_by_partition(const uint32_t *cell_to_node,
+                                              uint32_t num_cells,
+                                              uint32_t mesh_dim,
+                                              uint32_t num_points,
+                                              const uint32_t *partition,
+                                              uint32_t *perm);
+
+/* Mesh::getPointRenumberingPermutation (mesh.hpp:247-270), first touch. */
+fot_status fot_point_renumbering_first_touch(const uint32_t *cell_to_node,
+                                             uint32_t num_cells,
+                                             uint32_t mesh_dim,
+                                             uint32_t num_points,
+                                             uint32_t *perm);
+
+/* Mesh::reorder (mesh.hpp:199-225), mapping-0 part: renumber with
+ * point_perm, sort cells by their sorted point tuple.  Rewrites
+ * cell_to_node0 and returns the inverse cell permutation. */
+fot_status fot_mesh_reorder_cells(uint32_t *cell_to_node0, uint32_t num_cells,
+                                  uint32_t mesh_dim, const uint32_t *point_perm,
+                                  uint32_t *inv_perm);
+
+/* GraphCSR<T>(mesh) (reorder.hpp:39-83): point-to-point CSR.  endpoints
+ * holds num_cells*mesh_dim*(mesh_dim-1) entries; *num_arcs gets the count. */
+fot_status fot_graph_csr(const uint32_t *cell_to_node, uint32_t num_cells,
+                         uint32_t mesh_dim, uint32_t num_points,
+                         uint32_t *indices, uint32_t *endpoints,
+                         uint32_t *num_arcs);
+
+/* Mesh::getCellToCellGraph (mesh.hpp:317-344) as a CSR over cells with
+ * ascending neighbour rows (the GraphCSR of the pair mesh).  Two calls:
+ * endpoints == NULL returns *num_arcs only. */
+fot_status fot_cell_to_cell_csr(const uint32_t *const *cell_to_node,
+                                const uint32_t *mesh_dim, uint32_t num_mappings,
+                                uint32_t num_cells, const uint32_t *num_points,
+                                uint32_t *indices, uint32_t *endpoints,
+                                uint64_t *num_arcs);
+
+/* HeuristicalPartition (partition.cu:128-275) given a coarse partition of
+ * the cell-to-cell CSR.  result[num_cells] block ids; *num_blocks. */
+fot_status fot_heuristic_partition(const uint32_t *csr_indices,
+                                   const uint32_t *csr_endpoints,
+                                   uint32_t num_vertices,
+                                   const uint32_t *coarse_partition,
+                                   uint32_t block_size, uint32_t *result,
+                                   uint32_t *num_blocks);
+
+/* k-way partition of a CSR graph into parts of at most max_part vertices
+ * (stand-in for METIS_PartGraphKway, partition.cu:11-31, which is absent:
+ * greedy graph growing from a BFS-ordered seed sequence).  No reference
+ * counterpart for the algorithm; the interface follows partitionMetisEnh
+ * (partition.hpp:23-29): part sizes <= floor(block_size / tolerance). */
+fot_status fot_kway_partition(const uint32_t *csr_indices,
+                              const uint32_t *csr_endpoints,
+                              uint32_t num_vertices, uint32_t max_part,
+                              uint32_t *part, uint32_t *num_parts);
+
+/* Bandwidth-reducing point orderings of a CSR graph, permutation old->new.
+ * Stand-ins for Scotch strategy "g" (reorder.hpp:135-175, absent here) and
+ * scipy's reverse_cuthill_mckee (scripts/rcm.py:16-37). */
+fot_status fot_rcm_order(const uint32_t *csr_indices, const uint32_t *csr_endpoints,
+                         uint32_t num_vertices, uint32_t *perm);
+fot_status fot_gps_order(const uint32_t *csr_indices, const uint32_t *csr_endpoints,
+                         uint32_t num_vertices, uint32_t *perm);
+
+/* ------------------------------------------------------------------ */
+/* Synthetic mesh generators (no reference counterpart except where     */
+/* noted).  All randomness comes from SplitMix64(seed).                 */
+/* ------------------------------------------------------------------ */
+
+/* Grid::fillEdgeList (grid.hpp:77-104): N x M points, right and down edges;
+ * edges[2*((N-1)*M + N*(M-1))]. */
+fot_status fot_gen_grid_edges(uint32_t N, uint32_t M, uint32_t *edges);
+/* Grid::fillCellList3D (grid.hpp:415-460): N1 x N2 x N3 points, hexes with
+ * 8 corners each; cells[8*(N1-1)*(N2-1)*(N3-1)]. */
+fot_status fot_gen_hex_grid(uint32_t N1, uint32_t N2, uint32_t N3, uint32_t *cells);
+
+/* Airfoil-like quad mesh of nx x ny cells: (nx+1)(ny+1) nodes perturbed by
+ * U(-0.2,0.2), interior edges only, pedge/pecell oriented so that (dy,-dx)
+ * points from cell 1 to cell 2.  shuffle: bit 0 nodes, bit 1 cells, bit 2
+ * edges.  Sizes: num_nodes=(nx+1)(ny+1), num_cells=nx*ny,
+ * num_edges=nx*(ny-1)+(nx-1)*ny.  Any output may be NULL.  q and adt are
+ * AOS (q: 4 per cell). */
+fot_status fot_gen_airfoil(uint32_t nx, uint32_t ny, uint64_t seed,
+                           uint32_t shuffle, uint32_t *pedge, uint32_t *pecell,
+                           double *x, double *q, double *adt);
+
+/* Random permutation (Fisher-Yates on SplitMix64). */
+fot_status fot_gen_permutation(uint32_t n, uint64_t seed, uint32_t *perm);
+/* n doubles uniform in [lo, hi) / normal (mean, sd) (Box-Muller). */
+fot_status fot_gen_uniform(uint64_t n, uint64_t seed, double lo, double hi,
+                           double *out);
+fot_status fot_gen_normal(uint64_t n, uint64_t seed, double mean, double sd,
+                          double *out);
+
+/* ------------------------------------------------------------------ */
+/* Loop engine (MI355X-native fast path)                                 */
+/* Replaces Problem<SOA>::loopGPUCellCentred / loopGPUHierarchical      */
+/* (problem.hpp:89-339) and adds loopGPUEdgeCentred (atomics).           */
+/* ------------------------------------------------------------------ */
+typedef enum {
+  FOT_STRATEGY_ATOMIC = 0,        /* one launch, FP64 global atomics      */
+  FOT_STRATEGY_GLOBAL_COLOUR = 1, /* one launch per global colour        */
+  FOT_STRATEGY_HIERARCHICAL = 2   /* LDS blocks, thread + block colours  */
+} fot_strategy;
+
+typedef struct {
+  uint32_t num_cells;
+  uint32_t num_mappings;
+  const uint32_t *cell_to_node[FOT_MAX_MAPPINGS]; /* HOST, AOS */
+  uint32_t mesh_dim[FOT_MAX_MAPPINGS];
+  uint32_t num_points[FOT_MAX_MAPPINGS];
+  const double *point_data[FOT_MAX_MAPPINGS];   /* DEVICE; [0] = iterate */
+  uint32_t point_dim[FOT_MAX_MAPPINGS];
+  const double *cell_data[FOT_MAX_CELL_DATA];   /* DEVICE, SOA */
+  uint32_t cell_dim[FOT_MAX_CELL_DATA];
+  uint32_t num_cell_data;
+  double *point_data_out;                       /* DEVICE accumulation */
+  int soa;
+} fot_problem_desc;
+
+typedef struct {
+  uint32_t strategy;      /* fot_strategy */
+  uint32_t block_size;    /* threads (atomic/global) or cells per block */
+  const uint32_t *partition; /* HOST, nullable: block id per cell */
+  uint32_t use_graph;     /* capture one iteration in a hipGraph */
+  uint32_t flags;         /* FOT_LOOP_* */
+} fot_loop_config;
+
+/* hierarchical: stage mapping-0 input point data in LDS as well */
+#define FOT_LOOP_STAGE_INPUTS 1u
+/* hierarchical: single launch, blocks ordered by per-block dependency
+ * flags instead of one launch per block colour */
+#define FOT_LOOP_DEPENDENCY_ORDER 2u
+
+typedef struct fot_loop fot_loop;
+/* Builds the plan (host), uploads it, permutes cell data. */
+fot_status fot_loop_create(const fot_functor *f, const fot_problem_desc *desc,
+                           const fot_loop_config *cfg, fot_loop **loop);
+fot_status fot_loop_destroy(fot_loop *loop);
+/* One iteration's increments into point_data_out (no copy back). */
+fot_status fot_loop_step(fot_loop *loop, void *stream);
+/* `iters` iterations with the reference's semantics: out starts as a copy of
+ * point_data[0]; after each iteration point_data[0] = out. */
+fot_status fot_loop_run(fot_loop *loop, uint32_t iters, void *stream);
+
+typedef struct {
+  uint32_t strategy, num_launches_per_iter, num_colours, num_blocks;
+  uint32_t block_size, max_cached, lds_bytes, max_thread_colours;
+  uint64_t plan_device_bytes;
+  double avg_thread_colours, reuse_factor;
+} fot_loop_stats;
+fot_status fot_loop_stats_get(const fot_loop *loop, fot_loop_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FOT_CAPI_H */

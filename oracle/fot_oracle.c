@@ -1,0 +1,938 @@
+/*
+ * fot_oracle.c -- CPU oracle for the edge-centred indirect-increment loop.
+ *
+ * TEST INFRASTRUCTURE ONLY (see fot_oracle.h).  Built with -ffp-contract=off
+ * so every increment is the IEEE result of the reference's expression order.
+ * The reference cannot be built in this image (it needs cuda.h, METIS and
+ * Scotch), so this restatement is pinned by the reference's own known-answer
+ * tests, by invariants and by closed forms (tests/test_oracle_*.py).
+ */
+#include "fot_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static double now_seconds(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* ------------------------------------------------------------------------ */
+/* 128-bit colour sets: Mesh::colourset_t = std::bitset<128> (mesh.hpp:26)  */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  uint64_t w[2];
+} cset;
+
+static inline int cset_test(cset s, unsigned i) {
+  return (int)((s.w[i >> 6] >> (i & 63)) & 1u);
+}
+static inline void cset_set(cset *s, unsigned i) {
+  s->w[i >> 6] |= (uint64_t)1 << (i & 63);
+}
+static inline cset cset_or(cset a, cset b) {
+  cset r = {{a.w[0] | b.w[0], a.w[1] | b.w[1]}};
+  return r;
+}
+static inline cset cset_andnot(cset used, cset occ) { /* ~occ & used */
+  cset r = {{~occ.w[0] & used.w[0], ~occ.w[1] & used.w[1]}};
+  return r;
+}
+static inline int cset_none(cset a) { return (a.w[0] | a.w[1]) == 0; }
+static inline cset cset_shl1_set0(cset a) { /* (a << 1) | 1 */
+  cset r;
+  r.w[1] = (a.w[1] << 1) | (a.w[0] >> 63);
+  r.w[0] = (a.w[0] << 1) | 1u;
+  return r;
+}
+
+/* Mesh::getAvailableColour<true> (mesh.hpp:287-315): the available colour
+ * with the smallest set size; the first such index wins (strict '>'). */
+static unsigned avail_min_size(cset avail, const uint32_t *set_sizes,
+                               unsigned n) {
+  unsigned colour = n;
+  for (unsigned i = 0; i < n; ++i) {
+    if (cset_test(avail, i)) {
+      if (colour >= n || set_sizes[colour] > set_sizes[i]) colour = i;
+    }
+  }
+  return colour;
+}
+/* Mesh::getAvailableColour<false>: the first available colour. */
+static unsigned avail_first(cset avail) {
+  for (unsigned i = 0; i < 128; ++i)
+    if (cset_test(avail, i)) return i;
+  return 128;
+}
+
+/* ------------------------------------------------------------------------ */
+/* User functions                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* kernels/res_calc_func.hpp:11-57, same expression order.  The reference's
+ * 0.5f and 1.0f literals are exact in double. */
+void oracle_res_calc_func(const double *x1, const double *x2, const double *q1,
+                          const double *q2, const double *adt1,
+                          const double *adt2, double *res1, double *res2,
+                          unsigned x_stride, unsigned q_stride) {
+  const double gm1 = 0.4, eps = 0.05;
+  double dx, dy, mu, ri, p1, vol1, p2, vol2, f;
+  dx = x1[0 * x_stride] - x2[0 * x_stride];
+  dy = x1[1 * x_stride] - x2[1 * x_stride];
+  ri = 1.0 / q1[0];
+  p1 = gm1 * (q1[3 * q_stride] -
+              0.5 * ri *
+                  (q1[1 * q_stride] * q1[1 * q_stride] +
+                   q1[2 * q_stride] * q1[2 * q_stride]));
+  vol1 = ri * (q1[1 * q_stride] * dy - q1[2 * q_stride] * dx);
+  ri = 1.0 / q2[0];
+  p2 = gm1 * (q2[3 * q_stride] -
+              0.5 * ri *
+                  (q2[1 * q_stride] * q2[1 * q_stride] +
+                   q2[2 * q_stride] * q2[2 * q_stride]));
+  vol2 = ri * (q2[1 * q_stride] * dy - q2[2 * q_stride] * dx);
+  mu = 0.5 * ((*adt1) + (*adt2)) * eps;
+  f = 0.5 * (vol1 * q1[0] + vol2 * q2[0]) + mu * (q1[0] - q2[0]);
+  res1[0] = f;
+  res2[0] = -f;
+  f = 0.5 * (vol1 * q1[1 * q_stride] + p1 * dy + vol2 * q2[1 * q_stride] +
+             p2 * dy) +
+      mu * (q1[1 * q_stride] - q2[1 * q_stride]);
+  res1[1] = f;
+  res2[1] = -f;
+  f = 0.5 * (vol1 * q1[2 * q_stride] - p1 * dx + vol2 * q2[2 * q_stride] -
+             p2 * dx) +
+      mu * (q1[2 * q_stride] - q2[2 * q_stride]);
+  res1[2] = f;
+  res2[2] = -f;
+  f = 0.5 * (vol1 * (q1[3 * q_stride] + p1) + vol2 * (q2[3 * q_stride] + p2)) +
+      mu * (q1[3 * q_stride] - q2[3 * q_stride]);
+  res1[3] = f;
+  res2[3] = -f;
+}
+
+/* Index of element (point i, component d) of a point set (data_t.hpp:104-115). */
+static inline size_t pidx(int soa, uint32_t n, uint32_t i, uint32_t dim,
+                          uint32_t d) {
+  return soa ? (size_t)d * n + i : (size_t)i * dim + d;
+}
+
+/* Per-kernel increment: inc[slot * tdim + d] for the mesh_dim[0] target
+ * slots of cell `ind`, reading the iteration's input point data. */
+static void compute_inc(const oracle_problem *p, uint32_t ind, double *inc) {
+  const int soa = p->soa;
+  switch (p->kernel) {
+  case ORACLE_RES_CALC: {
+    /* res_calc::StepSeq (kernels/res_calc.hpp:29-89): x through mapping 1,
+     * q, adt and res through mapping 0. */
+    const uint32_t *m0 = p->cell_to_node[0], *m1 = p->cell_to_node[1];
+    const double *x = p->point_data[1], *q = p->point_data[2],
+                 *adt = p->point_data[3];
+    const uint32_t ux = soa ? 1 : 2, uq = soa ? 1 : 4;
+    const unsigned xs = soa ? p->num_points[1] : 1;
+    const unsigned qs = soa ? p->num_points[2] : 1;
+    oracle_res_calc_func(x + (size_t)ux * m1[2 * ind + 0],
+                         x + (size_t)ux * m1[2 * ind + 1],
+                         q + (size_t)uq * m0[2 * ind + 0],
+                         q + (size_t)uq * m0[2 * ind + 1],
+                         adt + m0[2 * ind + 0], adt + m0[2 * ind + 1], inc,
+                         inc + 4, xs, qs);
+    break;
+  }
+  case ORACLE_MINE2:
+  case ORACLE_MINE4:
+  case ORACLE_MINE8: {
+    /* mine::StepSeq<MeshDim> (kernels/mine.hpp:22-178) with mine_func
+     * (kernels/mine_func.hpp:11-99). */
+    const uint32_t md = p->mesh_dim[0], P = p->point_dim[0];
+    const uint32_t C = p->cell_dim[0];
+    const uint32_t *m0 = p->cell_to_node[0];
+    const double *in = p->point_data[0];
+    const double *w = p->cell_data[0] + ind;
+    const uint32_t n = p->num_points[0], cs = p->num_cells;
+    const double *pt[8];
+    for (uint32_t k = 0; k < md; ++k)
+      pt[k] = in + (size_t)(soa ? 1 : P) * m0[md * ind + k];
+    const uint32_t ps = soa ? n : 1;
+    for (uint32_t d = 0; d < P; ++d) {
+      const uint32_t cd = (C == P) ? d : 0;
+      const double wd = w[(size_t)cd * cs];
+      if (md == 2) {
+        inc[0 * P + d] = 2 * pt[1][d * ps] * wd;
+        inc[1 * P + d] = 2 * pt[0][d * ps] * wd;
+      } else {
+        for (uint32_t k = 0; k < md; ++k) {
+          const uint32_t a = (k + 1) % md, b = (k + md - 1) % md;
+          inc[k * P + d] = (pt[a][d * ps] + pt[b][d * ps]) * wd;
+        }
+      }
+    }
+    break;
+  }
+  case ORACLE_HEX_SCATTER: {
+    /* Scatter-only functor behind the lulesh struct API: the 24 corner
+     * forces of an element are cell data laid out as the reference's
+     * f_out[i * MESH_DIM + j] (kernels/lulesh.hpp:52-56). */
+    const double *F = p->cell_data[0] + ind;
+    const uint32_t cs = p->num_cells;
+    for (uint32_t j = 0; j < 8; ++j)
+      for (uint32_t i = 0; i < 3; ++i)
+        inc[j * 3 + i] = F[(size_t)(i * 8 + j) * cs];
+    break;
+  }
+  default:
+    break;
+  }
+}
+
+/* Adds the increments of cell `ind` into `out` in the reference's order;
+ * abs (nullable) accumulates |inc|. */
+static void add_inc(const oracle_problem *p, uint32_t ind, const double *inc,
+                    double *out, double *abs_acc) {
+  const uint32_t md = p->mesh_dim[0], P = p->point_dim[0];
+  const uint32_t n = p->num_points[0];
+  const uint32_t *m0 = p->cell_to_node[0];
+  if (p->kernel == ORACLE_HEX_SCATTER) { /* slot outer (lulesh.hpp:52-56) */
+    for (uint32_t k = 0; k < md; ++k)
+      for (uint32_t d = 0; d < P; ++d) {
+        const size_t g = pidx(p->soa, n, m0[md * ind + k], P, d);
+        out[g] += inc[k * P + d];
+        if (abs_acc) abs_acc[g] += fabs(inc[k * P + d]);
+      }
+  } else { /* component outer, slot inner (res_calc.hpp:144-147) */
+    for (uint32_t d = 0; d < P; ++d)
+      for (uint32_t k = 0; k < md; ++k) {
+        const size_t g = pidx(p->soa, n, m0[md * ind + k], P, d);
+        out[g] += inc[k * P + d];
+        if (abs_acc) abs_acc[g] += fabs(inc[k * P + d]);
+      }
+  }
+}
+
+static size_t target_len(const oracle_problem *p) {
+  return (size_t)p->num_points[0] * p->point_dim[0];
+}
+
+/* problem.hpp:362-380 (serial loop, cells in index order). */
+int oracle_loop_serial(const oracle_problem *p, uint32_t iters,
+                       double *abs_sum, double *loop_seconds) {
+  const size_t len = target_len(p);
+  double *temp = (double *)malloc(len * sizeof(double));
+  double inc[64];
+  if (!temp) return -1;
+  memcpy(temp, p->point_data[0], len * sizeof(double));
+  if (abs_sum)
+    for (size_t i = 0; i < len; ++i) abs_sum[i] = fabs(temp[i]);
+  const double t0 = now_seconds();
+  for (uint32_t it = 0; it < iters; ++it) {
+    for (uint32_t c = 0; c < p->num_cells; ++c) {
+      compute_inc(p, c, inc);
+      add_inc(p, c, inc, temp, abs_sum);
+    }
+    memcpy(p->point_data[0], temp, len * sizeof(double));
+  }
+  if (loop_seconds) *loop_seconds = now_seconds() - t0;
+  free(temp);
+  return 0;
+}
+
+/* problem.hpp:382-437 (colour-major loop; OpenMP inside a colour). */
+int oracle_loop_colour(const oracle_problem *p, uint32_t iters, int nthreads,
+                       double *loop_seconds) {
+  const size_t len = target_len(p);
+  const uint32_t nc = p->num_cells;
+  uint8_t *col = (uint8_t *)malloc(nc ? nc : 1);
+  uint32_t *order = (uint32_t *)malloc(sizeof(uint32_t) * (nc ? nc : 1));
+  uint32_t off[66] = {0};
+  double *temp = (double *)malloc(len * sizeof(double));
+  int ncol;
+  if (!col || !order || !temp) return -1;
+  ncol = oracle_colour_cells(p->cell_to_node[0], nc, p->mesh_dim[0],
+                             p->num_points[0], col);
+  if (ncol < 0) {
+    free(col), free(order), free(temp);
+    return -1;
+  }
+  for (uint32_t c = 0; c < nc; ++c) off[col[c] + 1]++;
+  for (int k = 0; k < ncol; ++k) off[k + 1] += off[k];
+  {
+    uint32_t fill[65];
+    memcpy(fill, off, sizeof(fill));
+    for (uint32_t c = 0; c < nc; ++c) order[fill[col[c]]++] = c;
+  }
+  memcpy(temp, p->point_data[0], len * sizeof(double));
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  /* timed like the reference: the iteration loop only (problem.hpp:410-436) */
+  const double t0 = now_seconds();
+  for (uint32_t it = 0; it < iters; ++it) {
+    for (int k = 0; k < ncol; ++k) {
+      const int64_t b = off[k], e = off[k + 1];
+#pragma omp parallel for schedule(static) if (nthreads > 1)
+      for (int64_t i = b; i < e; ++i) {
+        double inc[64];
+        compute_inc(p, order[i], inc);
+        add_inc(p, order[i], inc, temp, NULL);
+      }
+    }
+    memcpy(p->point_data[0], temp, len * sizeof(double));
+  }
+  if (loop_seconds) *loop_seconds = now_seconds() - t0;
+  free(col), free(order), free(temp);
+  return 0;
+}
+
+/* Hierarchical summation order (res_calc.hpp:377-442): zeroed cache,
+ * thread colours in order, then out += cache, blocks in plan order. */
+int oracle_loop_hier_replay(const oracle_problem *p, const oracle_hplan *h,
+                            uint32_t iters) {
+  const size_t len = target_len(p);
+  const uint32_t md = p->mesh_dim[0], P = p->point_dim[0];
+  const uint32_t n = p->num_points[0];
+  double *temp = (double *)malloc(len * sizeof(double));
+  uint32_t maxc = 0;
+  double *cache;
+  double inc[64];
+  if (!temp) return -1;
+  for (uint32_t c = 0; c < h->num_colours; ++c) {
+    const uint32_t bb = h->colour_block_offset[c] + c;
+    const uint32_t nb = h->colour_block_offset[c + 1] - h->colour_block_offset[c];
+    for (uint32_t b = 0; b < nb; ++b) {
+      const uint32_t m = h->points_to_be_cached_offsets[bb + b + 1] -
+                         h->points_to_be_cached_offsets[bb + b];
+      if (m > maxc) maxc = m;
+    }
+  }
+  cache = (double *)malloc(sizeof(double) * (size_t)(maxc ? maxc : 1) * P);
+  if (!cache) {
+    free(temp);
+    return -1;
+  }
+  memcpy(temp, p->point_data[0], len * sizeof(double));
+  for (uint32_t it = 0; it < iters; ++it) {
+    uint32_t blk = 0;
+    for (uint32_t c = 0; c < h->num_colours; ++c) {
+      const uint32_t c0 = h->colour_cell_offset[c];
+      const uint32_t ncell = h->colour_cell_offset[c + 1] - c0;
+      const uint32_t bb = h->colour_block_offset[c] + c; /* nb_c+1 entries */
+      const uint32_t nb = h->colour_block_offset[c + 1] - h->colour_block_offset[c];
+      const uint32_t cache0 = h->colour_cache_offset[c];
+      for (uint32_t b = 0; b < nb; ++b, ++blk) {
+        const uint32_t from = h->block_offsets[bb + b];
+        const uint32_t to = h->block_offsets[bb + b + 1];
+        const uint32_t po = h->points_to_be_cached_offsets[bb + b];
+        const uint32_t m = h->points_to_be_cached_offsets[bb + b + 1] - po;
+        const uint32_t ntc = h->num_cell_colours[blk];
+        memset(cache, 0, sizeof(double) * (size_t)m * P);
+        for (uint32_t tc = 0; tc < ntc; ++tc) {
+          for (uint32_t t = from; t < to; ++t) {
+            const uint32_t pos = c0 + t;
+            if (h->cell_colours[pos] != tc) continue;
+            const uint32_t cell = h->cell_inv_permutation[pos];
+            compute_inc(p, cell, inc);
+            if (p->kernel == ORACLE_HEX_SCATTER) {
+              for (uint32_t k = 0; k < md; ++k)
+                for (uint32_t d = 0; d < P; ++d) {
+                  const uint32_t l = h->cell_list[0][(size_t)md * c0 +
+                                                     (size_t)k * ncell + t];
+                  cache[(size_t)d * m + l] += inc[k * P + d];
+                }
+            } else {
+              for (uint32_t d = 0; d < P; ++d)
+                for (uint32_t k = 0; k < md; ++k) {
+                  const uint32_t l = h->cell_list[0][(size_t)md * c0 +
+                                                     (size_t)k * ncell + t];
+                  cache[(size_t)d * m + l] += inc[k * P + d];
+                }
+            }
+          }
+        }
+        for (uint32_t i = 0; i < m; ++i) {
+          const uint32_t g = h->points_to_be_cached[cache0 + po + i];
+          for (uint32_t d = 0; d < P; ++d) {
+            const size_t gi = pidx(p->soa, n, g, P, d);
+            temp[gi] = temp[gi] + cache[(size_t)d * m + i];
+          }
+        }
+      }
+    }
+    memcpy(p->point_data[0], temp, len * sizeof(double));
+  }
+  free(cache);
+  free(temp);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Planning                                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* mesh.hpp:117-162 */
+int oracle_colour_cells(const uint32_t *c2n, uint32_t num_cells,
+                        uint32_t mesh_dim, uint32_t num_points,
+                        uint8_t *colour_out) {
+  cset *pc = (cset *)calloc(num_points ? num_points : 1, sizeof(cset));
+  uint32_t set_sizes[64] = {0};
+  cset used = {{0, 0}};
+  int ncol = 0;
+  if (!pc) return -1;
+  for (uint32_t i = 0; i < num_cells; ++i) {
+    cset occ = {{0, 0}}, avail;
+    unsigned colour;
+    for (uint32_t j = 0; j < mesh_dim; ++j)
+      occ = cset_or(occ, pc[c2n[(size_t)mesh_dim * i + j]]);
+    avail = cset_andnot(used, occ);
+    if (cset_none(avail)) {
+      used = cset_shl1_set0(used);
+      ++ncol;
+      avail = cset_andnot(used, occ);
+    }
+    colour = avail_min_size(avail, set_sizes, 64);
+    if (colour >= 64 || ncol > 64) {
+      free(pc);
+      return -1; /* the reference warns "not enough colour" and overruns */
+    }
+    colour_out[i] = (uint8_t)colour;
+    for (uint32_t j = 0; j < mesh_dim; ++j)
+      cset_set(&pc[c2n[(size_t)mesh_dim * i + j]], colour);
+    ++set_sizes[colour];
+  }
+  free(pc);
+  return ncol;
+}
+
+/* getCellToCell (partition.cu:281-301) as CSR: for each cell, for each slot,
+ * the other cells on that point in ascending order (std::multimap equal_range
+ * keeps insertion order); duplicates kept. */
+static void cell_to_cell_local(const uint32_t *mapping, uint32_t nc,
+                               uint32_t md, uint32_t **adj_off_out,
+                               uint32_t **adj_out) {
+  const uint32_t len = nc * md;
+  uint32_t maxp = 0;
+  for (uint32_t k = 0; k < len; ++k)
+    if (mapping[k] + 1 > maxp) maxp = mapping[k] + 1;
+  uint32_t *poff = (uint32_t *)calloc(maxp + 1, sizeof(uint32_t));
+  uint32_t *pcell = (uint32_t *)malloc(sizeof(uint32_t) * (len ? len : 1));
+  uint32_t *fill = (uint32_t *)malloc(sizeof(uint32_t) * (maxp + 1));
+  for (uint32_t k = 0; k < len; ++k) poff[mapping[k] + 1]++;
+  for (uint32_t q = 0; q < maxp; ++q) poff[q + 1] += poff[q];
+  memcpy(fill, poff, sizeof(uint32_t) * (maxp + 1));
+  for (uint32_t k = 0; k < len; ++k) pcell[fill[mapping[k]]++] = k / md;
+  uint32_t *aoff = (uint32_t *)calloc(nc + 1, sizeof(uint32_t));
+  for (uint32_t i = 0; i < nc; ++i) {
+    uint32_t cnt = 0;
+    for (uint32_t o = 0; o < md; ++o) {
+      const uint32_t pt = mapping[md * i + o];
+      for (uint32_t r = poff[pt]; r < poff[pt + 1]; ++r)
+        if (pcell[r] != i) ++cnt;
+    }
+    aoff[i + 1] = aoff[i] + cnt;
+  }
+  uint32_t *adj = (uint32_t *)malloc(sizeof(uint32_t) * (aoff[nc] ? aoff[nc] : 1));
+  for (uint32_t i = 0; i < nc; ++i) {
+    uint32_t w = aoff[i];
+    for (uint32_t o = 0; o < md; ++o) {
+      const uint32_t pt = mapping[md * i + o];
+      for (uint32_t r = poff[pt]; r < poff[pt + 1]; ++r)
+        if (pcell[r] != i) adj[w++] = pcell[r];
+    }
+  }
+  free(poff), free(pcell), free(fill);
+  *adj_off_out = aoff;
+  *adj_out = adj;
+}
+
+/* colouring.cu:12-102 */
+int oracle_colour_cells_with_ordering(const uint32_t *mapping,
+                                      uint32_t num_cells, uint32_t mapping_dim,
+                                      uint8_t *colours_out) {
+  uint32_t *aoff, *adj;
+  const uint32_t nc = num_cells;
+  if (nc == 0) return 0;
+  cell_to_cell_local(mapping, nc, mapping_dim, &aoff, &adj);
+  /* getCellOrder: smallest-last, ties -> last index (colouring.cu:20-32) */
+  uint32_t *deg = (uint32_t *)malloc(sizeof(uint32_t) * nc);
+  uint8_t *done = (uint8_t *)calloc(nc, 1);
+  uint32_t *order = (uint32_t *)malloc(sizeof(uint32_t) * nc);
+  for (uint32_t i = 0; i < nc; ++i) deg[i] = aoff[i + 1] - aoff[i];
+  for (uint32_t it = 0; it < nc; ++it) {
+    uint32_t mi = nc;
+    for (uint32_t j = 0; j < nc; ++j) {
+      if (done[j]) continue;
+      if (mi == nc || deg[mi] >= deg[j]) mi = j;
+    }
+    done[mi] = 1;
+    order[it] = mi;
+    /* each occurrence of a neighbour decrements its degree once */
+    for (uint32_t r = aoff[mi]; r < aoff[mi + 1]; ++r) --deg[adj[r]];
+  }
+  /* colourCellToCell: reverse order, first-fit (colouring.cu:63-92) */
+  cset *cc = (cset *)calloc(nc, sizeof(cset));
+  cset used = {{0, 0}};
+  int ncol = 0;
+  for (uint32_t i = nc; i-- > 0;) {
+    const uint32_t v = order[i];
+    cset occ = {{0, 0}}, avail;
+    for (uint32_t r = aoff[v]; r < aoff[v + 1]; ++r) occ = cset_or(occ, cc[adj[r]]);
+    avail = cset_andnot(used, occ);
+    if (cset_none(avail)) {
+      if (ncol >= 128) {
+        ncol = -1;
+        break;
+      }
+      cset_set(&used, (unsigned)ncol++);
+      avail = cset_andnot(used, occ);
+    }
+    const unsigned colour = avail_first(avail);
+    colours_out[v] = (uint8_t)colour;
+    cset_set(&cc[v], colour);
+  }
+  free(deg), free(done), free(order), free(cc), free(aoff), free(adj);
+  return ncol;
+}
+
+typedef struct {
+  uint32_t point, local_cell, slot;
+} pt_ref;
+
+static int cmp_ptref(const void *a, const void *b) {
+  const pt_ref *x = (const pt_ref *)a, *y = (const pt_ref *)b;
+  if (x->point != y->point) return x->point < y->point ? -1 : 1;
+  if (x->local_cell != y->local_cell) return x->local_cell < y->local_cell ? -1 : 1;
+  if (x->slot != y->slot) return x->slot < y->slot ? -1 : 1;
+  return 0;
+}
+
+typedef struct {
+  uint64_t key; /* partition << 32 | cell */
+} pc_pair;
+static int cmp_u64(const void *a, const void *b) {
+  const uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* HierarchicalColourMemory ctor (colouring.hpp:44-147), colourBlock
+ * (:169-235), sortCellsByColours (:246-272), cellListSOA (:300-308). */
+int oracle_hplan_build(const uint32_t *const *c2n, const uint32_t *mesh_dim,
+                       uint32_t num_mappings, uint32_t num_cells,
+                       uint32_t num_points0, const uint32_t *partition,
+                       uint32_t block_size, oracle_hplan *out) {
+  const uint32_t nc = num_cells, md0 = mesh_dim[0];
+  uint32_t *pos_cell = (uint32_t *)malloc(sizeof(uint32_t) * nc); /* cwip */
+  uint32_t *blocks = (uint32_t *)malloc(sizeof(uint32_t) * (nc + 2));
+  uint32_t nblk = 0;
+  memset(out, 0, sizeof(*out));
+  if (nc == 0 || block_size == 0) return -1;
+  if (partition) {
+    uint64_t *pairs = (uint64_t *)malloc(sizeof(uint64_t) * nc);
+    for (uint32_t i = 0; i < nc; ++i)
+      pairs[i] = ((uint64_t)partition[i] << 32) | i;
+    qsort(pairs, nc, sizeof(uint64_t), cmp_u64);
+    uint32_t tid = 0;
+    blocks[nblk++] = 0;
+    pos_cell[0] = (uint32_t)pairs[0];
+    for (uint32_t i = 1; i < nc; ++i) {
+      pos_cell[i] = (uint32_t)pairs[i];
+      if (++tid == block_size) {
+        tid = 0;
+        blocks[nblk++] = i;
+        continue;
+      }
+      if ((pairs[i - 1] >> 32) != (pairs[i] >> 32)) {
+        tid = 0;
+        blocks[nblk++] = i;
+      }
+    }
+    blocks[nblk++] = nc;
+    free(pairs);
+  } else {
+    for (uint32_t i = 0; i < nc; i += block_size) blocks[nblk++] = i;
+    blocks[nblk++] = nc;
+    for (uint32_t i = 0; i < nc; ++i) pos_cell[i] = i;
+  }
+  const uint32_t nb = nblk - 1;
+  /* pass 1: block colours, thread colouring, sorting (per block) */
+  cset *pcol = (cset *)calloc(num_points0 ? num_points0 : 1, sizeof(cset));
+  cset used = {{0, 0}};
+  uint32_t set_sizes[128] = {0};
+  uint32_t ncol = 0;
+  uint8_t *bcol = (uint8_t *)malloc(nb ? nb : 1);
+  uint8_t *tcol = (uint8_t *)malloc(nc);        /* by position */
+  uint8_t *ntc = (uint8_t *)malloc(nb ? nb : 1);
+  uint32_t *local = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)nc * md0);
+  uint32_t *nuniq = (uint32_t *)malloc(sizeof(uint32_t) * (nb ? nb : 1));
+  uint32_t *uniq = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)nc * md0);
+  pt_ref *refs = (pt_ref *)malloc(sizeof(pt_ref) * (size_t)block_size * md0);
+  uint32_t *tmp_local = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)block_size * md0);
+  uint8_t *tmp_col = (uint8_t *)malloc(block_size);
+  uint32_t *perm = (uint32_t *)malloc(sizeof(uint32_t) * block_size);
+  uint32_t *tmp_pc = (uint32_t *)malloc(sizeof(uint32_t) * block_size);
+  uint32_t uniq_total = 0;
+  int err = 0;
+  for (uint32_t b = 0; b < nb && !err; ++b) {
+    const uint32_t from = blocks[b], to = blocks[b + 1], bs = to - from;
+    cset occ = {{0, 0}}, avail;
+    if (bs > block_size) {
+      err = 1;
+      break;
+    }
+    for (uint32_t i = from; i < to; ++i)
+      for (uint32_t j = 0; j < md0; ++j)
+        occ = cset_or(occ, pcol[c2n[0][(size_t)md0 * pos_cell[i] + j]]);
+    avail = cset_andnot(used, occ);
+    if (cset_none(avail)) {
+      if (ncol >= 128) {
+        err = 1;
+        break;
+      }
+      used = cset_shl1_set0(used);
+      set_sizes[ncol++] = 0;
+      avail = cset_andnot(used, occ);
+    }
+    const unsigned colour = avail_min_size(avail, set_sizes, ncol);
+    ++set_sizes[colour];
+    bcol[b] = (uint8_t)colour;
+    /* colourBlock: colour written points, sorted unique points, local ids */
+    uint32_t nr = 0;
+    for (uint32_t i = from; i < to; ++i)
+      for (uint32_t j = 0; j < md0; ++j) {
+        const uint32_t pt = c2n[0][(size_t)md0 * pos_cell[i] + j];
+        cset_set(&pcol[pt], colour);
+        refs[nr].point = pt;
+        refs[nr].local_cell = i - from;
+        refs[nr].slot = j;
+        ++nr;
+      }
+    qsort(refs, nr, sizeof(pt_ref), cmp_ptref);
+    uint32_t nu = 0;
+    for (uint32_t r = 0; r < nr; ++r) {
+      if (r == 0 || refs[r].point != refs[r - 1].point) uniq[uniq_total + nu++] = refs[r].point;
+      tmp_local[md0 * refs[r].local_cell + refs[r].slot] = nu - 1;
+    }
+    nuniq[b] = nu;
+    uniq_total += nu;
+    int nt = oracle_colour_cells_with_ordering(tmp_local, bs, md0, tmp_col);
+    if (nt < 0) {
+      err = 1;
+      break;
+    }
+    ntc[b] = (uint8_t)nt;
+    /* sortCellsByColours: stable by (colour, local index) */
+    {
+      uint32_t cnt[257] = {0};
+      for (uint32_t i = 0; i < bs; ++i) cnt[tmp_col[i] + 1]++;
+      for (int k = 0; k < 256; ++k) cnt[k + 1] += cnt[k];
+      for (uint32_t i = 0; i < bs; ++i) perm[cnt[tmp_col[i]]++] = i;
+    }
+    for (uint32_t i = 0; i < bs; ++i) {
+      tcol[from + i] = tmp_col[perm[i]];
+      tmp_pc[i] = pos_cell[from + perm[i]];
+      for (uint32_t j = 0; j < md0; ++j)
+        local[(size_t)md0 * (from + i) + j] = tmp_local[md0 * perm[i] + j];
+    }
+    memcpy(pos_cell + from, tmp_pc, sizeof(uint32_t) * bs);
+  }
+  free(refs), free(tmp_local), free(tmp_col), free(perm), free(tmp_pc), free(pcol);
+  if (err) {
+    free(pos_cell), free(blocks), free(bcol), free(tcol), free(ntc), free(local);
+    free(nuniq), free(uniq);
+    return -1;
+  }
+  /* pass 2: group by block colour (blocks keep their order within a colour) */
+  out->num_colours = ncol;
+  out->num_mappings = num_mappings;
+  for (uint32_t m = 0; m < num_mappings; ++m) out->mesh_dim[m] = mesh_dim[m];
+  out->num_cells = nc;
+  out->num_blocks = nb;
+  out->num_cached = uniq_total;
+  out->colour_cell_offset = (uint32_t *)calloc(ncol + 1, sizeof(uint32_t));
+  out->colour_block_offset = (uint32_t *)calloc(ncol + 1, sizeof(uint32_t));
+  out->colour_cache_offset = (uint32_t *)calloc(ncol + 1, sizeof(uint32_t));
+  uint32_t *uniq_off = (uint32_t *)malloc(sizeof(uint32_t) * (nb + 1));
+  uniq_off[0] = 0;
+  for (uint32_t b = 0; b < nb; ++b) {
+    uniq_off[b + 1] = uniq_off[b] + nuniq[b];
+    out->colour_cell_offset[bcol[b] + 1] += blocks[b + 1] - blocks[b];
+    out->colour_block_offset[bcol[b] + 1] += 1;
+    out->colour_cache_offset[bcol[b] + 1] += nuniq[b];
+  }
+  for (uint32_t c = 0; c < ncol; ++c) {
+    out->colour_cell_offset[c + 1] += out->colour_cell_offset[c];
+    out->colour_block_offset[c + 1] += out->colour_block_offset[c];
+    out->colour_cache_offset[c + 1] += out->colour_cache_offset[c];
+  }
+  for (uint32_t m = 0; m < num_mappings; ++m)
+    out->cell_list[m] = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)nc * mesh_dim[m]);
+  out->cell_colours = (uint8_t *)malloc(nc);
+  out->num_cell_colours = (uint8_t *)malloc(nb);
+  out->block_offsets = (uint32_t *)malloc(sizeof(uint32_t) * (nb + ncol));
+  out->points_to_be_cached = (uint32_t *)malloc(sizeof(uint32_t) * (uniq_total ? uniq_total : 1));
+  out->points_to_be_cached_offsets = (uint32_t *)malloc(sizeof(uint32_t) * (nb + ncol));
+  out->cell_inv_permutation = (uint32_t *)malloc(sizeof(uint32_t) * nc);
+  {
+    uint32_t blk = 0;
+    for (uint32_t c = 0; c < ncol; ++c) {
+      const uint32_t c0 = out->colour_cell_offset[c];
+      const uint32_t ncell = out->colour_cell_offset[c + 1] - c0;
+      const uint32_t bo = out->colour_block_offset[c] + c;
+      uint32_t cell_w = 0, cache_w = 0, k = 0;
+      out->block_offsets[bo] = 0;
+      out->points_to_be_cached_offsets[bo] = 0;
+      for (uint32_t b = 0; b < nb; ++b) {
+        if (bcol[b] != c) continue;
+        const uint32_t from = blocks[b], to = blocks[b + 1];
+        for (uint32_t i = from; i < to; ++i, ++cell_w) {
+          const uint32_t pos = c0 + cell_w;
+          const uint32_t cell = pos_cell[i];
+          out->cell_inv_permutation[pos] = cell;
+          out->cell_colours[pos] = tcol[i];
+          for (uint32_t j = 0; j < md0; ++j)
+            out->cell_list[0][(size_t)md0 * c0 + (size_t)j * ncell + cell_w] =
+                local[(size_t)md0 * i + j];
+          for (uint32_t m = 1; m < num_mappings; ++m) {
+            const uint32_t mdm = mesh_dim[m];
+            for (uint32_t j = 0; j < mdm; ++j)
+              out->cell_list[m][(size_t)mdm * c0 + (size_t)j * ncell + cell_w] =
+                  c2n[m][(size_t)mdm * cell + j];
+          }
+        }
+        memcpy(out->points_to_be_cached + out->colour_cache_offset[c] + cache_w,
+               uniq + uniq_off[b], sizeof(uint32_t) * nuniq[b]);
+        cache_w += nuniq[b];
+        out->num_cell_colours[out->colour_block_offset[c] + k] = ntc[b];
+        ++k;
+        out->block_offsets[bo + k] = cell_w;
+        out->points_to_be_cached_offsets[bo + k] = cache_w;
+        ++blk;
+      }
+    }
+    (void)blk;
+  }
+  free(uniq_off), free(pos_cell), free(blocks), free(bcol), free(tcol), free(ntc);
+  free(local), free(nuniq), free(uniq);
+  return 0;
+}
+
+void oracle_hplan_free(oracle_hplan *p) {
+  if (!p) return;
+  free(p->colour_cell_offset), free(p->colour_block_offset);
+  free(p->colour_cache_offset);
+  for (uint32_t m = 0; m < ORACLE_MAX_MAPPINGS; ++m) free(p->cell_list[m]);
+  free(p->cell_colours), free(p->num_cell_colours), free(p->block_offsets);
+  free(p->points_to_be_cached), free(p->points_to_be_cached_offsets);
+  free(p->cell_inv_permutation);
+  memset(p, 0, sizeof(*p));
+}
+
+/* mesh.hpp:227-245 */
+void oracle_reorder_to_partition(uint32_t *partition, uint32_t num_cells,
+                                 uint32_t *inv_perm_out) {
+  uint64_t *pairs = (uint64_t *)malloc(sizeof(uint64_t) * (num_cells ? num_cells : 1));
+  for (uint32_t i = 0; i < num_cells; ++i)
+    pairs[i] = ((uint64_t)partition[i] << 32) | i;
+  qsort(pairs, num_cells, sizeof(uint64_t), cmp_u64);
+  for (uint32_t i = 0; i < num_cells; ++i) {
+    partition[i] = (uint32_t)(pairs[i] >> 32);
+    inv_perm_out[i] = (uint32_t)pairs[i];
+  }
+  free(pairs);
+}
+
+/* Sorted unique partition ids per point (mesh.hpp:346-363), CSR. */
+typedef struct {
+  const uint32_t *off, *ids;
+} ptpart;
+static const ptpart *g_ptpart; /* qsort has no context argument */
+
+/* Comparator of getPointRenumberingPermutation2 (mesh.hpp:373-385) made a
+ * strict weak order: more partitions first, then lexicographically larger
+ * set first; two untouched points compare equal (the reference's lambda
+ * returns true both ways for them, leaving their order to the library). */
+static int cmp_ptpart(const void *a, const void *b) {
+  const uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+  const uint32_t nx = g_ptpart->off[x + 1] - g_ptpart->off[x];
+  const uint32_t ny = g_ptpart->off[y + 1] - g_ptpart->off[y];
+  if (nx != ny) return nx > ny ? -1 : 1;
+  for (uint32_t k = 0; k < nx; ++k) {
+    const uint32_t u = g_ptpart->ids[g_ptpart->off[x] + k];
+    const uint32_t v = g_ptpart->ids[g_ptpart->off[y] + k];
+    if (u != v) return u > v ? -1 : 1;
+  }
+  return x < y ? -1 : (x > y ? 1 : 0); /* stability */
+}
+
+void oracle_point_renumbering_by_partition(const uint32_t *c2n,
+                                           uint32_t num_cells,
+                                           uint32_t mesh_dim,
+                                           uint32_t num_points,
+                                           const uint32_t *partition,
+                                           uint32_t *perm_out) {
+  const size_t len = (size_t)num_cells * mesh_dim;
+  uint64_t *pairs = (uint64_t *)malloc(sizeof(uint64_t) * (len ? len : 1));
+  uint32_t *off = (uint32_t *)calloc(num_points + 1, sizeof(uint32_t));
+  uint32_t *ids = (uint32_t *)malloc(sizeof(uint32_t) * (len ? len : 1));
+  uint32_t *inv = (uint32_t *)malloc(sizeof(uint32_t) * (num_points ? num_points : 1));
+  size_t n = 0;
+  for (uint32_t i = 0; i < num_cells; ++i)
+    for (uint32_t j = 0; j < mesh_dim; ++j)
+      pairs[n++] = ((uint64_t)c2n[(size_t)mesh_dim * i + j] << 32) | partition[i];
+  qsort(pairs, n, sizeof(uint64_t), cmp_u64);
+  size_t w = 0;
+  for (size_t r = 0; r < n; ++r) {
+    if (r > 0 && pairs[r] == pairs[r - 1]) continue;
+    ids[w++] = (uint32_t)pairs[r];
+    off[(pairs[r] >> 32) + 1]++;
+  }
+  for (uint32_t q = 0; q < num_points; ++q) off[q + 1] += off[q];
+  ptpart pp = {off, ids};
+  g_ptpart = &pp;
+  for (uint32_t q = 0; q < num_points; ++q) inv[q] = q;
+  qsort(inv, num_points, sizeof(uint32_t), cmp_ptpart);
+  for (uint32_t i = 0; i < num_points; ++i) perm_out[inv[i]] = i;
+  free(pairs), free(off), free(ids), free(inv);
+}
+
+/* mesh.hpp:247-270 */
+void oracle_point_renumbering_first_touch(const uint32_t *c2n,
+                                          uint32_t num_cells,
+                                          uint32_t mesh_dim,
+                                          uint32_t num_points,
+                                          uint32_t *perm) {
+  uint32_t next = 0;
+  for (uint32_t q = 0; q < num_points; ++q) perm[q] = num_points;
+  for (size_t k = 0; k < (size_t)num_cells * mesh_dim; ++k)
+    if (perm[c2n[k]] == num_points) perm[c2n[k]] = next++;
+  for (uint32_t q = 0; q < num_points; ++q)
+    if (perm[q] == num_points) perm[q] = next++;
+}
+
+static uint32_t g_md;
+static const uint32_t *g_tuples;
+static int cmp_tuple(const void *a, const void *b) {
+  const uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+  for (uint32_t k = 0; k < g_md; ++k) {
+    const uint32_t u = g_tuples[(size_t)(g_md) * x + k];
+    const uint32_t v = g_tuples[(size_t)(g_md) * y + k];
+    if (u != v) return u < v ? -1 : 1;
+  }
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+static int cmp_u32(const void *a, const void *b) {
+  const uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* mesh.hpp:199-225 (mapping 0 part) */
+void oracle_mesh_reorder_cells(uint32_t *c2n0, uint32_t num_cells,
+                               uint32_t mesh_dim, const uint32_t *point_perm,
+                               uint32_t *inv_perm_out) {
+  const size_t len = (size_t)num_cells * mesh_dim;
+  uint32_t *tuples = (uint32_t *)malloc(sizeof(uint32_t) * (len ? len : 1));
+  uint32_t *old = (uint32_t *)malloc(sizeof(uint32_t) * (len ? len : 1));
+  for (size_t k = 0; k < len; ++k) c2n0[k] = point_perm[c2n0[k]];
+  memcpy(tuples, c2n0, sizeof(uint32_t) * len);
+  for (uint32_t i = 0; i < num_cells; ++i)
+    qsort(tuples + (size_t)mesh_dim * i, mesh_dim, sizeof(uint32_t), cmp_u32);
+  g_md = mesh_dim;
+  g_tuples = tuples;
+  for (uint32_t i = 0; i < num_cells; ++i) inv_perm_out[i] = i;
+  qsort(inv_perm_out, num_cells, sizeof(uint32_t), cmp_tuple);
+  memcpy(old, c2n0, sizeof(uint32_t) * len);
+  for (uint32_t i = 0; i < num_cells; ++i)
+    memcpy(c2n0 + (size_t)mesh_dim * i, old + (size_t)mesh_dim * inv_perm_out[i],
+           sizeof(uint32_t) * mesh_dim);
+  free(tuples), free(old);
+}
+
+/* reorder.hpp:39-83: points in ascending order; per point its incident
+ * cells in ascending order (multimap insertion order); neighbour appended
+ * unless already present in this point's row. */
+uint32_t oracle_graph_csr(const uint32_t *c2n, uint32_t num_cells,
+                          uint32_t mesh_dim, uint32_t num_points,
+                          uint32_t *indices, uint32_t *endpoints) {
+  const size_t len = (size_t)num_cells * mesh_dim;
+  uint32_t *poff = (uint32_t *)calloc(num_points + 1, sizeof(uint32_t));
+  uint32_t *pcell = (uint32_t *)malloc(sizeof(uint32_t) * (len ? len : 1));
+  uint32_t *fill = (uint32_t *)malloc(sizeof(uint32_t) * (num_points + 1));
+  uint32_t arcs = 0;
+  for (size_t k = 0; k < len; ++k) poff[c2n[k] + 1]++;
+  for (uint32_t q = 0; q < num_points; ++q) poff[q + 1] += poff[q];
+  memcpy(fill, poff, sizeof(uint32_t) * (num_points + 1));
+  for (size_t k = 0; k < len; ++k) pcell[fill[c2n[k]]++] = (uint32_t)(k / mesh_dim);
+  indices[0] = 0;
+  for (uint32_t q = 0; q < num_points; ++q) {
+    const uint32_t row = arcs;
+    for (uint32_t r = poff[q]; r < poff[q + 1]; ++r) {
+      const uint32_t cell = pcell[r];
+      for (uint32_t j = 0; j < mesh_dim; ++j) {
+        const uint32_t other = c2n[(size_t)mesh_dim * cell + j];
+        if (other == q) continue;
+        int found = 0;
+        for (uint32_t s = row; s < arcs; ++s)
+          if (endpoints[s] == other) {
+            found = 1;
+            break;
+          }
+        if (!found) endpoints[arcs++] = other;
+      }
+    }
+    indices[q + 1] = arcs;
+  }
+  free(poff), free(pcell), free(fill);
+  return arcs;
+}
+
+/* mesh.hpp:317-344 */
+uint64_t oracle_cell_to_cell_graph(const uint32_t *const *c2n,
+                                   const uint32_t *mesh_dim,
+                                   uint32_t num_mappings, uint32_t num_cells,
+                                   const uint32_t *num_points,
+                                   uint32_t *pairs_out, uint64_t max_pairs) {
+  uint64_t cap = 1024, n = 0;
+  uint64_t *keys = (uint64_t *)malloc(sizeof(uint64_t) * cap);
+  for (uint32_t m = 0; m < num_mappings; ++m) {
+    const uint32_t md = mesh_dim[m], np = num_points[m];
+    const size_t len = (size_t)num_cells * md;
+    uint32_t *poff = (uint32_t *)calloc(np + 1, sizeof(uint32_t));
+    uint32_t *pcell = (uint32_t *)malloc(sizeof(uint32_t) * (len ? len : 1));
+    uint32_t *fill = (uint32_t *)malloc(sizeof(uint32_t) * (np + 1));
+    for (size_t k = 0; k < len; ++k) poff[c2n[m][k] + 1]++;
+    for (uint32_t q = 0; q < np; ++q) poff[q + 1] += poff[q];
+    memcpy(fill, poff, sizeof(uint32_t) * (np + 1));
+    for (size_t k = 0; k < len; ++k) pcell[fill[c2n[m][k]]++] = (uint32_t)(k / md);
+    for (uint32_t i = 0; i < num_cells; ++i)
+      for (uint32_t o = 0; o < md; ++o) {
+        const uint32_t pt = c2n[m][(size_t)md * i + o];
+        for (uint32_t r = poff[pt]; r < poff[pt + 1]; ++r)
+          if (pcell[r] > i) {
+            if (n == cap) {
+              cap *= 2;
+              keys = (uint64_t *)realloc(keys, sizeof(uint64_t) * cap);
+            }
+            keys[n++] = ((uint64_t)i << 32) | pcell[r];
+          }
+      }
+    free(poff), free(pcell), free(fill);
+  }
+  qsort(keys, n, sizeof(uint64_t), cmp_u64);
+  uint64_t w = 0;
+  for (uint64_t r = 0; r < n; ++r) {
+    if (r > 0 && keys[r] == keys[r - 1]) continue;
+    if (w < max_pairs) {
+      pairs_out[2 * w] = (uint32_t)(keys[r] >> 32);
+      pairs_out[2 * w + 1] = (uint32_t)keys[r];
+    }
+    ++w;
+  }
+  free(keys);
+  return w;
+}

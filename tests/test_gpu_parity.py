@@ -1,0 +1,152 @@
+"""GPU parity of the loop engine against the CPU oracle (calls libfot's C-ABI).
+
+Bars: the global-colour and hierarchical GPU loops keep a summation order
+fixed by their plans, and the user functions are compiled without FP
+contraction, so they are bit-identical to the oracle's replay of the same
+order (oracle_loop_colour / oracle_loop_hier_replay).  The atomics loop adds
+in arbitrary order and is held to |gpu - serial| <= 1e-12 * max(|serial|, S)
+with S the oracle's sum of absolute increments.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from cuda_fot_amd import capi
+from helpers import (airfoil_problem, assert_bitwise, assert_close_scaled, hex_problem,
+                     mine_problem, oracle_of)
+
+pytestmark = pytest.mark.gpu
+
+ITERS = 3
+RES_CALC = capi.functor(capi.KERNEL_RES_CALC)
+
+
+def _serial(p, kernel, iters=ITERS):
+    o = oracle_of(p, kernel)
+    S = o.loop_serial(iters, with_scale=True)
+    return o.target.copy(), S
+
+
+def _colour(p, kernel, iters=ITERS):
+    o = oracle_of(p, kernel)
+    o.loop_colour(iters, 1)
+    return o.target.copy()
+
+
+def _replay(p, kernel, block_size, partition=None, iters=ITERS):
+    o = oracle_of(p, kernel)
+    pl = O.OraclePlan(p.maps, p.num_points[0], block_size, partition)
+    o.loop_hier_replay(pl, iters)
+    return o.target.copy()
+
+
+@pytest.mark.parametrize("soa", [False, True])
+def test_res_calc_atomic_vs_serial(gpu, soa):
+    p = airfoil_problem(soa=soa, res0_seed=3)
+    ref, S = _serial(p, O.RES_CALC)
+    p.gpu_loop(RES_CALC, capi.STRATEGY_ATOMIC, ITERS, block_size=256)
+    assert_close_scaled(p.point_data[0], ref, S, what="atomic")
+
+
+@pytest.mark.parametrize("soa", [False, True])
+def test_res_calc_global_colour_bitwise(gpu, soa):
+    p = airfoil_problem(soa=soa, res0_seed=4)
+    ref = _colour(p, O.RES_CALC)
+    p.gpu_loop(RES_CALC, capi.STRATEGY_GLOBAL_COLOUR, ITERS, block_size=128)
+    assert_bitwise(p.point_data[0], ref, "global colour")
+
+
+@pytest.mark.parametrize("soa,bs,flags", [(False, 128, 0), (True, 128, 0),
+                                          (False, 256, capi.LOOP_STAGE_INPUTS),
+                                          (True, 64, capi.LOOP_STAGE_INPUTS),
+                                          (False, 1024, capi.LOOP_STAGE_INPUTS)])
+def test_res_calc_hierarchical_bitwise(gpu, soa, bs, flags):
+    p = airfoil_problem(soa=soa, res0_seed=5, block_size=bs)
+    ref = _replay(p, O.RES_CALC, bs)
+    serial, S = _serial(p, O.RES_CALC)
+    p.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, ITERS, flags=flags)
+    assert_bitwise(p.point_data[0], ref, "hierarchical")
+    assert_close_scaled(p.point_data[0], serial, S, what="hierarchical vs serial")
+
+
+@pytest.mark.parametrize("method", ["kway", "ours"])
+def test_res_calc_partitioned_renumbered(gpu, method):
+    p = airfoil_problem(nx=64, ny=32, res0_seed=6, block_size=256)
+    orig = airfoil_problem(nx=64, ny=32, res0_seed=6, block_size=256)
+    if method == "kway":
+        p.reorder("gps")
+        p.partition(1.001)
+    else:
+        p.partition_ours(1.001, 4)
+    p.reorder_to_partition()
+    p.renumber_points()
+    ref = _replay(p, O.RES_CALC, 256, p.partition_vector)
+    p.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, ITERS, flags=capi.LOOP_STAGE_INPUTS)
+    assert_bitwise(p.point_data[0], ref, "partitioned hierarchical")
+    # reordering invariance (res_calc.cu:129-181), through applied_permutation
+    serial, S = _serial(orig, O.RES_CALC)
+    assert_close_scaled(p.target_in_original_order(), serial, S, what="reordered vs original")
+
+
+def test_res_calc_graph_replay_matches_eager(gpu):
+    p1 = airfoil_problem(res0_seed=7)
+    p2 = airfoil_problem(res0_seed=7)
+    p1.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, 5, flags=capi.LOOP_STAGE_INPUTS)
+    p2.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, 5, flags=capi.LOOP_STAGE_INPUTS,
+                use_graph=True)
+    assert_bitwise(p2.point_data[0], p1.point_data[0], "graph vs eager")
+
+
+def test_res_calc_atomic_run_to_run(gpu):
+    a = airfoil_problem(res0_seed=8)
+    b = airfoil_problem(res0_seed=8)
+    ref, S = _serial(airfoil_problem(res0_seed=8), O.RES_CALC)
+    a.gpu_loop(RES_CALC, capi.STRATEGY_ATOMIC, ITERS)
+    b.gpu_loop(RES_CALC, capi.STRATEGY_ATOMIC, ITERS)
+    assert_close_scaled(a.point_data[0], b.point_data[0], S, what="atomic run to run")
+
+
+@pytest.mark.parametrize("P,C,soa", [(4, 4, False), (4, 1, True), (1, 1, False), (8, 8, True)])
+def test_mine_strategies(gpu, P, C, soa):
+    f = capi.functor(capi.KERNEL_MINE, 2, P, C)
+    base = mine_problem(P=P, C=C, soa=soa)
+    ref_serial, S = _serial(base, O.MINE2)
+    ref_colour = _colour(mine_problem(P=P, C=C, soa=soa), O.MINE2)
+    ref_hier = _replay(mine_problem(P=P, C=C, soa=soa), O.MINE2, 128)
+    g = mine_problem(P=P, C=C, soa=soa)
+    g.gpu_loop(f, capi.STRATEGY_GLOBAL_COLOUR, ITERS)
+    assert_bitwise(g.point_data[0], ref_colour, "mine global colour")
+    h = mine_problem(P=P, C=C, soa=soa)
+    h.gpu_loop(f, capi.STRATEGY_HIERARCHICAL, ITERS)
+    assert_bitwise(h.point_data[0], ref_hier, "mine hierarchical")
+    a = mine_problem(P=P, C=C, soa=soa)
+    a.gpu_loop(f, capi.STRATEGY_ATOMIC, ITERS)
+    assert_close_scaled(a.point_data[0], ref_serial, S, what="mine atomic")
+
+
+@pytest.mark.parametrize("soa", [False, True])
+def test_hex_scatter_strategies(gpu, soa):
+    f = capi.functor(capi.KERNEL_HEX_SCATTER)
+    ref_serial, S = _serial(hex_problem(soa=soa), O.HEX_SCATTER)
+    ref_hier = _replay(hex_problem(soa=soa), O.HEX_SCATTER, 128)
+    ref_colour = _colour(hex_problem(soa=soa), O.HEX_SCATTER)
+    for strategy, ref, exact in [(capi.STRATEGY_HIERARCHICAL, ref_hier, True),
+                                 (capi.STRATEGY_GLOBAL_COLOUR, ref_colour, True),
+                                 (capi.STRATEGY_ATOMIC, ref_serial, False)]:
+        p = hex_problem(soa=soa)
+        p.gpu_loop(f, strategy, ITERS)
+        if exact:
+            assert_bitwise(p.point_data[0], ref, f"hex strategy {strategy}")
+        else:
+            assert_close_scaled(p.point_data[0], ref, S, what="hex atomic")
+
+
+def test_empty_and_single_cell(gpu):
+    # a mesh of one edge: every strategy adds exactly one increment pair
+    m = airfoil_problem(nx=2, ny=2, res0_seed=9)
+    ref, S = _serial(m, O.RES_CALC)
+    for strategy in (capi.STRATEGY_ATOMIC, capi.STRATEGY_GLOBAL_COLOUR,
+                     capi.STRATEGY_HIERARCHICAL):
+        p = airfoil_problem(nx=2, ny=2, res0_seed=9)
+        p.gpu_loop(RES_CALC, strategy, ITERS)
+        assert_close_scaled(p.point_data[0], ref, S, what=f"tiny mesh strategy {strategy}")
