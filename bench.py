@@ -46,9 +46,16 @@ def parse():
     ap.add_argument("--strategy", default="hierarchical",
                     choices=["atomic", "global", "hierarchical"])
     ap.add_argument("--block-size", type=int, default=512)
-    ap.add_argument("--partition", default="kway", choices=["none", "kway", "ours", "gps"])
+    ap.add_argument("--partition", default="points",
+                    choices=["none", "kway", "ours", "gps", "points"],
+                    help="points: owner partition of mapping-0 points (owner compute)")
+    ap.add_argument("--points-per-block", type=int, default=256)
     ap.add_argument("--no-stage", action="store_true")
-    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--order", default="owner", choices=["owner", "dep", "launch"],
+                    help="hierarchical block scheduling: owner compute in one launch (owner), "
+                         "block colours ordered by completion flags in one launch (dep) or "
+                         "one launch per block colour (launch)")
+    ap.add_argument("--graph", action="store_true", help="replay each step from a hipGraph")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-atomic-ref", action="store_true")
@@ -70,19 +77,25 @@ def build_problem(args, partition_kind):
         p.partition_ours(1.001, 8)
         p.reorder_to_partition()
         p.renumber_points()
+    elif partition_kind == "points":
+        p.partition_points(args.points_per_block)
     return mesh, p
 
 
-def make_loop(p, strategy, block_size, stage, graph):
+def make_loop(p, strategy, block_size, stage, graph, order="launch"):
     from cuda_fot_amd import DeviceArray, Loop, capi
     strat = {"atomic": capi.STRATEGY_ATOMIC, "global": capi.STRATEGY_GLOBAL_COLOUR,
              "hierarchical": capi.STRATEGY_HIERARCHICAL}[strategy]
     dev = [DeviceArray.from_host(d) for d in p.point_data]
     out = DeviceArray.from_host(p.point_data[0])
-    flags = capi.LOOP_STAGE_INPUTS if stage else 0
+    flags = (capi.LOOP_STAGE_INPUTS if stage else 0) | \
+        {"launch": 0, "dep": capi.LOOP_DEPENDENCY_ORDER, "owner": capi.LOOP_OWNER_COMPUTE}[order]
+    part = p.partition_vector if strat == capi.STRATEGY_HIERARCHICAL else None
+    if strat == capi.STRATEGY_HIERARCHICAL and order == "owner" and p.point_partition is not None:
+        part = p.point_partition
+        flags |= capi.LOOP_POINT_PARTITION
     loop = Loop(capi.functor(capi.KERNEL_RES_CALC), p.maps, p.num_points, dev, out,
-                strategy=strat, block_size=block_size,
-                partition=p.partition_vector if strat == capi.STRATEGY_HIERARCHICAL else None,
+                strategy=strat, block_size=block_size, partition=part,
                 use_graph=graph, flags=flags)
     return loop, dev, out
 
@@ -144,12 +157,13 @@ def main():
         print("no HIP device", file=sys.stderr)
         sys.exit(1)
     t_plan = time.perf_counter()
-    mesh, p = build_problem(args, args.partition if args.strategy == "hierarchical" else "none")
+    mesh, p = build_problem(args, args.partition)
     loop, dev, out = make_loop(p, args.strategy, args.block_size, not args.no_stage,
-                               not args.no_graph)
+                               args.graph, args.order)
     plan_s = time.perf_counter() - t_plan
     stats = loop.stats()
     wall, ev = time_loop(loop, args.steps, args.warmup)
+    capi.call("fot_loop_check", loop.handle)
     ne = mesh["num_edges"]
     ubytes = unique_bytes(mesh["num_nodes"], mesh["num_cells"], ne)
     achieved = ubytes * args.steps / ev
@@ -169,12 +183,15 @@ def main():
         "config": {"workload": f"res_calc airfoil {args.nx}x{args.ny} quad cells, "
                                f"{ne} interior edges, shuffled, seed 0",
                    "strategy": args.strategy, "block_size": args.block_size,
-                   "partition": args.partition if args.strategy == "hierarchical" else "none",
-                   "stage_inputs": not args.no_stage, "graph": not args.no_graph,
+                   "partition": args.partition,
+                   "points_per_block": args.points_per_block if args.partition == "points" else None,
+                   "stage_inputs": not args.no_stage, "graph": args.graph,
+                   "block_order": args.order,
                    "launches_per_step": stats["num_launches_per_iter"],
                    "block_colours": stats["num_colours"],
                    "avg_thread_colours": round(stats["avg_thread_colours"], 3),
                    "reuse_factor": round(stats["reuse_factor"], 4),
+                   "redundancy": round(stats["redundancy"], 4),
                    "plan_seconds": round(plan_s, 2)},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": None,
@@ -183,12 +200,19 @@ def main():
     }
     loop.close()
     if args.strategy != "atomic" and not args.no_atomic_ref:
-        _, pa = build_problem(args, "none")
-        la, _, _ = make_loop(pa, "atomic", 256, False, not args.no_graph)
-        wa, eva = time_loop(la, args.steps, args.warmup)
+        # the plain atomics loop on the same (reordered) mesh, and on the
+        # shuffled mesh as generated
+        la, _, _ = make_loop(p, "atomic", 256, False, args.graph)
+        wa, _ = time_loop(la, args.steps, args.warmup)
         la.close()
+        _, ps = build_problem(args, "none")
+        ls, _, _ = make_loop(ps, "atomic", 256, False, args.graph)
+        wsh, _ = time_loop(ls, args.steps, args.warmup)
+        ls.close()
         result["atomic_edges_per_s"] = ne * args.steps / wa
-        result["speedup_vs_atomic"] = result["value"] / result["atomic_edges_per_s"]
+        result["atomic_shuffled_edges_per_s"] = ne * args.steps / wsh
+        result["speedup_vs_atomic"] = result["value"] / max(result["atomic_edges_per_s"],
+                                                            result["atomic_shuffled_edges_per_s"])
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(mesh, args.cpu_seconds)
     print(json.dumps(result), flush=True)

@@ -44,6 +44,8 @@ STRATEGY_HIERARCHICAL = 2
 
 LOOP_STAGE_INPUTS = 1
 LOOP_DEPENDENCY_ORDER = 2
+LOOP_OWNER_COMPUTE = 4
+LOOP_POINT_PARTITION = 8
 
 
 class FotError(RuntimeError):
@@ -96,7 +98,17 @@ class LoopStats(C.Structure):
                 ("block_size", C.c_uint32), ("max_cached", C.c_uint32),
                 ("lds_bytes", C.c_uint32), ("max_thread_colours", C.c_uint32),
                 ("plan_device_bytes", C.c_uint64),
-                ("avg_thread_colours", C.c_double), ("reuse_factor", C.c_double)]
+                ("avg_thread_colours", C.c_double), ("reuse_factor", C.c_double),
+                ("redundancy", C.c_double)]
+
+
+class OplanView(C.Structure):
+    _fields_ = [("mesh_dim", C.c_uint32), ("num_blocks", C.c_uint32),
+                ("num_positions", C.c_uint32), ("num_cached", C.c_uint32),
+                ("blk_cell_off", C.POINTER(C.c_uint32)), ("blk_cache_off", C.POINTER(C.c_uint32)),
+                ("blk_owned", C.POINTER(C.c_uint32)), ("cached", C.POINTER(C.c_uint32)),
+                ("cells", C.POINTER(C.c_uint32)), ("loc", C.POINTER(C.c_uint32)),
+                ("colour", C.POINTER(C.c_uint8)), ("blk_ncol", C.POINTER(C.c_uint8))]
 
 
 class HplanInfo(C.Structure):
@@ -161,6 +173,9 @@ _PROTOS = {
     "fot_hplan_free": [P],
     "fot_hplan_info_get": [P, P],
     "fot_hplan_colour_get": [P, U32, P],
+    "fot_oplan_build": [P, U32, U32, U32, P, U32, P, P],
+    "fot_oplan_get": [P, P],
+    "fot_oplan_free": [P],
     "fot_reorder_to_partition": [P, U32, P],
     "fot_point_renumbering_by_partition": [P, U32, U32, U32, P, P],
     "fot_point_renumbering_first_touch": [P, U32, U32, U32, P],
@@ -181,11 +196,12 @@ _PROTOS = {
     "fot_loop_destroy": [P],
     "fot_loop_step": [P, P],
     "fot_loop_run": [P, U32, P],
+    "fot_loop_check": [P],
     "fot_loop_stats_get": [P, P],
 }
 
 _RESTYPES = {"fot_status_string": C.c_char_p, "fot_last_error_message": C.c_char_p,
-             "fot_version": I32, "fot_hplan_free": None}
+             "fot_version": I32, "fot_hplan_free": None, "fot_oplan_free": None}
 
 
 def _load() -> C.CDLL:

@@ -218,8 +218,27 @@ FOT_DEV void add_to_cache(double *cache, uint32_t m, const uint32_t *loc, const 
   }
 }
 
-// Runs `body.template operator()<F, SOA>()` for the functor a description
-// names (defined in fot_step.hip).
+// As add_to_cache, for owner-compute blocks: only slots of owned points
+// (local id below `owned`) accumulate; halo increments belong to another
+// block.
+template <class F>
+FOT_DEV void add_to_cache_owned(double *cache, uint32_t m, const uint32_t *loc, const double *inc,
+                                uint32_t owned) {
+  if (F::SLOT_OUTER) {
+#pragma unroll
+    for (uint32_t k = 0; k < F::MD; ++k)
+#pragma unroll
+      for (uint32_t d = 0; d < F::TD; ++d)
+        if (loc[k] < owned) cache[d * m + loc[k]] += inc[k * F::TD + d];
+  } else {
+#pragma unroll
+    for (uint32_t d = 0; d < F::TD; ++d)
+#pragma unroll
+      for (uint32_t k = 0; k < F::MD; ++k)
+        if (loc[k] < owned) cache[d * m + loc[k]] += inc[k * F::TD + d];
+  }
+}
+
 } // namespace fot
 
 #endif

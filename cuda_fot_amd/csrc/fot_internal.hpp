@@ -72,6 +72,30 @@ struct fot_hplan {
 };
 
 namespace fot {
+// colourCellsWithOrdering on block-local ids; returns colours or -1.
+int colour_block(const uint32_t *local, uint32_t nc, uint32_t md, uint8_t *colours);
+void split_blocks(uint32_t num_cells, const uint32_t *partition, uint32_t block_size,
+                  std::vector<uint32_t> &order, std::vector<uint32_t> &blocks);
+
+// Owner-compute plan: every mapping-0 point is owned by one block; a block
+// runs every cell touching a point it owns (cells on a block boundary run in
+// each owner block) and writes back only its owned points.
+struct OwnerPlan {
+  uint32_t mesh_dim = 0;
+  std::vector<uint32_t> blk_cell_off{0};  // [nb+1] ranges of plan positions
+  std::vector<uint32_t> blk_cache_off{0}; // [nb+1] ranges of `cached`
+  std::vector<uint32_t> blk_owned;        // [nb] owned points (a prefix of the range)
+  std::vector<uint32_t> cached;           // owned points ascending, then halo ascending
+  std::vector<uint32_t> cells;            // original cell of each plan position
+  std::vector<uint32_t> loc;              // [positions * mesh_dim] block-local ids
+  std::vector<uint8_t> colour;            // thread colour per position
+  std::vector<uint8_t> blk_ncol;          // thread colours per block
+};
+fot_status build_owner_plan(const uint32_t *c2n0, uint32_t num_cells, uint32_t mesh_dim,
+                            uint32_t num_points, const uint32_t *partition,
+                            uint32_t block_size, const uint32_t *point_partition,
+                            OwnerPlan &out);
+
 fot_status build_hplan(const uint32_t *const *c2n, const uint32_t *mesh_dim,
                        uint32_t num_mappings, uint32_t num_cells,
                        uint32_t num_points0, const uint32_t *partition,

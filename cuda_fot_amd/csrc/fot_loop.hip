@@ -45,6 +45,8 @@ struct HierDev {
   const uint16_t *loc;           // [num_cells * MD] block-local ids (AOS)
   const uint8_t *colour;         // [num_cells] thread colour
   const uint8_t *blk_ncol;       // [nb] thread colours per block
+  const uint32_t *blk_owned;     // [nb] owned points (owner-compute plans only)
+  const uint32_t *packed;        // res_calc owner compute: l0 | l1 << 12 | colour << 24
 };
 
 template <uint32_t MD>
@@ -118,14 +120,104 @@ __global__ __launch_bounds__(1024) void res_calc_atomic_kernel(EngineArgs a) {
 // ---- hierarchical launches --------------------------------------------
 // One workgroup per plan block: increments accumulate in an LDS cache of the
 // block's points thread colour by thread colour (one barrier each), then the
-// cache is added to the output.  One launch covers one block colour, whose
-// blocks share no point, so the writeback is a plain read-modify-write and
-// the summation order is fixed by the plan.
-template <class F, bool SOA>
-__global__ __launch_bounds__(1024) void eng_hier_kernel(EngineArgs a, HierDev h,
+// cache is added to the output.  Blocks of one block colour share no point.
+// The writeback order between colours is enforced either by launch
+// boundaries (one launch per block colour) or, in a single launch, by
+// per-block completion flags: a block writes back only after every
+// lower-colour block it shares a point with has published its writeback.
+// Both keep the summation order of the plan, so they give identical bits.
+
+struct DepSync {
+  uint32_t ticket;     // next plan block to hand out
+  uint32_t done;       // blocks retired in this launch
+  uint32_t generation; // launches completed
+  uint32_t error;      // a bounded wait ran out
+};
+
+struct DepDev {
+  const uint32_t *dep_off; // [nb+1]
+  const uint32_t *deps;    // immediate lower-colour neighbours of each block
+  uint32_t *flags;         // [nb] = generation + 1 once written back
+  DepSync *sync;
+  uint32_t nb;
+};
+
+// ~0.1 s of s_sleep polling; a wait that long means a broken plan.
+constexpr uint32_t kSpinLimit = 1u << 21;
+
+// The plan block of this workgroup.  Per-colour launches remap blockIdx so
+// each XCD gets a contiguous run of blocks; the single launch hands out
+// tickets in plan (colour-major) order, so every block a workgroup may wait
+// on has already been taken by a running workgroup.
+template <bool DEP>
+FOT_DEV uint32_t pick_block(const DepDev &dd, uint32_t blk_begin, uint32_t nbc, uint32_t &gen) {
+  if constexpr (!DEP) {
+    gen = 0;
+    return blk_begin + xcd_block(blockIdx.x, nbc);
+  } else {
+    __shared__ uint32_t s_block, s_gen;
+    if (threadIdx.x == 0) {
+      s_block = atomicAdd(&dd.sync->ticket, 1u);
+      s_gen = __hip_atomic_load(&dd.sync->generation, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    gen = s_gen;
+    return s_block;
+  }
+}
+
+// Waits (first wave) for the lower-colour neighbours of block b.
+template <bool DEP>
+FOT_DEV void wait_predecessors(const DepDev &dd, uint32_t b, uint32_t gen) {
+  if constexpr (DEP) {
+    if (threadIdx.x < 64) {
+      const uint32_t d0 = dd.dep_off[b], nd = dd.dep_off[b + 1] - d0;
+      for (uint32_t k = threadIdx.x; k < nd; k += 64) {
+        const uint32_t p = dd.deps[d0 + k];
+        uint32_t spins = 0;
+        while (__hip_atomic_load(&dd.flags[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
+               gen + 1u) {
+          __builtin_amdgcn_s_sleep(1);
+          if (++spins == kSpinLimit) {
+            atomicOr(&dd.sync->error, 1u);
+            break;
+          }
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+  }
+}
+
+// Publishes block b's writeback; the last block of the launch re-arms the
+// ticket counter for the next launch.
+template <bool DEP>
+FOT_DEV void publish(const DepDev &dd, uint32_t b, uint32_t gen) {
+  if constexpr (DEP) {
+    __builtin_amdgcn_s_waitcnt(0); // this wave's stores are in L2
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __hip_atomic_store(&dd.flags[b], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (atomicAdd(&dd.sync->done, 1u) == dd.nb - 1u) {
+        __hip_atomic_store(&dd.sync->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&dd.sync->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&dd.sync->generation, gen + 1u, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  }
+}
+
+// MODE: 0 one launch per block colour, 1 dependency order
+template <class F, bool SOA, int MODE>
+__global__ __launch_bounds__(1024) void eng_hier_kernel(EngineArgs a, HierDev h, DepDev dd,
                                                         uint32_t blk_begin, uint32_t nbc) {
+  constexpr bool DEP = MODE == 1;
   extern __shared__ double cache[];
-  const uint32_t b = blk_begin + xcd_block(blockIdx.x, nbc);
+  uint32_t gen;
+  const uint32_t b = pick_block<DEP>(dd, blk_begin, nbc, gen);
   const uint32_t tid = threadIdx.x;
   const uint32_t e0 = h.blk_cell_off[b], ne = h.blk_cell_off[b + 1] - e0;
   const uint32_t c0 = h.blk_cache_off[b], m = h.blk_cache_off[b + 1] - c0;
@@ -147,6 +239,7 @@ __global__ __launch_bounds__(1024) void eng_hier_kernel(EngineArgs a, HierDev h,
     if (colour == c) add_to_cache<F>(cache, m, loc, inc);
     __syncthreads();
   }
+  wait_predecessors<DEP>(dd, b, gen);
   const uint32_t np = a.num_points[0];
   for (uint32_t i = tid; i < m; i += blockDim.x) {
     const uint32_t g = h.cached[c0 + i];
@@ -156,17 +249,21 @@ __global__ __launch_bounds__(1024) void eng_hier_kernel(EngineArgs a, HierDev h,
       *p = *p + cache[d * m + i];
     }
   }
+  publish<DEP>(dd, b, gen);
 }
 
 // res_calc hierarchical with the block's cells' q and adt staged in LDS next
 // to the res cache (they are read through the same mapping-0 points), so
 // the edge loop gathers only x from HBM.
-template <bool SOA>
+template <bool SOA, int MODE>
 __global__ __launch_bounds__(1024) void res_calc_hier_staged_kernel(EngineArgs a, HierDev h,
+                                                                    DepDev dd,
                                                                     uint32_t blk_begin,
                                                                     uint32_t nbc) {
+  constexpr bool DEP = MODE == 1;
   extern __shared__ double lds[];
-  const uint32_t b = blk_begin + xcd_block(blockIdx.x, nbc);
+  uint32_t gen;
+  const uint32_t b = pick_block<DEP>(dd, blk_begin, nbc, gen);
   const uint32_t tid = threadIdx.x;
   const uint32_t e0 = h.blk_cell_off[b], ne = h.blk_cell_off[b + 1] - e0;
   const uint32_t c0 = h.blk_cache_off[b], m = h.blk_cache_off[b + 1] - c0;
@@ -188,16 +285,13 @@ __global__ __launch_bounds__(1024) void res_calc_hier_staged_kernel(EngineArgs a
     for (uint32_t d = 0; d < 4; ++d) res_s[d * m + i] = 0.0;
   }
   uint32_t colour = 0xffu, l0 = 0, l1 = 0;
-  uint2 nn = make_uint2(0, 0);
+  double x1[2] = {0, 0}, x2[2] = {0, 0};
   if (tid < ne) {
     const uint32_t pos = e0 + tid;
     colour = h.colour[pos];
     const uint32_t lv = reinterpret_cast<const uint32_t *>(h.loc)[pos];
     l0 = lv & 0xffffu, l1 = lv >> 16;
-    nn = reinterpret_cast<const uint2 *>(a.maps[1])[pos];
-  }
-  double x1[2], x2[2];
-  if (tid < ne) {
+    const uint2 nn = reinterpret_cast<const uint2 *>(a.maps[1])[pos];
     if (SOA) {
       const uint32_t nx = a.num_points[1];
       x1[0] = x[nn.x], x1[1] = x[nx + nn.x], x2[0] = x[nn.y], x2[1] = x[nx + nn.y];
@@ -227,6 +321,7 @@ __global__ __launch_bounds__(1024) void res_calc_hier_staged_kernel(EngineArgs a
     }
     __syncthreads();
   }
+  wait_predecessors<DEP>(dd, b, gen);
   const uint32_t np = a.num_points[0];
   for (uint32_t i = tid; i < m; i += blockDim.x) {
     const uint32_t g = h.cached[c0 + i];
@@ -245,6 +340,147 @@ __global__ __launch_bounds__(1024) void res_calc_hier_staged_kernel(EngineArgs a
       v1.y = v1.y + res_s[3 * m + i];
       o[0] = v0;
       o[1] = v1;
+    }
+  }
+  publish<DEP>(dd, b, gen);
+}
+
+// ---- owner-compute launches -----------------------------------------------
+// One launch over all blocks.  A block's LDS cache holds the current values
+// of its owned points, loaded while its inputs are staged; the increments of
+// its cells are added thread colour by thread colour (halo slots belong to
+// another block and are skipped) and the cache is stored back.  No block
+// shares a written point with another, so there is no order between blocks.
+
+template <class F, bool SOA>
+__global__ __launch_bounds__(1024) void eng_owner_kernel(EngineArgs a, HierDev h, uint32_t nb) {
+  extern __shared__ double lds[];
+  const uint32_t b = xcd_block(blockIdx.x, nb);
+  const uint32_t tid = threadIdx.x;
+  const uint32_t e0 = h.blk_cell_off[b], ne = h.blk_cell_off[b + 1] - e0;
+  const uint32_t c0 = h.blk_cache_off[b];
+  const uint32_t nw = h.blk_owned[b];
+  double *cache = lds;
+  uint32_t *g_s = reinterpret_cast<uint32_t *>(lds + size_t(F::TD) * nw);
+  const uint32_t np = a.num_points[0];
+  for (uint32_t i = tid; i < nw; i += blockDim.x) {
+    const uint32_t g = h.cached[c0 + i];
+    g_s[i] = g;
+#pragma unroll
+    for (uint32_t d = 0; d < F::TD; ++d) cache[d * nw + i] = a.out[pidx<SOA>(np, g, F::TD, d)];
+  }
+  uint32_t loc[F::MD], tgt[F::MD];
+  double inc[F::MD * F::TD];
+  uint32_t colour = 0xffu;
+  if (tid < ne) {
+    const uint32_t pos = e0 + tid;
+    colour = h.colour[pos];
+    load_loc<F::MD>(h.loc + size_t(pos) * F::MD, loc);
+#pragma unroll
+    for (uint32_t k = 0; k < F::MD; ++k) tgt[k] = h.cached[c0 + loc[k]];
+    F::template compute<SOA>(a, pos, tgt, inc);
+  }
+  __syncthreads();
+  const uint32_t ncol = h.blk_ncol[b];
+  for (uint32_t c = 0; c < ncol; ++c) {
+    if (colour == c) add_to_cache_owned<F>(cache, nw, loc, inc, nw);
+    __syncthreads();
+  }
+  for (uint32_t i = tid; i < nw; i += blockDim.x) {
+    const uint32_t g = g_s[i];
+#pragma unroll
+    for (uint32_t d = 0; d < F::TD; ++d) a.out[pidx<SOA>(np, g, F::TD, d)] = cache[d * nw + i];
+  }
+}
+
+// res_calc owner compute: q and adt of owned and halo cells staged in LDS
+// next to the owned res values; one packed word per edge holds both local
+// ids (12 bits each) and the thread colour (8 bits).
+template <bool SOA>
+__global__ __launch_bounds__(1024) void res_calc_owner_kernel(EngineArgs a, HierDev h,
+                                                              uint32_t nb) {
+  extern __shared__ double lds[];
+  const uint32_t b = xcd_block(blockIdx.x, nb);
+  const uint32_t tid = threadIdx.x;
+  const uint32_t e0 = h.blk_cell_off[b], ne = h.blk_cell_off[b + 1] - e0;
+  const uint32_t c0 = h.blk_cache_off[b], m = h.blk_cache_off[b + 1] - c0;
+  const uint32_t nw = h.blk_owned[b];
+  double *res_s = lds, *q_s = lds + 4 * nw, *adt_s = q_s + 4 * m;
+  uint32_t *g_s = reinterpret_cast<uint32_t *>(adt_s + m);
+  const double *x = a.point_data[1], *q = a.point_data[2], *adt = a.point_data[3];
+  const uint32_t np = a.num_points[0];
+  // edge words first: their x gathers overlap the staging below
+  uint32_t pk = 0xff000000u;
+  uint2 nn = make_uint2(0, 0);
+  if (tid < ne) {
+    pk = h.packed[e0 + tid];
+    nn = reinterpret_cast<const uint2 *>(a.maps[1])[e0 + tid];
+  }
+  for (uint32_t i = tid; i < m; i += blockDim.x) {
+    const uint32_t g = h.cached[c0 + i];
+    if (SOA) {
+      const uint32_t nq = a.num_points[2];
+#pragma unroll
+      for (uint32_t d = 0; d < 4; ++d) q_s[d * m + i] = q[size_t(d) * nq + g];
+      if (i < nw) {
+#pragma unroll
+        for (uint32_t d = 0; d < 4; ++d) res_s[d * nw + i] = a.out[size_t(d) * np + g];
+      }
+    } else {
+      const double2 *q2p = reinterpret_cast<const double2 *>(q);
+      const double2 v0 = q2p[2 * size_t(g)], v1 = q2p[2 * size_t(g) + 1];
+      q_s[0 * m + i] = v0.x, q_s[1 * m + i] = v0.y, q_s[2 * m + i] = v1.x, q_s[3 * m + i] = v1.y;
+      if (i < nw) {
+        const double2 *o = reinterpret_cast<const double2 *>(a.out) + 2 * size_t(g);
+        const double2 r0 = o[0], r1 = o[1];
+        res_s[0 * nw + i] = r0.x, res_s[1 * nw + i] = r0.y;
+        res_s[2 * nw + i] = r1.x, res_s[3 * nw + i] = r1.y;
+      }
+    }
+    adt_s[i] = adt[g];
+    if (i < nw) g_s[i] = g;
+  }
+  double x1[2] = {0, 0}, x2[2] = {0, 0};
+  if (tid < ne) {
+    if (SOA) {
+      const uint32_t nx = a.num_points[1];
+      x1[0] = x[nn.x], x1[1] = x[nx + nn.x], x2[0] = x[nn.y], x2[1] = x[nx + nn.y];
+    } else {
+      const double2 xa = reinterpret_cast<const double2 *>(x)[nn.x];
+      const double2 xb = reinterpret_cast<const double2 *>(x)[nn.y];
+      x1[0] = xa.x, x1[1] = xa.y, x2[0] = xb.x, x2[1] = xb.y;
+    }
+  }
+  __syncthreads();
+  const uint32_t l0 = pk & 0xfffu, l1 = (pk >> 12) & 0xfffu, colour = pk >> 24;
+  double inc[8];
+  if (tid < ne) {
+    double q1[4], q2[4];
+#pragma unroll
+    for (uint32_t d = 0; d < 4; ++d) q1[d] = q_s[d * m + l0], q2[d] = q_s[d * m + l1];
+    const double a1 = adt_s[l0], a2 = adt_s[l1];
+    res_calc_user(x1, x2, q1, q2, &a1, &a2, inc, inc + 4, 1, 1);
+  }
+  const uint32_t ncol = h.blk_ncol[b];
+  for (uint32_t c = 0; c < ncol; ++c) {
+    if (colour == c) {
+#pragma unroll
+      for (uint32_t d = 0; d < 4; ++d) {
+        if (l0 < nw) res_s[d * nw + l0] += inc[d];
+        if (l1 < nw) res_s[d * nw + l1] += inc[4 + d];
+      }
+    }
+    __syncthreads();
+  }
+  for (uint32_t i = tid; i < nw; i += blockDim.x) {
+    const uint32_t g = g_s[i];
+    if (SOA) {
+#pragma unroll
+      for (uint32_t d = 0; d < 4; ++d) a.out[size_t(d) * np + g] = res_s[d * nw + i];
+    } else {
+      double2 *o = reinterpret_cast<double2 *>(a.out) + 2 * size_t(g);
+      o[0] = make_double2(res_s[0 * nw + i], res_s[1 * nw + i]);
+      o[1] = make_double2(res_s[2 * nw + i], res_s[3 * nw + i]);
     }
   }
 }
@@ -284,6 +520,9 @@ struct fot_loop {
   std::vector<uint32_t> colour_blk_off;
   uint32_t lds_bytes = 0, threads = 0;
   bool staged = false;
+  bool dep_order = false;
+  bool owner = false;
+  DepDev dep{};
   // graph of one iteration
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -396,20 +635,61 @@ fot_status fot_loop::build_global_colour() {
 }
 
 fot_status fot_loop::build_hierarchical() {
-  fot_hplan plan;
-  fot_status st = build_hplan(desc.cell_to_node, desc.mesh_dim, desc.num_mappings,
-                              desc.num_cells, desc.num_points[0], cfg.partition,
-                              cfg.block_size, plan);
-  if (st != FOT_OK) return st;
+  fot_status st;
   const uint32_t md = desc.mesh_dim[0];
-  std::vector<uint32_t> blk_cell_off{0}, blk_cache_off{0}, cached, plan_cell;
+  std::vector<uint32_t> blk_cell_off{0}, blk_cache_off{0}, cached, plan_cell, blk_owned;
   std::vector<uint16_t> loc;
   std::vector<uint8_t> colour, blk_ncol;
   colour_blk_off.assign(1, 0);
-  uint32_t max_cached = 0, max_tc = 0;
+  uint32_t max_cached = 0, max_tc = 0, num_plan_colours = 0;
   double tc_sum = 0;
   const uint32_t mask = mapping_mask(f);
   std::vector<std::vector<uint32_t>> plan_maps(desc.num_mappings);
+  owner = (cfg.flags & FOT_LOOP_OWNER_COMPUTE) != 0;
+  if (owner && (cfg.flags & FOT_LOOP_DEPENDENCY_ORDER))
+    return fail(FOT_ERR_INVALID_ARGUMENT, "owner compute needs no dependency order");
+  if (owner) {
+    OwnerPlan op;
+    const uint32_t *cell_part = cfg.partition, *point_part = nullptr;
+    if (cfg.flags & FOT_LOOP_POINT_PARTITION) {
+      if (!cfg.partition) return fail(FOT_ERR_INVALID_ARGUMENT, "point partition missing");
+      point_part = cfg.partition;
+      cell_part = nullptr;
+    }
+    st = build_owner_plan(desc.cell_to_node[0], desc.num_cells, md, desc.num_points[0],
+                          cell_part, cfg.block_size, point_part, op);
+    if (st != FOT_OK) return st;
+    const uint32_t nb = uint32_t(op.blk_owned.size());
+    blk_cell_off = op.blk_cell_off;
+    blk_cache_off = op.blk_cache_off;
+    blk_owned = op.blk_owned;
+    cached = op.cached;
+    plan_cell = op.cells;
+    colour = op.colour;
+    blk_ncol = op.blk_ncol;
+    loc.assign(op.loc.begin(), op.loc.end());
+    for (uint32_t b = 0; b < nb; ++b) {
+      max_cached = std::max(max_cached, blk_cache_off[b + 1] - blk_cache_off[b]);
+      max_tc = std::max<uint32_t>(max_tc, blk_ncol[b]);
+      tc_sum += blk_ncol[b];
+    }
+    for (uint32_t m = 1; m < desc.num_mappings; ++m) {
+      if (!((mask >> m) & 1u)) continue;
+      const uint32_t dm = desc.mesh_dim[m];
+      plan_maps[m].resize(size_t(plan_cell.size()) * dm);
+      for (size_t i = 0; i < plan_cell.size(); ++i)
+        std::memcpy(&plan_maps[m][i * dm], desc.cell_to_node[m] + size_t(plan_cell[i]) * dm,
+                    dm * sizeof(uint32_t));
+    }
+    colour_blk_off.push_back(nb);
+    num_plan_colours = 1;
+  } else {
+  fot_hplan plan;
+  st = build_hplan(desc.cell_to_node, desc.mesh_dim, desc.num_mappings,
+                              desc.num_cells, desc.num_points[0], cfg.partition,
+                              cfg.block_size, plan);
+  if (st != FOT_OK) return st;
+  num_plan_colours = uint32_t(plan.colours.size());
   for (const HostColour &hc : plan.colours) {
     const uint32_t nb = uint32_t(hc.num_cell_colours.size());
     const uint32_t n = uint32_t(hc.cell_colours.size());
@@ -440,6 +720,7 @@ fot_status fot_loop::build_hierarchical() {
     }
     colour_blk_off.push_back(colour_blk_off.back() + nb);
   }
+  }
   const uint32_t nb_total = colour_blk_off.back();
   uint32_t *d32 = nullptr;
   uint16_t *d16 = nullptr;
@@ -456,16 +737,72 @@ fot_status fot_loop::build_hierarchical() {
   hier.colour = d8;
   if ((st = upload(blk_ncol.data(), blk_ncol.size(), &d8)) != FOT_OK) return st;
   hier.blk_ncol = d8;
+  if (owner) {
+    if ((st = upload(blk_owned.data(), blk_owned.size(), &d32)) != FOT_OK) return st;
+    hier.blk_owned = d32;
+    if (f.kernel == FOT_KERNEL_RES_CALC) {
+      std::vector<uint32_t> packed(plan_cell.size());
+      for (size_t i = 0; i < plan_cell.size(); ++i) {
+        const uint32_t l0 = loc[2 * i], l1 = loc[2 * i + 1];
+        if (l0 > 0xfffu || l1 > 0xfffu)
+          return fail(FOT_ERR_UNSUPPORTED, "owner block with more than 4096 points");
+        packed[i] = l0 | (l1 << 12) | (uint32_t(colour[i]) << 24);
+      }
+      if ((st = upload(packed.data(), packed.size(), &d32)) != FOT_OK) return st;
+      hier.packed = d32;
+    }
+  }
   plan_args = base;
   for (uint32_t m = 1; m < desc.num_mappings; ++m) {
     if (!((mask >> m) & 1u)) continue;
     if ((st = upload(plan_maps[m].data(), plan_maps[m].size(), &d32)) != FOT_OK) return st;
     plan_args.maps[m] = d32;
   }
-  if ((st = gather_cell_data(plan_cell.data(), desc.num_cells, plan_args)) != FOT_OK) return st;
+  if ((st = gather_cell_data(plan_cell.data(), uint32_t(plan_cell.size()), plan_args)) != FOT_OK) return st;
+  if (cfg.flags & FOT_LOOP_DEPENDENCY_ORDER) {
+    // immediate lower-colour neighbour of each block through each point: in
+    // colour-major plan order the last block seen on a point is it
+    std::vector<uint32_t> last(desc.num_points[0], 0xffffffffu), dep_off{0}, deps, mine;
+    for (uint32_t b = 0; b < nb_total; ++b) {
+      mine.clear();
+      for (uint32_t i = blk_cache_off[b]; i < blk_cache_off[b + 1]; ++i) {
+        const uint32_t q = last[cached[i]];
+        if (q != 0xffffffffu) mine.push_back(q);
+      }
+      std::sort(mine.begin(), mine.end());
+      mine.erase(std::unique(mine.begin(), mine.end()), mine.end());
+      deps.insert(deps.end(), mine.begin(), mine.end());
+      dep_off.push_back(uint32_t(deps.size()));
+      for (uint32_t i = blk_cache_off[b]; i < blk_cache_off[b + 1]; ++i) last[cached[i]] = b;
+    }
+    if ((st = upload(dep_off.data(), dep_off.size(), &d32)) != FOT_OK) return st;
+    dep.dep_off = d32;
+    if (deps.empty()) deps.push_back(0);
+    if ((st = upload(deps.data(), deps.size(), &d32)) != FOT_OK) return st;
+    dep.deps = d32;
+    void *pf = nullptr;
+    if ((st = alloc(sizeof(uint32_t) * nb_total + sizeof(DepSync), &pf)) != FOT_OK) return st;
+    FOT_HIP(hipMemset(pf, 0, sizeof(uint32_t) * nb_total + sizeof(DepSync)));
+    FOT_HIP(hipDeviceSynchronize());
+    dep.sync = static_cast<DepSync *>(pf);
+    dep.flags = reinterpret_cast<uint32_t *>(static_cast<char *>(pf) + sizeof(DepSync));
+    dep.nb = nb_total;
+    dep_order = true;
+  }
   const uint32_t td = desc.point_dim[0];
   staged = (f.kernel == FOT_KERNEL_RES_CALC) && (cfg.flags & FOT_LOOP_STAGE_INPUTS);
   lds_bytes = uint32_t(sizeof(double)) * max_cached * (staged ? 9u : td);
+  if (owner) {
+    // owned values (+ ids), and for res_calc the staged q / adt of all points
+    lds_bytes = 0;
+    for (size_t b = 0; b + 1 < blk_cache_off.size(); ++b) {
+      const uint32_t m = blk_cache_off[b + 1] - blk_cache_off[b], nw = blk_owned[b];
+      const uint32_t bytes = f.kernel == FOT_KERNEL_RES_CALC
+                                 ? 8u * (4u * nw + 5u * m) + 4u * nw
+                                 : 8u * td * nw + 4u * nw;
+      lds_bytes = std::max(lds_bytes, bytes);
+    }
+  }
   if (lds_bytes > 160u * 1024u)
     return fail(FOT_ERR_UNSUPPORTED, "block cache needs %u B of LDS (max 160 KiB)", lds_bytes);
   uint32_t max_block = 0;
@@ -473,14 +810,15 @@ fot_status fot_loop::build_hierarchical() {
     max_block = std::max(max_block, blk_cell_off[b + 1] - blk_cell_off[b]);
   threads = std::max<uint32_t>(64, (max_block + 63) / 64 * 64);
   if (threads > 1024) return fail(FOT_ERR_UNSUPPORTED, "blocks above 1024 cells");
-  stats.num_colours = uint32_t(plan.colours.size());
+  stats.num_colours = num_plan_colours;
   stats.num_blocks = nb_total;
-  stats.num_launches_per_iter = stats.num_colours;
+  stats.num_launches_per_iter = (dep_order || owner) ? 1u : stats.num_colours;
   stats.max_cached = max_cached;
   stats.lds_bytes = lds_bytes;
   stats.max_thread_colours = max_tc;
   stats.avg_thread_colours = nb_total ? tc_sum / nb_total : 0.0;
   stats.reuse_factor = double(cached.size()) / (double(md) * desc.num_cells);
+  if (owner) stats.redundancy = double(plan_cell.size()) / desc.num_cells;
   return FOT_OK;
 }
 
@@ -499,19 +837,41 @@ fot_status fot_loop::enqueue(hipStream_t s) {
         hipLaunchKernelGGL((eng_direct_kernel<F, SOA, false>), dim3((a.num_cells + bs - 1) / bs),
                            dim3(bs), 0, s, a);
       }
+    } else if (owner) {
+      const uint32_t nb = colour_blk_off.back();
+      if constexpr (std::is_same<F, ResCalcF>::value)
+        hipLaunchKernelGGL((res_calc_owner_kernel<SOA>), dim3(nb), dim3(threads), lds_bytes, s,
+                           plan_args, hier, nb);
+      else
+        hipLaunchKernelGGL((eng_owner_kernel<F, SOA>), dim3(nb), dim3(threads), lds_bytes, s,
+                           plan_args, hier, nb);
+    } else if (dep_order) {
+      // one launch over every plan block
+      const uint32_t nb = colour_blk_off.back();
+      bool done = false;
+      if constexpr (std::is_same<F, ResCalcF>::value) {
+        if (staged) {
+          hipLaunchKernelGGL((res_calc_hier_staged_kernel<SOA, 1>), dim3(nb), dim3(threads),
+                             lds_bytes, s, plan_args, hier, dep, 0u, nb);
+          done = true;
+        }
+      }
+      if (!done)
+        hipLaunchKernelGGL((eng_hier_kernel<F, SOA, 1>), dim3(nb), dim3(threads), lds_bytes, s,
+                           plan_args, hier, dep, 0u, nb);
     } else {
       for (size_t c = 0; c + 1 < colour_blk_off.size(); ++c) {
         const uint32_t b0 = colour_blk_off[c], nbc = colour_blk_off[c + 1] - b0;
         if (nbc == 0) continue;
         if constexpr (std::is_same<F, ResCalcF>::value) {
           if (staged) {
-            hipLaunchKernelGGL((res_calc_hier_staged_kernel<SOA>), dim3(nbc), dim3(threads),
-                               lds_bytes, s, plan_args, hier, b0, nbc);
+            hipLaunchKernelGGL((res_calc_hier_staged_kernel<SOA, 0>), dim3(nbc),
+                               dim3(threads), lds_bytes, s, plan_args, hier, dep, b0, nbc);
             continue;
           }
         }
-        hipLaunchKernelGGL((eng_hier_kernel<F, SOA>), dim3(nbc), dim3(threads), lds_bytes, s,
-                           plan_args, hier, b0, nbc);
+        hipLaunchKernelGGL((eng_hier_kernel<F, SOA, 0>), dim3(nbc), dim3(threads), lds_bytes,
+                           s, plan_args, hier, dep, b0, nbc);
       }
     }
     const hipError_t e = hipGetLastError();
@@ -621,6 +981,16 @@ extern "C" fot_status fot_loop_run(fot_loop *loop, uint32_t iters, void *stream)
       if ((st = fot_copy(in, d.point_data_out, bytes, stream)) != FOT_OK) return st;
   }
   FOT_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return fot_loop_check(loop);
+}
+
+extern "C" fot_status fot_loop_check(const fot_loop *loop) {
+  if (!loop) return fail(FOT_ERR_INVALID_ARGUMENT, "null loop");
+  if (!loop->dep_order) return FOT_OK;
+  DepSync host{};
+  FOT_HIP(hipMemcpy(&host, loop->dep.sync, sizeof(DepSync), hipMemcpyDeviceToHost));
+  if (host.error)
+    return fail(FOT_ERR_TIMEOUT, "a block waited too long for its predecessors");
   return FOT_OK;
 }
 

@@ -41,6 +41,10 @@ void point_cells(const uint32_t *map, uint32_t num_cells, uint32_t dim,
   for (size_t k = 0; k < len; ++k) cells[fill[map[k]]++] = uint32_t(k / dim);
 }
 
+} // namespace
+
+namespace fot {
+
 // In-block thread colouring, colourCellsWithOrdering (colouring.cu:12-102):
 // cell graph with one entry per shared (slot, slot) pair, smallest-last
 // order (minimum degree, the LAST such cell on ties), then first fit in
@@ -123,6 +127,43 @@ int colour_block(const uint32_t *local, uint32_t nc, uint32_t md,
   return ncol;
 }
 
+// Block boundaries of the hierarchical plan (colouring.hpp:70-115): cells
+// stably sorted by partition id, a new block where the id changes or the
+// block reaches block_size; without a partition, consecutive chunks.
+void split_blocks(uint32_t num_cells, const uint32_t *partition, uint32_t block_size,
+                  std::vector<uint32_t> &order, std::vector<uint32_t> &blocks) {
+  order.resize(num_cells);
+  blocks.clear();
+  if (partition) {
+    std::vector<uint64_t> key(num_cells);
+    for (uint32_t i = 0; i < num_cells; ++i) key[i] = (uint64_t(partition[i]) << 32) | i;
+    std::sort(key.begin(), key.end());
+    blocks.push_back(0);
+    order[0] = uint32_t(key[0]);
+    uint32_t tid = 0;
+    for (uint32_t i = 1; i < num_cells; ++i) {
+      order[i] = uint32_t(key[i]);
+      if (++tid == block_size) {
+        tid = 0;
+        blocks.push_back(i);
+        continue;
+      }
+      if ((key[i - 1] >> 32) != (key[i] >> 32)) {
+        tid = 0;
+        blocks.push_back(i);
+      }
+    }
+  } else {
+    std::iota(order.begin(), order.end(), 0u);
+    for (uint32_t i = 0; i < num_cells; i += block_size) blocks.push_back(i);
+  }
+  blocks.push_back(num_cells);
+}
+
+} // namespace fot
+
+namespace {
+using fot::colour_block;
 } // namespace
 
 extern "C" fot_status fot_colour_cells(const uint32_t *c2n, uint32_t num_cells,
@@ -192,32 +233,8 @@ fot_status build_hplan(const uint32_t *const *c2n, const uint32_t *mesh_dim,
   out.num_cells = num_cells;
   for (uint32_t m = 0; m < num_mappings; ++m) out.mesh_dim[m] = mesh_dim[m];
   // block boundaries over positions of `order`
-  std::vector<uint32_t> order(num_cells), blocks;
-  if (partition) {
-    std::vector<uint64_t> key(num_cells);
-    for (uint32_t i = 0; i < num_cells; ++i)
-      key[i] = (uint64_t(partition[i]) << 32) | i;
-    std::sort(key.begin(), key.end());
-    blocks.push_back(0);
-    order[0] = uint32_t(key[0]);
-    uint32_t tid = 0;
-    for (uint32_t i = 1; i < num_cells; ++i) {
-      order[i] = uint32_t(key[i]);
-      if (++tid == block_size) {
-        tid = 0;
-        blocks.push_back(i);
-        continue;
-      }
-      if ((key[i - 1] >> 32) != (key[i] >> 32)) {
-        tid = 0;
-        blocks.push_back(i);
-      }
-    }
-  } else {
-    std::iota(order.begin(), order.end(), 0u);
-    for (uint32_t i = 0; i < num_cells; i += block_size) blocks.push_back(i);
-  }
-  blocks.push_back(num_cells);
+  std::vector<uint32_t> order, blocks;
+  split_blocks(num_cells, partition, block_size, order, blocks);
   for (uint32_t m = 0; m < num_mappings; ++m)
     for (size_t k = 0; k < size_t(num_cells) * mesh_dim[m]; ++k)
       if (m == 0 && c2n[0][k] >= num_points0)

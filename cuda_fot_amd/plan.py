@@ -71,6 +71,34 @@ def hplan(maps, num_points0: int, block_size: int, partition=None) -> list[dict]
         lib.fot_hplan_free(h)
 
 
+def owner_plan(maps0, num_points: int, block_size: int, partition=None,
+               point_partition=None) -> dict:
+    """Owner-compute plan (fot_oplan_build) as a dict of numpy arrays:
+    owners from point_partition (one part id per mapping-0 point) when given,
+    else from the cell split (partition per cell, or chunks)."""
+    m = u32(maps0)
+    part = None if partition is None else u32(partition)
+    ppart = None if point_partition is None else u32(point_partition)
+    h = C.c_void_p()
+    call("fot_oplan_build", ptr(m), m.shape[0], m.shape[1], num_points, ptr(part), block_size,
+         ptr(ppart), C.byref(h))
+    try:
+        v = capi.OplanView()
+        call("fot_oplan_get", h, C.byref(v))
+        nb, npos, nc = v.num_blocks, v.num_positions, v.num_cached
+
+        def cp(p, n):
+            return np.ctypeslib.as_array(p, shape=(n,)).copy()
+
+        return {"mesh_dim": v.mesh_dim, "blk_cell_off": cp(v.blk_cell_off, nb + 1),
+                "blk_cache_off": cp(v.blk_cache_off, nb + 1), "blk_owned": cp(v.blk_owned, nb),
+                "cached": cp(v.cached, nc), "cells": cp(v.cells, npos),
+                "loc": cp(v.loc, npos * v.mesh_dim), "colour": cp(v.colour, npos),
+                "blk_ncol": cp(v.blk_ncol, nb)}
+    finally:
+        lib.fot_oplan_free(h)
+
+
 def reorder_to_partition(partition) -> tuple[np.ndarray, np.ndarray]:
     """Mesh::reorderToPartition (mesh.hpp:227-245): (sorted partition, inverse
     cell permutation)."""
@@ -176,6 +204,13 @@ def partition_metis_enh(maps, num_points, block_size: int, tolerance: float = 1.
     ind, end = cell_to_cell_csr(maps, num_points)
     max_part = int(np.floor(block_size / tolerance))
     return kway_partition(ind, end, max_part)[0]
+
+
+def partition_points(maps0, num_points: int, points_per_part: int) -> np.ndarray:
+    """Parts of at most points_per_part points of mapping 0's point graph
+    (the owner partition of the owner-compute plan)."""
+    ind, end = graph_csr(maps0, num_points)
+    return kway_partition(ind, end, points_per_part)[0]
 
 
 def partition_ours(maps, num_points, block_size: int, num_blocks_in_coarse: int,

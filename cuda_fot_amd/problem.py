@@ -43,6 +43,7 @@ class Problem:
         self.block_size = int(block_size)
         self.soa = bool(soa)
         self.partition_vector: np.ndarray | None = None
+        self.point_partition: np.ndarray | None = None
         self.applied_permutation: np.ndarray | None = None
 
     @property
@@ -86,6 +87,33 @@ class Problem:
         """Problem::partitionOurs (problem.hpp:501-515)."""
         self.partition_vector = plan.partition_ours(self.maps, self.num_points, self.block_size,
                                                     num_blocks_in_coarse, tolerance)
+
+    def partition_points(self, points_per_block: int) -> None:
+        """Owner partition for the owner-compute loop: parts of at most
+        points_per_block mapping-0 points, then the mesh renumbered so every
+        part's points are contiguous (mappings identical to mapping 0 follow),
+        cells sorted by the part of their first point, and the other mappings
+        renumbered by first touch."""
+        part = plan.partition_points(self.maps[0], self.num_points[0], points_per_block)
+        order = np.lexsort((np.arange(part.size), part)).astype(np.uint32)
+        perm = np.empty_like(order)
+        perm[order] = np.arange(order.size, dtype=np.uint32)
+        same = [k for k in range(1, len(self.maps)) if np.array_equal(self.maps[k], self.maps[0])]
+        for k in [0] + same:
+            self.maps[k] = perm[self.maps[k]]
+            self._reorder_point(k, perm)
+        self._compose(perm)
+        self.point_partition = part[order]
+        cell_key = self.point_partition[self.maps[0][:, 0]]
+        inv = np.lexsort((np.arange(cell_key.size), cell_key)).astype(np.uint32)
+        self.maps = [m[inv] for m in self.maps]
+        self.cell_data = [reorder_data_inverse_cells(c, inv) for c in self.cell_data]
+        for k in range(1, len(self.maps)):
+            if k in same:
+                continue
+            p = plan.point_renumbering_first_touch(self.maps[k], self.num_points[k])
+            self.maps[k] = p[self.maps[k]]
+            self._reorder_point(k, p)
 
     def reorder_to_partition(self) -> None:
         """Problem::reorderToPartition (problem.hpp:517-526)."""
@@ -131,6 +159,9 @@ class Problem:
         out = DeviceArray(self.point_data[0].shape, np.float64)
         cdev = [DeviceArray.from_host(c) for c in self.cell_data]
         part = self.partition_vector if (partition and strategy == capi.STRATEGY_HIERARCHICAL) else None
+        if (flags & capi.LOOP_OWNER_COMPUTE) and partition and self.point_partition is not None:
+            part = self.point_partition
+            flags |= capi.LOOP_POINT_PARTITION
         loop = Loop(func, self.maps, self.num_points, dev, out, cdev, soa=self.soa,
                     strategy=strategy, block_size=block_size or self.block_size,
                     partition=part, use_graph=use_graph, flags=flags)

@@ -235,6 +235,38 @@ typedef struct {
 fot_status fot_hplan_colour_get(const fot_hplan *plan, uint32_t colour,
                                 fot_hplan_colour *out);
 
+/* Owner-compute plan (no reference counterpart): each mapping-0 point is
+ * owned by one block -- its part in point_partition when given; else, with a
+ * per-cell partition, the first block of the hierarchical cell split whose
+ * cells touch it; with neither, its part in a partition of mapping 0's point
+ * graph into parts of at most block_size points.  Blocks running more than
+ * 1024 cells split their owned points.  A block runs every cell touching an owned
+ * point, in thread colours that separate cells sharing an owned point, and
+ * writes back only its owned points: its LDS cache starts from the owned
+ * points' current values, takes the increments colour by colour and is
+ * stored back.  Blocks never share a written point, so one launch needs no
+ * block colouring and the result does not depend on block order. */
+typedef struct fot_oplan fot_oplan;
+typedef struct {
+  uint32_t mesh_dim, num_blocks, num_positions, num_cached;
+  const uint32_t *blk_cell_off;  /* num_blocks + 1 */
+  const uint32_t *blk_cache_off; /* num_blocks + 1 */
+  const uint32_t *blk_owned;     /* num_blocks: owned points (prefix) */
+  const uint32_t *cached;        /* owned ascending, then halo ascending */
+  const uint32_t *cells;         /* original cell per plan position */
+  const uint32_t *loc;           /* num_positions * mesh_dim local ids */
+  const uint8_t *colour;         /* num_positions */
+  const uint8_t *blk_ncol;       /* num_blocks */
+} fot_oplan_view;
+fot_status fot_oplan_build(const uint32_t *cell_to_node0, uint32_t num_cells,
+                           uint32_t mesh_dim, uint32_t num_points,
+                           const uint32_t *partition /* nullable, per cell */,
+                           uint32_t block_size,
+                           const uint32_t *point_partition /* nullable, per point */,
+                           fot_oplan **plan);
+fot_status fot_oplan_get(const fot_oplan *plan, fot_oplan_view *view);
+void fot_oplan_free(fot_oplan *plan);
+
 /* Mesh::reorderToPartition (mesh.hpp:227-245): sorts partition in place,
  * writes the inverse cell permutation (new cell i = old inv_perm[i]). */
 fot_status fot_reorder_to_partition(uint32_t *partition, uint32_t num_cells,
@@ -375,6 +407,11 @@ typedef struct {
 /* hierarchical: single launch, blocks ordered by per-block dependency
  * flags instead of one launch per block colour */
 #define FOT_LOOP_DEPENDENCY_ORDER 2u
+/* hierarchical: owner-compute plan (fot_oplan_build), one launch, no block
+ * colours.  With FOT_LOOP_POINT_PARTITION, `partition` holds a part id per
+ * mapping-0 point; ownership as fot_oplan_build defines it. */
+#define FOT_LOOP_OWNER_COMPUTE 4u
+#define FOT_LOOP_POINT_PARTITION 8u
 
 typedef struct fot_loop fot_loop;
 /* Builds the plan (host), uploads it, permutes cell data. */
@@ -386,12 +423,16 @@ fot_status fot_loop_step(fot_loop *loop, void *stream);
 /* `iters` iterations with the reference's semantics: out starts as a copy of
  * point_data[0]; after each iteration point_data[0] = out. */
 fot_status fot_loop_run(fot_loop *loop, uint32_t iters, void *stream);
+/* Synchronises and reports FOT_ERR_TIMEOUT if a dependency-ordered launch
+ * ran out of its bounded wait (no reference counterpart). */
+fot_status fot_loop_check(const fot_loop *loop);
 
 typedef struct {
   uint32_t strategy, num_launches_per_iter, num_colours, num_blocks;
   uint32_t block_size, max_cached, lds_bytes, max_thread_colours;
   uint64_t plan_device_bytes;
   double avg_thread_colours, reuse_factor;
+  double redundancy; /* owner compute: cell runs per cell (>= 1) */
 } fot_loop_stats;
 fot_status fot_loop_stats_get(const fot_loop *loop, fot_loop_stats *stats);
 

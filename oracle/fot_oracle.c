@@ -372,6 +372,60 @@ int oracle_loop_hier_replay(const oracle_problem *p, const oracle_hplan *h,
   return 0;
 }
 
+int oracle_loop_owner_replay(const oracle_problem *p, const oracle_oplan *o,
+                             uint32_t iters) {
+  const size_t len = target_len(p);
+  const uint32_t md = p->mesh_dim[0], P = p->point_dim[0];
+  const uint32_t n = p->num_points[0];
+  double *temp = (double *)malloc(len * sizeof(double));
+  uint32_t maxo = 1;
+  double inc[64];
+  if (!temp || md != o->mesh_dim) {
+    free(temp);
+    return -1;
+  }
+  for (uint32_t b = 0; b < o->num_blocks; ++b)
+    if (o->blk_owned[b] > maxo) maxo = o->blk_owned[b];
+  double *cache = (double *)malloc(sizeof(double) * (size_t)maxo * P);
+  if (!cache) {
+    free(temp);
+    return -1;
+  }
+  memcpy(temp, p->point_data[0], len * sizeof(double));
+  for (uint32_t it = 0; it < iters; ++it) {
+    for (uint32_t b = 0; b < o->num_blocks; ++b) {
+      const uint32_t no = o->blk_owned[b], c0 = o->blk_cache_off[b];
+      for (uint32_t i = 0; i < no; ++i)
+        for (uint32_t d = 0; d < P; ++d)
+          cache[(size_t)d * no + i] = temp[pidx(p->soa, n, o->cached[c0 + i], P, d)];
+      for (uint32_t tc = 0; tc < o->blk_ncol[b]; ++tc)
+        for (uint32_t pos = o->blk_cell_off[b]; pos < o->blk_cell_off[b + 1]; ++pos) {
+          if (o->colour[pos] != tc) continue;
+          compute_inc(p, o->cells[pos], inc);
+          const uint32_t *l = o->loc + (size_t)pos * md;
+          if (p->kernel == ORACLE_HEX_SCATTER) {
+            for (uint32_t k = 0; k < md; ++k)
+              for (uint32_t d = 0; d < P; ++d)
+                if (l[k] < no) cache[(size_t)d * no + l[k]] += inc[k * P + d];
+          } else {
+            for (uint32_t d = 0; d < P; ++d)
+              for (uint32_t k = 0; k < md; ++k)
+                if (l[k] < no) cache[(size_t)d * no + l[k]] += inc[k * P + d];
+          }
+        }
+      for (uint32_t i = 0; i < no; ++i) {
+        const uint32_t g = o->cached[c0 + i];
+        for (uint32_t d = 0; d < P; ++d)
+          temp[pidx(p->soa, n, g, P, d)] = cache[(size_t)d * no + i];
+      }
+    }
+    memcpy(p->point_data[0], temp, len * sizeof(double));
+  }
+  free(cache);
+  free(temp);
+  return 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Planning                                                                  */
 /* ------------------------------------------------------------------------ */

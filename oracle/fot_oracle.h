@@ -90,6 +90,21 @@ struct oracle_hplan;
 int oracle_loop_hier_replay(const oracle_problem *p,
                             const struct oracle_hplan *plan, uint32_t iters);
 
+/* Owner-compute plan as libfot exports it (fot_oplan_view): blocks own
+ * mapping-0 points; every cell touching an owned point runs in the block. */
+typedef struct {
+  uint32_t mesh_dim, num_blocks;
+  const uint32_t *blk_cell_off, *blk_cache_off, *blk_owned, *cached, *cells, *loc;
+  const uint8_t *colour, *blk_ncol;
+} oracle_oplan;
+
+/* Replays the summation order of an owner-compute plan: per block a cache
+ * holding its owned points' current values takes the increments of its
+ * cells thread colour by thread colour (halo slots dropped), then replaces
+ * those values. */
+int oracle_loop_owner_replay(const oracle_problem *p, const oracle_oplan *plan,
+                             uint32_t iters);
+
 /* ---- planning ---------------------------------------------------------- */
 /* Mesh::colourCells (mesh.hpp:117-162): greedy colouring on mapping 0,
  * least-used available colour among 64.  colour_out[num_cells].  Returns the

@@ -61,6 +61,7 @@ def _load():
         "oracle_loop_serial": [V, U, V, V],
         "oracle_loop_colour": [V, U, C.c_int, V],
         "oracle_loop_hier_replay": [V, V, U],
+        "oracle_loop_owner_replay": [V, V, U],
         "oracle_colour_cells": [V, U, U, U, V],
         "oracle_colour_cells_with_ordering": [V, U, U, V],
         "oracle_hplan_build": [V, V, U, U, U, V, U, V],
@@ -155,6 +156,27 @@ class OracleProblem:
     def loop_hier_replay(self, plan: "OraclePlan", iters: int) -> None:
         if lib.oracle_loop_hier_replay(C.byref(self.s), C.byref(plan.h), iters) != 0:
             raise RuntimeError("oracle_loop_hier_replay failed")
+
+
+class OwnerPlanView(C.Structure):
+    _fields_ = [("mesh_dim", C.c_uint32), ("num_blocks", C.c_uint32)] + [
+        (n, C.c_void_p) for n in ("blk_cell_off", "blk_cache_off", "blk_owned", "cached",
+                                  "cells", "loc", "colour", "blk_ncol")]
+
+
+def loop_owner_replay(problem: "OracleProblem", plan: dict, iters: int) -> None:
+    """Replays an owner-compute plan (dict of numpy arrays as
+    cuda_fot_amd.plan.owner_plan returns it)."""
+    arrs = {k: np.ascontiguousarray(plan[k]) for k in
+            ("blk_cell_off", "blk_cache_off", "blk_owned", "cached", "cells", "loc",
+             "colour", "blk_ncol")}
+    v = OwnerPlanView()
+    v.mesh_dim = int(plan["mesh_dim"])
+    v.num_blocks = int(arrs["blk_owned"].size)
+    for k, a in arrs.items():
+        setattr(v, k, _p(a))
+    if lib.oracle_loop_owner_replay(C.byref(problem.s), C.byref(v), iters) != 0:
+        raise RuntimeError("oracle_loop_owner_replay failed")
 
 
 class OraclePlan:

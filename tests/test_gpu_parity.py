@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from cuda_fot_amd import capi
+from cuda_fot_amd import capi, plan
 from helpers import (airfoil_problem, assert_bitwise, assert_close_scaled, hex_problem,
                      mine_problem, oracle_of)
 
@@ -56,10 +56,57 @@ def test_res_calc_global_colour_bitwise(gpu, soa):
     assert_bitwise(p.point_data[0], ref, "global colour")
 
 
+DEP = capi.LOOP_DEPENDENCY_ORDER
+STAGE = capi.LOOP_STAGE_INPUTS
+OWN = capi.LOOP_OWNER_COMPUTE
+
+
+def _owner_replay(p, kernel, block_size, partition=None, iters=ITERS, point_partition=None):
+    o = oracle_of(p, kernel)
+    O.loop_owner_replay(o, plan.owner_plan(p.maps[0], p.num_points[0], block_size, partition,
+                                           point_partition), iters)
+    return o.target.copy()
+
+
+@pytest.mark.parametrize("soa,bs,flags,method", [(False, 128, OWN, None),
+                                                 (True, 256, OWN | STAGE, None),
+                                                 (False, 512, OWN | STAGE, "kway"),
+                                                 (False, 256, OWN, "ours"),
+                                                 (True, 1024, OWN | STAGE, "kway"),
+                                                 (False, 64, OWN, "points"),
+                                                 (True, 96, OWN, "points")])
+def test_res_calc_owner_compute_bitwise(gpu, soa, bs, flags, method):
+    p = airfoil_problem(nx=64, ny=32, soa=soa, res0_seed=10, block_size=bs)
+    orig = airfoil_problem(nx=64, ny=32, soa=soa, res0_seed=10, block_size=bs)
+    if method == "kway":
+        p.reorder("gps")
+        p.partition(1.001)
+    elif method == "ours":
+        p.partition_ours(1.001, 4)
+    if method in ("kway", "ours"):
+        p.reorder_to_partition()
+        p.renumber_points()
+    if method == "points":
+        p.partition_points(bs)
+    ref = _owner_replay(p, O.RES_CALC, bs, p.partition_vector, point_partition=p.point_partition)
+    serial, S = _serial(orig, O.RES_CALC)
+    p.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, ITERS, flags=flags)
+    assert_bitwise(p.point_data[0], ref, "owner compute")
+    assert_close_scaled(p.target_in_original_order(), serial, S, what="owner compute vs serial")
+
+
+def test_res_calc_owner_engine_partition(gpu):
+    # owner compute without a partition: the engine partitions the points
+    p = airfoil_problem(nx=64, ny=32, res0_seed=11, block_size=128)
+    serial, S = _serial(p, O.RES_CALC)
+    p.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, ITERS, flags=OWN, partition=False)
+    assert_close_scaled(p.point_data[0], serial, S, what="owner compute, engine partition")
+
+
 @pytest.mark.parametrize("soa,bs,flags", [(False, 128, 0), (True, 128, 0),
-                                          (False, 256, capi.LOOP_STAGE_INPUTS),
-                                          (True, 64, capi.LOOP_STAGE_INPUTS),
-                                          (False, 1024, capi.LOOP_STAGE_INPUTS)])
+                                          (False, 256, STAGE), (True, 64, STAGE),
+                                          (False, 1024, STAGE), (False, 128, DEP),
+                                          (True, 256, DEP | STAGE), (False, 512, DEP | STAGE)])
 def test_res_calc_hierarchical_bitwise(gpu, soa, bs, flags):
     p = airfoil_problem(soa=soa, res0_seed=5, block_size=bs)
     ref = _replay(p, O.RES_CALC, bs)
@@ -69,8 +116,9 @@ def test_res_calc_hierarchical_bitwise(gpu, soa, bs, flags):
     assert_close_scaled(p.point_data[0], serial, S, what="hierarchical vs serial")
 
 
-@pytest.mark.parametrize("method", ["kway", "ours"])
-def test_res_calc_partitioned_renumbered(gpu, method):
+@pytest.mark.parametrize("method,flags", [("kway", STAGE), ("ours", STAGE),
+                                          ("kway", STAGE | DEP), ("ours", DEP)])
+def test_res_calc_partitioned_renumbered(gpu, method, flags):
     p = airfoil_problem(nx=64, ny=32, res0_seed=6, block_size=256)
     orig = airfoil_problem(nx=64, ny=32, res0_seed=6, block_size=256)
     if method == "kway":
@@ -81,20 +129,23 @@ def test_res_calc_partitioned_renumbered(gpu, method):
     p.reorder_to_partition()
     p.renumber_points()
     ref = _replay(p, O.RES_CALC, 256, p.partition_vector)
-    p.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, ITERS, flags=capi.LOOP_STAGE_INPUTS)
+    p.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, ITERS, flags=flags)
     assert_bitwise(p.point_data[0], ref, "partitioned hierarchical")
     # reordering invariance (res_calc.cu:129-181), through applied_permutation
     serial, S = _serial(orig, O.RES_CALC)
     assert_close_scaled(p.target_in_original_order(), serial, S, what="reordered vs original")
 
 
-def test_res_calc_graph_replay_matches_eager(gpu):
+@pytest.mark.parametrize("flags", [STAGE, STAGE | DEP])
+def test_res_calc_graph_replay_matches_eager(gpu, flags):
+    # several replays of one captured iteration, host work in between
     p1 = airfoil_problem(res0_seed=7)
     p2 = airfoil_problem(res0_seed=7)
-    p1.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, 5, flags=capi.LOOP_STAGE_INPUTS)
-    p2.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, 5, flags=capi.LOOP_STAGE_INPUTS,
-                use_graph=True)
+    p1.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, 6, flags=flags)
+    p2.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, 6, flags=flags, use_graph=True)
     assert_bitwise(p2.point_data[0], p1.point_data[0], "graph vs eager")
+    ref = _replay(airfoil_problem(res0_seed=7), O.RES_CALC, 128, None, iters=6)
+    assert_bitwise(p2.point_data[0], ref, "graph vs replay")
 
 
 def test_res_calc_atomic_run_to_run(gpu):
@@ -116,9 +167,14 @@ def test_mine_strategies(gpu, P, C, soa):
     g = mine_problem(P=P, C=C, soa=soa)
     g.gpu_loop(f, capi.STRATEGY_GLOBAL_COLOUR, ITERS)
     assert_bitwise(g.point_data[0], ref_colour, "mine global colour")
-    h = mine_problem(P=P, C=C, soa=soa)
-    h.gpu_loop(f, capi.STRATEGY_HIERARCHICAL, ITERS)
-    assert_bitwise(h.point_data[0], ref_hier, "mine hierarchical")
+    for flags in (0, DEP):
+        h = mine_problem(P=P, C=C, soa=soa)
+        h.gpu_loop(f, capi.STRATEGY_HIERARCHICAL, ITERS, flags=flags)
+        assert_bitwise(h.point_data[0], ref_hier, f"mine hierarchical flags {flags}")
+    ref_own = _owner_replay(mine_problem(P=P, C=C, soa=soa), O.MINE2, 128)
+    w = mine_problem(P=P, C=C, soa=soa)
+    w.gpu_loop(f, capi.STRATEGY_HIERARCHICAL, ITERS, flags=OWN)
+    assert_bitwise(w.point_data[0], ref_own, "mine owner compute")
     a = mine_problem(P=P, C=C, soa=soa)
     a.gpu_loop(f, capi.STRATEGY_ATOMIC, ITERS)
     assert_close_scaled(a.point_data[0], ref_serial, S, what="mine atomic")
@@ -130,11 +186,14 @@ def test_hex_scatter_strategies(gpu, soa):
     ref_serial, S = _serial(hex_problem(soa=soa), O.HEX_SCATTER)
     ref_hier = _replay(hex_problem(soa=soa), O.HEX_SCATTER, 128)
     ref_colour = _colour(hex_problem(soa=soa), O.HEX_SCATTER)
-    for strategy, ref, exact in [(capi.STRATEGY_HIERARCHICAL, ref_hier, True),
-                                 (capi.STRATEGY_GLOBAL_COLOUR, ref_colour, True),
-                                 (capi.STRATEGY_ATOMIC, ref_serial, False)]:
+    ref_own = _owner_replay(hex_problem(soa=soa), O.HEX_SCATTER, 128)
+    for strategy, ref, exact, flags in [(capi.STRATEGY_HIERARCHICAL, ref_hier, True, 0),
+                                        (capi.STRATEGY_HIERARCHICAL, ref_own, True, OWN),
+                                        (capi.STRATEGY_HIERARCHICAL, ref_hier, True, DEP),
+                                        (capi.STRATEGY_GLOBAL_COLOUR, ref_colour, True, 0),
+                                        (capi.STRATEGY_ATOMIC, ref_serial, False, 0)]:
         p = hex_problem(soa=soa)
-        p.gpu_loop(f, strategy, ITERS)
+        p.gpu_loop(f, strategy, ITERS, flags=flags)
         if exact:
             assert_bitwise(p.point_data[0], ref, f"hex strategy {strategy}")
         else:
