@@ -46,6 +46,7 @@ LOOP_STAGE_INPUTS = 1
 LOOP_DEPENDENCY_ORDER = 2
 LOOP_OWNER_COMPUTE = 4
 LOOP_POINT_PARTITION = 8
+LOOP_TWO_PER_THREAD = 16
 
 
 class FotError(RuntimeError):

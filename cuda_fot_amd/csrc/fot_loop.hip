@@ -393,31 +393,46 @@ __global__ __launch_bounds__(1024) void eng_owner_kernel(EngineArgs a, HierDev h
   }
 }
 
-// res_calc owner compute: q and adt of owned and halo cells staged in LDS
-// next to the owned res values; one packed word per edge holds both local
-// ids (12 bits each) and the thread colour (8 bits).
-template <bool SOA>
-__global__ __launch_bounds__(1024) void res_calc_owner_kernel(EngineArgs a, HierDev h,
-                                                              uint32_t nb) {
+// Per-block record of the owner-compute res_calc kernel: one scalar load.
+struct OwnerMeta {
+  uint32_t cell_off, ncells, cache_off, ncached;
+  uint32_t nowned, own_start, ncol, contiguous; // owned ids own_start + i when contiguous
+};
+
+// res_calc owner compute, latency-oriented: block record in one scalar load,
+// owned points addressed directly when the mesh numbers them contiguously,
+// EPT edges per thread so each wave keeps more gathers in flight.
+template <bool SOA, int EPT>
+__global__ __launch_bounds__(1024) void res_calc_owner2_kernel(EngineArgs a,
+                                                               const OwnerMeta *__restrict__ meta,
+                                                               const uint32_t *__restrict__ cached,
+                                                               const uint32_t *__restrict__ packed,
+                                                               uint32_t nb) {
   extern __shared__ double lds[];
   const uint32_t b = xcd_block(blockIdx.x, nb);
-  const uint32_t tid = threadIdx.x;
-  const uint32_t e0 = h.blk_cell_off[b], ne = h.blk_cell_off[b + 1] - e0;
-  const uint32_t c0 = h.blk_cache_off[b], m = h.blk_cache_off[b + 1] - c0;
-  const uint32_t nw = h.blk_owned[b];
+  const OwnerMeta mt = meta[b];
+  const uint32_t tid = threadIdx.x, bd = blockDim.x;
+  const uint32_t m = mt.ncached, nw = mt.nowned, ne = mt.ncells;
+  const bool contig = mt.contiguous != 0;
   double *res_s = lds, *q_s = lds + 4 * nw, *adt_s = q_s + 4 * m;
   uint32_t *g_s = reinterpret_cast<uint32_t *>(adt_s + m);
   const double *x = a.point_data[1], *q = a.point_data[2], *adt = a.point_data[3];
   const uint32_t np = a.num_points[0];
-  // edge words first: their x gathers overlap the staging below
-  uint32_t pk = 0xff000000u;
-  uint2 nn = make_uint2(0, 0);
-  if (tid < ne) {
-    pk = h.packed[e0 + tid];
-    nn = reinterpret_cast<const uint2 *>(a.maps[1])[e0 + tid];
+  const uint32_t *pcached = cached + mt.cache_off;
+  uint32_t pk[EPT];
+  uint2 nn[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const uint32_t t = tid + e * bd;
+    pk[e] = 0xff000000u;
+    nn[e] = make_uint2(0, 0);
+    if (t < ne) {
+      pk[e] = packed[mt.cell_off + t];
+      nn[e] = reinterpret_cast<const uint2 *>(a.maps[1])[mt.cell_off + t];
+    }
   }
-  for (uint32_t i = tid; i < m; i += blockDim.x) {
-    const uint32_t g = h.cached[c0 + i];
+  for (uint32_t i = tid; i < m; i += bd) {
+    const uint32_t g = (contig && i < nw) ? mt.own_start + i : pcached[i];
     if (SOA) {
       const uint32_t nq = a.num_points[2];
 #pragma unroll
@@ -438,42 +453,53 @@ __global__ __launch_bounds__(1024) void res_calc_owner_kernel(EngineArgs a, Hier
       }
     }
     adt_s[i] = adt[g];
-    if (i < nw) g_s[i] = g;
+    if (!contig && i < nw) g_s[i] = g;
   }
-  double x1[2] = {0, 0}, x2[2] = {0, 0};
-  if (tid < ne) {
-    if (SOA) {
-      const uint32_t nx = a.num_points[1];
-      x1[0] = x[nn.x], x1[1] = x[nx + nn.x], x2[0] = x[nn.y], x2[1] = x[nx + nn.y];
-    } else {
-      const double2 xa = reinterpret_cast<const double2 *>(x)[nn.x];
-      const double2 xb = reinterpret_cast<const double2 *>(x)[nn.y];
-      x1[0] = xa.x, x1[1] = xa.y, x2[0] = xb.x, x2[1] = xb.y;
+  double x1[EPT][2], x2[EPT][2];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    x1[e][0] = x1[e][1] = x2[e][0] = x2[e][1] = 0.0;
+    if (tid + e * bd < ne) {
+      if (SOA) {
+        const uint32_t nx = a.num_points[1];
+        x1[e][0] = x[nn[e].x], x1[e][1] = x[nx + nn[e].x];
+        x2[e][0] = x[nn[e].y], x2[e][1] = x[nx + nn[e].y];
+      } else {
+        const double2 xa = reinterpret_cast<const double2 *>(x)[nn[e].x];
+        const double2 xb = reinterpret_cast<const double2 *>(x)[nn[e].y];
+        x1[e][0] = xa.x, x1[e][1] = xa.y, x2[e][0] = xb.x, x2[e][1] = xb.y;
+      }
     }
   }
   __syncthreads();
-  const uint32_t l0 = pk & 0xfffu, l1 = (pk >> 12) & 0xfffu, colour = pk >> 24;
-  double inc[8];
-  if (tid < ne) {
-    double q1[4], q2[4];
+  double inc[EPT][8];
 #pragma unroll
-    for (uint32_t d = 0; d < 4; ++d) q1[d] = q_s[d * m + l0], q2[d] = q_s[d * m + l1];
-    const double a1 = adt_s[l0], a2 = adt_s[l1];
-    res_calc_user(x1, x2, q1, q2, &a1, &a2, inc, inc + 4, 1, 1);
+  for (int e = 0; e < EPT; ++e) {
+    if (tid + e * bd < ne) {
+      const uint32_t l0 = pk[e] & 0xfffu, l1 = (pk[e] >> 12) & 0xfffu;
+      double q1[4], q2[4];
+#pragma unroll
+      for (uint32_t d = 0; d < 4; ++d) q1[d] = q_s[d * m + l0], q2[d] = q_s[d * m + l1];
+      const double a1 = adt_s[l0], a2 = adt_s[l1];
+      res_calc_user(x1[e], x2[e], q1, q2, &a1, &a2, inc[e], inc[e] + 4, 1, 1);
+    }
   }
-  const uint32_t ncol = h.blk_ncol[b];
-  for (uint32_t c = 0; c < ncol; ++c) {
-    if (colour == c) {
+  for (uint32_t c = 0; c < mt.ncol; ++c) {
 #pragma unroll
-      for (uint32_t d = 0; d < 4; ++d) {
-        if (l0 < nw) res_s[d * nw + l0] += inc[d];
-        if (l1 < nw) res_s[d * nw + l1] += inc[4 + d];
+    for (int e = 0; e < EPT; ++e) {
+      if ((pk[e] >> 24) == c) {
+        const uint32_t l0 = pk[e] & 0xfffu, l1 = (pk[e] >> 12) & 0xfffu;
+#pragma unroll
+        for (uint32_t d = 0; d < 4; ++d) {
+          if (l0 < nw) res_s[d * nw + l0] += inc[e][d];
+          if (l1 < nw) res_s[d * nw + l1] += inc[e][4 + d];
+        }
       }
     }
     __syncthreads();
   }
-  for (uint32_t i = tid; i < nw; i += blockDim.x) {
-    const uint32_t g = g_s[i];
+  for (uint32_t i = tid; i < nw; i += bd) {
+    const uint32_t g = contig ? mt.own_start + i : g_s[i];
     if (SOA) {
 #pragma unroll
       for (uint32_t d = 0; d < 4; ++d) a.out[size_t(d) * np + g] = res_s[d * nw + i];
@@ -522,6 +548,8 @@ struct fot_loop {
   bool staged = false;
   bool dep_order = false;
   bool owner = false;
+  const OwnerMeta *owner_meta = nullptr;
+  uint32_t ept = 1;
   DepDev dep{};
   // graph of one iteration
   hipGraph_t graph = nullptr;
@@ -750,6 +778,26 @@ fot_status fot_loop::build_hierarchical() {
       }
       if ((st = upload(packed.data(), packed.size(), &d32)) != FOT_OK) return st;
       hier.packed = d32;
+      std::vector<OwnerMeta> meta(blk_owned.size());
+      for (size_t b = 0; b < meta.size(); ++b) {
+        OwnerMeta &mt = meta[b];
+        mt.cell_off = blk_cell_off[b];
+        mt.ncells = blk_cell_off[b + 1] - blk_cell_off[b];
+        mt.cache_off = blk_cache_off[b];
+        mt.ncached = blk_cache_off[b + 1] - blk_cache_off[b];
+        mt.nowned = blk_owned[b];
+        mt.own_start = mt.nowned ? cached[mt.cache_off] : 0;
+        mt.ncol = blk_ncol[b];
+        mt.contiguous = 1;
+        for (uint32_t i = 1; i < mt.nowned; ++i)
+          if (cached[mt.cache_off + i] != mt.own_start + i) {
+            mt.contiguous = 0;
+            break;
+          }
+      }
+      OwnerMeta *dm = nullptr;
+      if ((st = upload(meta.data(), meta.size(), &dm)) != FOT_OK) return st;
+      owner_meta = dm;
     }
   }
   plan_args = base;
@@ -808,7 +856,8 @@ fot_status fot_loop::build_hierarchical() {
   uint32_t max_block = 0;
   for (size_t b = 0; b + 1 < blk_cell_off.size(); ++b)
     max_block = std::max(max_block, blk_cell_off[b + 1] - blk_cell_off[b]);
-  threads = std::max<uint32_t>(64, (max_block + 63) / 64 * 64);
+  ept = (owner && f.kernel == FOT_KERNEL_RES_CALC && (cfg.flags & FOT_LOOP_TWO_PER_THREAD)) ? 2 : 1;
+  threads = std::max<uint32_t>(64, ((max_block + ept - 1) / ept + 63) / 64 * 64);
   if (threads > 1024) return fail(FOT_ERR_UNSUPPORTED, "blocks above 1024 cells");
   stats.num_colours = num_plan_colours;
   stats.num_blocks = nb_total;
@@ -839,10 +888,14 @@ fot_status fot_loop::enqueue(hipStream_t s) {
       }
     } else if (owner) {
       const uint32_t nb = colour_blk_off.back();
-      if constexpr (std::is_same<F, ResCalcF>::value)
-        hipLaunchKernelGGL((res_calc_owner_kernel<SOA>), dim3(nb), dim3(threads), lds_bytes, s,
-                           plan_args, hier, nb);
-      else
+      if constexpr (std::is_same<F, ResCalcF>::value) {
+        if (ept == 2)
+          hipLaunchKernelGGL((res_calc_owner2_kernel<SOA, 2>), dim3(nb), dim3(threads), lds_bytes,
+                             s, plan_args, owner_meta, hier.cached, hier.packed, nb);
+        else
+          hipLaunchKernelGGL((res_calc_owner2_kernel<SOA, 1>), dim3(nb), dim3(threads), lds_bytes,
+                             s, plan_args, owner_meta, hier.cached, hier.packed, nb);
+      } else
         hipLaunchKernelGGL((eng_owner_kernel<F, SOA>), dim3(nb), dim3(threads), lds_bytes, s,
                            plan_args, hier, nb);
     } else if (dep_order) {

@@ -412,6 +412,8 @@ typedef struct {
  * mapping-0 point; ownership as fot_oplan_build defines it. */
 #define FOT_LOOP_OWNER_COMPUTE 4u
 #define FOT_LOOP_POINT_PARTITION 8u
+/* owner compute: two cells per thread (more loads in flight per wave) */
+#define FOT_LOOP_TWO_PER_THREAD 16u
 
 typedef struct fot_loop fot_loop;
 /* Builds the plan (host), uploads it, permutes cell data. */

@@ -59,6 +59,7 @@ def test_res_calc_global_colour_bitwise(gpu, soa):
 DEP = capi.LOOP_DEPENDENCY_ORDER
 STAGE = capi.LOOP_STAGE_INPUTS
 OWN = capi.LOOP_OWNER_COMPUTE
+TWO = capi.LOOP_TWO_PER_THREAD
 
 
 def _owner_replay(p, kernel, block_size, partition=None, iters=ITERS, point_partition=None):
@@ -74,7 +75,10 @@ def _owner_replay(p, kernel, block_size, partition=None, iters=ITERS, point_part
                                                  (False, 256, OWN, "ours"),
                                                  (True, 1024, OWN | STAGE, "kway"),
                                                  (False, 64, OWN, "points"),
-                                                 (True, 96, OWN, "points")])
+                                                 (True, 96, OWN, "points"),
+                                                 (False, 128, OWN | TWO, "points"),
+                                                 (True, 200, OWN | TWO, "points"),
+                                                 (False, 256, OWN | TWO, None)])
 def test_res_calc_owner_compute_bitwise(gpu, soa, bs, flags, method):
     p = airfoil_problem(nx=64, ny=32, soa=soa, res0_seed=10, block_size=bs)
     orig = airfoil_problem(nx=64, ny=32, soa=soa, res0_seed=10, block_size=bs)
