@@ -49,10 +49,13 @@ def parse():
     ap.add_argument("--partition", default="points",
                     choices=["none", "kway", "ours", "gps", "points"],
                     help="points: owner partition of mapping-0 points (owner compute)")
-    ap.add_argument("--points-per-block", type=int, default=256)
-    ap.add_argument("--ept", type=int, default=1, choices=[1, 2],
+    ap.add_argument("--points-per-block", type=int, default=192)
+    ap.add_argument("--ept", type=int, default=2, choices=[1, 2],
                     help="owner compute: cells per thread")
     ap.add_argument("--no-stage", action="store_true")
+    ap.add_argument("--nodes", default="gather", choices=["gather", "stage"],
+                    help="owner compute: gather node coordinates per edge, or stage each "
+                         "block's nodes in LDS first")
     ap.add_argument("--order", default="owner", choices=["owner", "dep", "launch"],
                     help="hierarchical block scheduling: owner compute in one launch (owner), "
                          "block colours ordered by completion flags in one launch (dep) or "
@@ -84,7 +87,7 @@ def build_problem(args, partition_kind):
     return mesh, p
 
 
-def make_loop(p, strategy, block_size, stage, graph, order="launch", ept=1):
+def make_loop(p, strategy, block_size, stage, graph, order="launch", ept=1, gather=False):
     from cuda_fot_amd import DeviceArray, Loop, capi
     strat = {"atomic": capi.STRATEGY_ATOMIC, "global": capi.STRATEGY_GLOBAL_COLOUR,
              "hierarchical": capi.STRATEGY_HIERARCHICAL}[strategy]
@@ -94,6 +97,8 @@ def make_loop(p, strategy, block_size, stage, graph, order="launch", ept=1):
         {"launch": 0, "dep": capi.LOOP_DEPENDENCY_ORDER, "owner": capi.LOOP_OWNER_COMPUTE}[order]
     if ept == 2:
         flags |= capi.LOOP_TWO_PER_THREAD
+    if gather:
+        flags |= capi.LOOP_GATHER_NODES
     part = p.partition_vector if strat == capi.STRATEGY_HIERARCHICAL else None
     if strat == capi.STRATEGY_HIERARCHICAL and order == "owner" and p.point_partition is not None:
         part = p.point_partition
@@ -163,7 +168,7 @@ def main():
     t_plan = time.perf_counter()
     mesh, p = build_problem(args, args.partition)
     loop, dev, out = make_loop(p, args.strategy, args.block_size, not args.no_stage,
-                               args.graph, args.order, args.ept)
+                               args.graph, args.order, args.ept, args.nodes == "gather")
     plan_s = time.perf_counter() - t_plan
     stats = loop.stats()
     wall, ev = time_loop(loop, args.steps, args.warmup)
@@ -191,6 +196,7 @@ def main():
                    "points_per_block": args.points_per_block if args.partition == "points" else None,
                    "stage_inputs": not args.no_stage, "graph": args.graph,
                    "block_order": args.order, "cells_per_thread": args.ept,
+                   "nodes": args.nodes,
                    "launches_per_step": stats["num_launches_per_iter"],
                    "block_colours": stats["num_colours"],
                    "avg_thread_colours": round(stats["avg_thread_colours"], 3),

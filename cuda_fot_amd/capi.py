@@ -47,6 +47,7 @@ LOOP_DEPENDENCY_ORDER = 2
 LOOP_OWNER_COMPUTE = 4
 LOOP_POINT_PARTITION = 8
 LOOP_TWO_PER_THREAD = 16
+LOOP_GATHER_NODES = 32
 
 
 class FotError(RuntimeError):

@@ -394,19 +394,27 @@ __global__ __launch_bounds__(1024) void eng_owner_kernel(EngineArgs a, HierDev h
 }
 
 // Per-block record of the owner-compute res_calc kernel: one scalar load.
+// entries appended to the owner res_calc plan arrays: covers reads up to
+// threads * max(cells, staged points) per thread past a block's start
+constexpr uint32_t kOwnerPad = 4096;
+
 struct OwnerMeta {
   uint32_t cell_off, ncells, cache_off, ncached;
   uint32_t nowned, own_start, ncol, contiguous; // owned ids own_start + i when contiguous
+  uint32_t node_off, nnodes, pad0, pad1;        // the block's mapping-1 points (x)
+  uint32_t pad2, pad3, pad4, pad5;
 };
 
 // res_calc owner compute, latency-oriented: block record in one scalar load,
 // owned points addressed directly when the mesh numbers them contiguously,
 // EPT edges per thread so each wave keeps more gathers in flight.
-template <bool SOA, int EPT>
+template <bool SOA, int EPT, bool SX, int KS>
 __global__ __launch_bounds__(1024) void res_calc_owner2_kernel(EngineArgs a,
                                                                const OwnerMeta *__restrict__ meta,
                                                                const uint32_t *__restrict__ cached,
                                                                const uint32_t *__restrict__ packed,
+                                                               const uint32_t *__restrict__ nodes,
+                                                               const uint32_t *__restrict__ nloc,
                                                                uint32_t nb) {
   extern __shared__ double lds[];
   const uint32_t b = xcd_block(blockIdx.x, nb);
@@ -414,52 +422,97 @@ __global__ __launch_bounds__(1024) void res_calc_owner2_kernel(EngineArgs a,
   const uint32_t tid = threadIdx.x, bd = blockDim.x;
   const uint32_t m = mt.ncached, nw = mt.nowned, ne = mt.ncells;
   const bool contig = mt.contiguous != 0;
-  double *res_s = lds, *q_s = lds + 4 * nw, *adt_s = q_s + 4 * m;
+  const uint32_t nx_b = SX ? mt.nnodes : 0;
+  // staged nodes first: dynamic LDS starts 16-byte aligned
+  double2 *x_s = reinterpret_cast<double2 *>(lds);
+  double *res_s = lds + 2 * nx_b, *q_s = res_s + 4 * nw, *adt_s = q_s + 4 * m;
   uint32_t *g_s = reinterpret_cast<uint32_t *>(adt_s + m);
   const double *x = a.point_data[1], *q = a.point_data[2], *adt = a.point_data[3];
   const uint32_t np = a.num_points[0];
   const uint32_t *pcached = cached + mt.cache_off;
   uint32_t pk[EPT];
   uint2 nn[EPT];
+  // plan arrays are padded: these loads need no bounds branch, and branch-free
+  // loads keep the compiler from waiting on each one at a join
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     const uint32_t t = tid + e * bd;
-    pk[e] = 0xff000000u;
-    nn[e] = make_uint2(0, 0);
-    if (t < ne) {
-      pk[e] = packed[mt.cell_off + t];
+    const uint32_t pv = packed[mt.cell_off + t];
+    pk[e] = t < ne ? pv : 0xff000000u;
+    if (SX) {
+      const uint32_t v = nloc[mt.cell_off + t];
+      nn[e] = make_uint2(v & 0xffffu, v >> 16);
+    } else {
       nn[e] = reinterpret_cast<const uint2 *>(a.maps[1])[mt.cell_off + t];
     }
   }
-  for (uint32_t i = tid; i < m; i += bd) {
-    const uint32_t g = (contig && i < nw) ? mt.own_start + i : pcached[i];
+  // Staging in two rounds so that every wave has all its loads in flight at
+  // once: first the point ids (KS per thread), then the data they name.
+  uint32_t gq[KS], gx[KS];
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const uint32_t i = tid + k * bd;
+    const uint32_t pc = pcached[i];
+    gq[k] = (contig && i < nw) ? mt.own_start + i : (i < m ? pc : 0u);
+    gx[k] = 0;
+    if (SX) {
+      const uint32_t pn = nodes[mt.node_off + i];
+      gx[k] = i < nx_b ? pn : 0u;
+    }
+  }
+  // every id is a valid point (point 0 past the block's end): unconditional
+  double2 vq0[KS], vq1[KS], vr0[KS], vr1[KS], vx[KS];
+  double vadt[KS];
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const uint32_t g = gq[k];
     if (SOA) {
       const uint32_t nq = a.num_points[2];
-#pragma unroll
-      for (uint32_t d = 0; d < 4; ++d) q_s[d * m + i] = q[size_t(d) * nq + g];
-      if (i < nw) {
-#pragma unroll
-        for (uint32_t d = 0; d < 4; ++d) res_s[d * nw + i] = a.out[size_t(d) * np + g];
-      }
+      vq0[k] = make_double2(q[g], q[size_t(nq) + g]);
+      vq1[k] = make_double2(q[2 * size_t(nq) + g], q[3 * size_t(nq) + g]);
+      vr0[k] = make_double2(a.out[g], a.out[size_t(np) + g]);
+      vr1[k] = make_double2(a.out[2 * size_t(np) + g], a.out[3 * size_t(np) + g]);
     } else {
       const double2 *q2p = reinterpret_cast<const double2 *>(q);
-      const double2 v0 = q2p[2 * size_t(g)], v1 = q2p[2 * size_t(g) + 1];
-      q_s[0 * m + i] = v0.x, q_s[1 * m + i] = v0.y, q_s[2 * m + i] = v1.x, q_s[3 * m + i] = v1.y;
+      vq0[k] = q2p[2 * size_t(g)], vq1[k] = q2p[2 * size_t(g) + 1];
+      const double2 *o = reinterpret_cast<const double2 *>(a.out) + 2 * size_t(g);
+      vr0[k] = o[0], vr1[k] = o[1];
+    }
+    vadt[k] = adt[g];
+    if (SX) {
+      const uint32_t nxp = a.num_points[1];
+      vx[k] = SOA ? make_double2(x[gx[k]], x[size_t(nxp) + gx[k]])
+                  : reinterpret_cast<const double2 *>(x)[gx[k]];
+    }
+  }
+  // one wait for all of them: without this use the compiler sinks each load
+  // into the guarded LDS store below and waits at every guard
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    asm volatile("" ::"v"(vq0[k].x), "v"(vq0[k].y), "v"(vq1[k].x), "v"(vq1[k].y));
+    asm volatile("" ::"v"(vr0[k].x), "v"(vr0[k].y), "v"(vr1[k].x), "v"(vr1[k].y), "v"(vadt[k]));
+    if (SX) asm volatile("" ::"v"(vx[k].x), "v"(vx[k].y));
+  }
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const uint32_t i = tid + k * bd;
+    if (i < m) {
+      q_s[0 * m + i] = vq0[k].x, q_s[1 * m + i] = vq0[k].y;
+      q_s[2 * m + i] = vq1[k].x, q_s[3 * m + i] = vq1[k].y;
+      adt_s[i] = vadt[k];
       if (i < nw) {
-        const double2 *o = reinterpret_cast<const double2 *>(a.out) + 2 * size_t(g);
-        const double2 r0 = o[0], r1 = o[1];
-        res_s[0 * nw + i] = r0.x, res_s[1 * nw + i] = r0.y;
-        res_s[2 * nw + i] = r1.x, res_s[3 * nw + i] = r1.y;
+        res_s[0 * nw + i] = vr0[k].x, res_s[1 * nw + i] = vr0[k].y;
+        res_s[2 * nw + i] = vr1[k].x, res_s[3 * nw + i] = vr1[k].y;
+        if (!contig) g_s[i] = gq[k];
       }
     }
-    adt_s[i] = adt[g];
-    if (!contig && i < nw) g_s[i] = g;
+    if (SX && i < nx_b) x_s[i] = vx[k];
   }
   double x1[EPT][2], x2[EPT][2];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     x1[e][0] = x1[e][1] = x2[e][0] = x2[e][1] = 0.0;
-    if (tid + e * bd < ne) {
+    if (!SX) { // nn is a valid node pair for every thread (padded map)
       if (SOA) {
         const uint32_t nx = a.num_points[1];
         x1[e][0] = x[nn[e].x], x1[e][1] = x[nx + nn[e].x];
@@ -476,6 +529,10 @@ __global__ __launch_bounds__(1024) void res_calc_owner2_kernel(EngineArgs a,
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     if (tid + e * bd < ne) {
+      if (SX) {
+        const double2 xa = x_s[nn[e].x], xb = x_s[nn[e].y];
+        x1[e][0] = xa.x, x1[e][1] = xa.y, x2[e][0] = xb.x, x2[e][1] = xb.y;
+      }
       const uint32_t l0 = pk[e] & 0xfffu, l1 = (pk[e] >> 12) & 0xfffu;
       double q1[4], q2[4];
 #pragma unroll
@@ -549,6 +606,9 @@ struct fot_loop {
   bool dep_order = false;
   bool owner = false;
   const OwnerMeta *owner_meta = nullptr;
+  const uint32_t *owner_nodes = nullptr, *owner_nloc = nullptr;
+  bool stage_x = false;
+  uint32_t stage_per_thread = 1; // owner res_calc: staged points per thread
   uint32_t ept = 1;
   DepDev dep{};
   // graph of one iteration
@@ -665,6 +725,7 @@ fot_status fot_loop::build_global_colour() {
 fot_status fot_loop::build_hierarchical() {
   fot_status st;
   const uint32_t md = desc.mesh_dim[0];
+  uint32_t max_nodes = 0;
   std::vector<uint32_t> blk_cell_off{0}, blk_cache_off{0}, cached, plan_cell, blk_owned;
   std::vector<uint16_t> loc;
   std::vector<uint8_t> colour, blk_ncol;
@@ -676,6 +737,16 @@ fot_status fot_loop::build_hierarchical() {
   owner = (cfg.flags & FOT_LOOP_OWNER_COMPUTE) != 0;
   if (owner && (cfg.flags & FOT_LOOP_DEPENDENCY_ORDER))
     return fail(FOT_ERR_INVALID_ARGUMENT, "owner compute needs no dependency order");
+  if (f.kernel == FOT_KERNEL_RES_CALC &&
+      (owner || (cfg.flags & FOT_LOOP_STAGE_INPUTS))) {
+    // these kernels stage q and adt through mapping-0 ids (res_calc.cu:22-25
+    // passes the edge->cell mapping for res, q and adt alike)
+    const size_t n = size_t(desc.num_cells) * 2 * sizeof(uint32_t);
+    if (n && (std::memcmp(desc.cell_to_node[2], desc.cell_to_node[0], n) ||
+              std::memcmp(desc.cell_to_node[3], desc.cell_to_node[0], n)))
+      return fail(FOT_ERR_UNSUPPORTED,
+                  "staged res_calc needs the q and adt mappings equal to the res mapping");
+  }
   if (owner) {
     OwnerPlan op;
     const uint32_t *cell_part = cfg.partition, *point_part = nullptr;
@@ -757,6 +828,10 @@ fot_status fot_loop::build_hierarchical() {
   hier.blk_cell_off = d32;
   if ((st = upload(blk_cache_off.data(), blk_cache_off.size(), &d32)) != FOT_OK) return st;
   hier.blk_cache_off = d32;
+  // owner res_calc loads plan arrays without bounds branches (clamped reads
+  // past a block's end land in the next block or this padding)
+  const bool owner_rc = owner && f.kernel == FOT_KERNEL_RES_CALC;
+  if (owner_rc) cached.resize(cached.size() + kOwnerPad, 0u);
   if ((st = upload(cached.data(), cached.size(), &d32)) != FOT_OK) return st;
   hier.cached = d32;
   if ((st = upload(loc.data(), loc.size(), &d16)) != FOT_OK) return st;
@@ -776,6 +851,7 @@ fot_status fot_loop::build_hierarchical() {
           return fail(FOT_ERR_UNSUPPORTED, "owner block with more than 4096 points");
         packed[i] = l0 | (l1 << 12) | (uint32_t(colour[i]) << 24);
       }
+      packed.resize(packed.size() + kOwnerPad, 0xff000000u);
       if ((st = upload(packed.data(), packed.size(), &d32)) != FOT_OK) return st;
       hier.packed = d32;
       std::vector<OwnerMeta> meta(blk_owned.size());
@@ -795,6 +871,44 @@ fot_status fot_loop::build_hierarchical() {
             break;
           }
       }
+      // each block's mapping-1 points (x) for staging: ascending unique node
+      // ids and per-cell local pairs
+      std::vector<uint32_t> nodes_all, nloc(plan_cell.size()), tmp;
+      stage_x = (cfg.flags & FOT_LOOP_GATHER_NODES) == 0;
+      for (size_t b = 0; b < meta.size() && stage_x; ++b) {
+        OwnerMeta &mt = meta[b];
+        tmp.clear();
+        for (uint32_t i = mt.cell_off; i < mt.cell_off + mt.ncells; ++i) {
+          tmp.push_back(plan_maps[1][2 * size_t(i)]);
+          tmp.push_back(plan_maps[1][2 * size_t(i) + 1]);
+        }
+        std::sort(tmp.begin(), tmp.end());
+        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+        if (tmp.size() > 0xffffu) {
+          stage_x = false;
+          break;
+        }
+        mt.node_off = uint32_t(nodes_all.size());
+        mt.nnodes = uint32_t(tmp.size());
+        max_nodes = std::max(max_nodes, mt.nnodes);
+        for (uint32_t i = mt.cell_off; i < mt.cell_off + mt.ncells; ++i) {
+          const uint32_t a0 = uint32_t(std::lower_bound(tmp.begin(), tmp.end(),
+                                                        plan_maps[1][2 * size_t(i)]) - tmp.begin());
+          const uint32_t a1 = uint32_t(std::lower_bound(tmp.begin(), tmp.end(),
+                                                        plan_maps[1][2 * size_t(i) + 1]) -
+                                       tmp.begin());
+          nloc[i] = a0 | (a1 << 16);
+        }
+        nodes_all.insert(nodes_all.end(), tmp.begin(), tmp.end());
+      }
+      if (stage_x) {
+        nodes_all.resize(nodes_all.size() + kOwnerPad, 0u);
+        nloc.resize(nloc.size() + kOwnerPad, 0u);
+        if ((st = upload(nodes_all.data(), nodes_all.size(), &d32)) != FOT_OK) return st;
+        owner_nodes = d32;
+        if ((st = upload(nloc.data(), nloc.size(), &d32)) != FOT_OK) return st;
+        owner_nloc = d32;
+      }
       OwnerMeta *dm = nullptr;
       if ((st = upload(meta.data(), meta.size(), &dm)) != FOT_OK) return st;
       owner_meta = dm;
@@ -803,6 +917,7 @@ fot_status fot_loop::build_hierarchical() {
   plan_args = base;
   for (uint32_t m = 1; m < desc.num_mappings; ++m) {
     if (!((mask >> m) & 1u)) continue;
+    if (owner_rc) plan_maps[m].resize(plan_maps[m].size() + 2 * size_t(kOwnerPad), 0u);
     if ((st = upload(plan_maps[m].data(), plan_maps[m].size(), &d32)) != FOT_OK) return st;
     plan_args.maps[m] = d32;
   }
@@ -850,6 +965,7 @@ fot_status fot_loop::build_hierarchical() {
                                  : 8u * td * nw + 4u * nw;
       lds_bytes = std::max(lds_bytes, bytes);
     }
+    if (f.kernel == FOT_KERNEL_RES_CALC && stage_x) lds_bytes += 16u * max_nodes; // staged x
   }
   if (lds_bytes > 160u * 1024u)
     return fail(FOT_ERR_UNSUPPORTED, "block cache needs %u B of LDS (max 160 KiB)", lds_bytes);
@@ -859,6 +975,22 @@ fot_status fot_loop::build_hierarchical() {
   ept = (owner && f.kernel == FOT_KERNEL_RES_CALC && (cfg.flags & FOT_LOOP_TWO_PER_THREAD)) ? 2 : 1;
   threads = std::max<uint32_t>(64, ((max_block + ept - 1) / ept + 63) / 64 * 64);
   if (threads > 1024) return fail(FOT_ERR_UNSUPPORTED, "blocks above 1024 cells");
+  if (owner && f.kernel == FOT_KERNEL_RES_CALC) {
+    // the staging rounds cover max(cached, nodes) of every block
+    uint32_t most = max_nodes;
+    for (size_t b = 0; b + 1 < blk_cache_off.size(); ++b)
+      most = std::max(most, blk_cache_off[b + 1] - blk_cache_off[b]);
+    stage_per_thread = (most + threads - 1) / threads;
+    if (stage_per_thread > 4) {
+      // widen the block rather than stage more than 4 points per thread
+      threads = std::min<uint32_t>(1024, ((most + 3) / 4 + 63) / 64 * 64);
+      stage_per_thread = (most + threads - 1) / threads;
+      if (stage_per_thread > 4)
+        return fail(FOT_ERR_UNSUPPORTED, "owner block stages %u points (max %u)", most,
+                    4u * 1024u);
+    }
+    stage_per_thread = std::max<uint32_t>(1, stage_per_thread);
+  }
   stats.num_colours = num_plan_colours;
   stats.num_blocks = nb_total;
   stats.num_launches_per_iter = (dep_order || owner) ? 1u : stats.num_colours;
@@ -888,13 +1020,29 @@ fot_status fot_loop::enqueue(hipStream_t s) {
       }
     } else if (owner) {
       const uint32_t nb = colour_blk_off.back();
+      // no cells: nothing to add (and the staging reads assume point 0 exists)
+      if (nb == 0 || desc.num_cells == 0) return FOT_OK;
       if constexpr (std::is_same<F, ResCalcF>::value) {
-        if (ept == 2)
-          hipLaunchKernelGGL((res_calc_owner2_kernel<SOA, 2>), dim3(nb), dim3(threads), lds_bytes,
-                             s, plan_args, owner_meta, hier.cached, hier.packed, nb);
-        else
-          hipLaunchKernelGGL((res_calc_owner2_kernel<SOA, 1>), dim3(nb), dim3(threads), lds_bytes,
-                             s, plan_args, owner_meta, hier.cached, hier.packed, nb);
+#define FOT_OWNER2(E_, X_, K_)                                                          \
+  hipLaunchKernelGGL((res_calc_owner2_kernel<SOA, E_, X_, K_>), dim3(nb), dim3(threads),     \
+                     lds_bytes, s, plan_args, owner_meta, hier.cached, hier.packed,          \
+                     owner_nodes, owner_nloc, nb)
+#define FOT_OWNER2_K(E_, X_)                                                            \
+  switch (stage_per_thread) {                                                           \
+  case 1: FOT_OWNER2(E_, X_, 1); break;                                                 \
+  case 2: FOT_OWNER2(E_, X_, 2); break;                                                 \
+  case 3: FOT_OWNER2(E_, X_, 3); break;                                                 \
+  default: FOT_OWNER2(E_, X_, 4); break;                                                \
+  }
+        if (ept == 2) {
+          if (stage_x) FOT_OWNER2_K(2, true)
+          else FOT_OWNER2_K(2, false)
+        } else {
+          if (stage_x) FOT_OWNER2_K(1, true)
+          else FOT_OWNER2_K(1, false)
+        }
+#undef FOT_OWNER2_K
+#undef FOT_OWNER2
       } else
         hipLaunchKernelGGL((eng_owner_kernel<F, SOA>), dim3(nb), dim3(threads), lds_bytes, s,
                            plan_args, hier, nb);

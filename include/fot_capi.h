@@ -414,6 +414,9 @@ typedef struct {
 #define FOT_LOOP_POINT_PARTITION 8u
 /* owner compute: two cells per thread (more loads in flight per wave) */
 #define FOT_LOOP_TWO_PER_THREAD 16u
+/* owner compute, res_calc: gather mapping-1 (node) data straight from HBM per
+ * cell instead of staging each block's nodes in LDS first */
+#define FOT_LOOP_GATHER_NODES 32u
 
 typedef struct fot_loop fot_loop;
 /* Builds the plan (host), uploads it, permutes cell data. */

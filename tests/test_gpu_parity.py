@@ -60,6 +60,7 @@ DEP = capi.LOOP_DEPENDENCY_ORDER
 STAGE = capi.LOOP_STAGE_INPUTS
 OWN = capi.LOOP_OWNER_COMPUTE
 TWO = capi.LOOP_TWO_PER_THREAD
+GATHER = capi.LOOP_GATHER_NODES
 
 
 def _owner_replay(p, kernel, block_size, partition=None, iters=ITERS, point_partition=None):
@@ -78,6 +79,8 @@ def _owner_replay(p, kernel, block_size, partition=None, iters=ITERS, point_part
                                                  (True, 96, OWN, "points"),
                                                  (False, 128, OWN | TWO, "points"),
                                                  (True, 200, OWN | TWO, "points"),
+                                                 (False, 192, OWN | TWO | GATHER, "points"),
+                                                 (True, 128, OWN | GATHER, "points"),
                                                  (False, 256, OWN | TWO, None)])
 def test_res_calc_owner_compute_bitwise(gpu, soa, bs, flags, method):
     p = airfoil_problem(nx=64, ny=32, soa=soa, res0_seed=10, block_size=bs)
