@@ -739,8 +739,9 @@ fot_status fot_loop::build_hierarchical() {
     return fail(FOT_ERR_INVALID_ARGUMENT, "owner compute needs no dependency order");
   if (f.kernel == FOT_KERNEL_RES_CALC &&
       (owner || (cfg.flags & FOT_LOOP_STAGE_INPUTS))) {
-    // these kernels stage q and adt through mapping-0 ids (res_calc.cu:22-25
-    // passes the edge->cell mapping for res, q and adt alike)
+    // these kernels stage q and adt through mapping-0 ids; res_calc reads its
+    // res, q and adt mappings from three streams (res_calc.cu:13-20) that the
+    // airfoil mesh fills with the one edge->cell map
     const size_t n = size_t(desc.num_cells) * 2 * sizeof(uint32_t);
     if (n && (std::memcmp(desc.cell_to_node[2], desc.cell_to_node[0], n) ||
               std::memcmp(desc.cell_to_node[3], desc.cell_to_node[0], n)))

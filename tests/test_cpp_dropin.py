@@ -1,0 +1,112 @@
+"""The C++ drop-in (include/fot/problem.hpp, kernels.hpp, data.hpp) driven
+like the reference's Problem<SOA> by tests/cpp/dropin_test.cpp; its outputs
+are checked here against the oracle on the same seeded inputs.  The CPU test
+also covers the mesh / point / cell / partition stream formats (self-checks
+inside the driver)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cuda_fot_amd import meshgen
+from helpers import assert_bitwise, assert_close_scaled
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BIN = os.path.join(HERE, "cpp", "bin", "dropin_test")
+ITERS_RES, ITERS_MINE, ITERS_HEX = 3, 3, 2
+
+
+def _run(mode, out):
+    r = subprocess.run([BIN, mode, str(out)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def _load(out, name, dim):
+    return np.fromfile(os.path.join(out, name + ".bin"), dtype=np.float64).reshape(-1, dim)
+
+
+def _airfoil_oracle():
+    # the driver's airfoil: 40 x 24 cells, seed 3, shuffled, res0 U[-1,1) seed 11
+    m = meshgen.airfoil(40, 24, seed=3, shuffle=7)
+    res0 = meshgen.uniform(m["num_cells"] * 4, 11, -1.0, 1.0).reshape(-1, 4)
+    maps = [m["pecell"], m["pedge"], m["pecell"], m["pecell"]]
+    npts = [m["num_cells"], m["num_nodes"], m["num_cells"], m["num_cells"]]
+    return lambda: O.OracleProblem(O.RES_CALC, maps, npts, [res0, m["x"], m["q"], m["adt"]])
+
+
+def _mine_oracle():
+    N, M = 20, 17
+    e = meshgen.grid_edges(N, M)
+    pd = meshgen.uniform(N * M * 4, 5).reshape(-1, 4)
+    w = meshgen.uniform(e.shape[0] * 4, 6).reshape(4, -1)
+    return lambda: O.OracleProblem(O.MINE2, [e], [N * M], [pd], [w])
+
+
+def _hex_oracle():
+    c = meshgen.hex_grid(5, 4, 3)
+    F = meshgen.normal(c.shape[0] * 24, 9).reshape(24, -1)
+    return lambda: O.OracleProblem(O.HEX_SCATTER, [c], [60], [np.zeros((60, 3))], [F])
+
+
+def _serial(make, iters):
+    o = make()
+    S = o.loop_serial(iters, with_scale=True)
+    return o.target.copy(), S
+
+
+def _colour(make, iters):
+    o = make()
+    o.loop_colour(iters, 1)
+    return o.target.copy()
+
+
+@pytest.fixture(scope="module")
+def built():
+    # CPU-only build (g++ against libfot; a no-op when up to date);
+    # `__graft_entry__.build()` makes it too
+    subprocess.run(["make", "-C", os.path.join(HERE, "cpp")], check=True, capture_output=True)
+    return BIN
+
+
+def test_cpp_dropin_cpu_loops(built, tmp_path):
+    out = _run("cpu", tmp_path)
+    assert "io round trips ok" in out
+    air, mine, hexp = _airfoil_oracle(), _mine_oracle(), _hex_oracle()
+    serial, S = _serial(air, ITERS_RES)
+    colour = _colour(air, ITERS_RES)
+    for tag in ("aos", "soa"):
+        # the same user function in the same order: identical bits
+        assert_bitwise(_load(tmp_path, "res_serial_" + tag, 4), serial, "C++ serial " + tag)
+        assert_bitwise(_load(tmp_path, "res_omp_" + tag, 4), colour, "C++ OMP " + tag)
+        # reordered / partitioned problems: same result within the FP bound
+        assert_close_scaled(_load(tmp_path, "res_reordered_" + tag, 4), serial, S,
+                            what="C++ reordered " + tag)
+        assert_close_scaled(_load(tmp_path, "res_ours_" + tag, 4), serial, S,
+                            what="C++ partitionOurs " + tag)
+    ms, _ = _serial(mine, ITERS_MINE)
+    hs, _ = _serial(hexp, ITERS_HEX)
+    for tag in ("aos", "soa"):
+        assert_bitwise(_load(tmp_path, "mine_serial_" + tag, 4), ms, "C++ mine " + tag)
+        assert_bitwise(_load(tmp_path, "hex_serial_" + tag, 3), hs, "C++ hex " + tag)
+
+
+@pytest.mark.gpu
+def test_cpp_dropin_gpu_loops(gpu, tmp_path):
+    assert os.path.exists(BIN), "tests/cpp/bin/dropin_test not built (run build())"
+    _run("gpu", tmp_path)
+    air, mine, hexp = _airfoil_oracle(), _mine_oracle(), _hex_oracle()
+    serial, S = _serial(air, ITERS_RES)
+    colour = _colour(air, ITERS_RES)
+    ms, mS = _serial(mine, ITERS_MINE)
+    hs, hS = _serial(hexp, ITERS_HEX)
+    hc = _colour(hexp, ITERS_HEX)
+    for tag in ("aos", "soa"):
+        assert_bitwise(_load(tmp_path, "res_global_" + tag, 4), colour, "global " + tag)
+        for name in ("res_atomic_", "res_owner_", "res_hier_"):
+            assert_close_scaled(_load(tmp_path, name + tag, 4), serial, S, what=name + tag)
+        assert_close_scaled(_load(tmp_path, "mine_owner_" + tag, 4), ms, mS, what="mine " + tag)
+        assert_close_scaled(_load(tmp_path, "hex_atomic_" + tag, 3), hs, hS, what="hex " + tag)
+        assert_bitwise(_load(tmp_path, "hex_global_" + tag, 3), hc, "hex global " + tag)
