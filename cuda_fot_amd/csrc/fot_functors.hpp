@@ -166,6 +166,35 @@ struct HexScatterF {
   }
 };
 
+// volna (kernels/volna.hpp:27-78, volna_func.hpp:16-36) in FP64: the cell
+// volumes are point data 1, read through mapping-0 ids as the reference
+// does; cell data 0..3 = edge fluxes (3), bathymetry source (2), edge
+// normals (2), right-boundary flag (0.0 / 1.0).  The increments start at 0
+// and are updated with the reference's `-=` / `+=`, so even signed zeros
+// match.  The fourth state component gets no increment.
+struct VolnaF {
+  static constexpr uint32_t MD = 2, TD = 4;
+  static constexpr bool SLOT_OUTER = false;
+  template <bool SOA, class A>
+  FOT_DEV static void compute(const A &a, uint32_t ind, const uint32_t *tgt, double *inc) {
+    const double *vol = a.pd(1);
+    const uint32_t cs = a.cs();
+    const double *fl = a.cd(0) + ind, *bs = a.cd(1) + ind, *nm = a.cd(2) + ind;
+    const double boundary = a.cd(3)[ind];
+    const double v0 = vol[tgt[0]], v1 = vol[tgt[1]];
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) inc[k] = 0.0;
+    inc[0] -= fl[0] / v0;
+    inc[1] -= (fl[cs] + bs[0] * nm[0]) / v0;
+    inc[2] -= (fl[2 * cs] + bs[0] * nm[cs]) / v0;
+    if (boundary == 0.0) {
+      inc[4] += fl[0] / v1;
+      inc[5] += (fl[cs] + bs[cs] * nm[0]) / v1;
+      inc[6] += (fl[2 * cs] + bs[cs] * nm[cs]) / v1;
+    }
+  }
+};
+
 // Mapping-0 ids of cell `ind` (AOS row).
 template <class F, class A>
 FOT_DEV void targets(const A &a, uint32_t ind, uint32_t *tgt) {

@@ -1110,6 +1110,14 @@ extern "C" fot_status fot_loop_create(const fot_functor *f, const fot_problem_de
   if ((f->kernel == FOT_KERNEL_MINE || f->kernel == FOT_KERNEL_HEX_SCATTER) &&
       (desc->num_cell_data < 1 || desc->cell_dim[0] != f->cell_dim))
     return fail(FOT_ERR_INVALID_ARGUMENT, "functor needs cell data of dim %u", f->cell_dim);
+  if (f->kernel == FOT_KERNEL_VOLNA &&
+      (desc->num_mappings < 2 || desc->point_dim[1] != 1 ||
+       desc->num_points[1] < desc->num_points[0] || desc->num_cell_data < 4 ||
+       desc->cell_dim[0] != 3 || desc->cell_dim[1] != 2 || desc->cell_dim[2] != 2 ||
+       desc->cell_dim[3] != 1))
+    return fail(FOT_ERR_INVALID_ARGUMENT,
+                "volna needs state/volume mappings and fluxes(3), bathy(2), normals(2), "
+                "boundary(1) cell data");
   if (cfg->block_size == 0 || cfg->block_size > 1024)
     return fail(FOT_ERR_INVALID_ARGUMENT, "block size %u", cfg->block_size);
   std::unique_ptr<fot_loop> L(new fot_loop);

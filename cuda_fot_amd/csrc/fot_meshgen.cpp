@@ -187,3 +187,148 @@ extern "C" fot_status fot_gen_airfoil(uint32_t nx, uint32_t ny, uint64_t seed,
   }
   return FOT_OK;
 }
+
+namespace {
+struct Pt {
+  double x, y;
+};
+} // namespace
+
+// Triangle mesh of the shallow-water edge loop (volna, kernels/volna.hpp):
+// an (nx+1) x (ny+1) node grid perturbed by U(-0.2, 0.2), every quad split
+// along a random diagonal.  Per quad (i, j) with corners a=(i,j), b=(i+1,j),
+// c=(i+1,j+1), d=(i,j+1): diagonal a-c gives T0=(a,b,c), T1=(a,c,d);
+// diagonal b-d gives T0=(a,b,d), T1=(b,c,d).  Edges in order: horizontal,
+// vertical, diagonal.  An interior edge maps to (left, right) with its unit
+// normal pointing from left to right; a boundary edge maps to (cell, cell)
+// with is_boundary 1 and the outward normal.
+extern "C" fot_status fot_gen_tri_mesh(uint32_t nx, uint32_t ny, uint64_t seed, uint32_t shuffle,
+                                       uint32_t *edge_cells, double *is_boundary,
+                                       double *normals, double *fluxes, double *bathy,
+                                       double *volume, double *state) {
+  if (nx < 1 || ny < 1) return fail(FOT_ERR_INVALID_ARGUMENT, "tri mesh needs nx,ny >= 1");
+  const uint64_t nc64 = 2ull * nx * ny;
+  const uint64_t ne64 = uint64_t(nx) * (ny + 1) + uint64_t(nx + 1) * ny + uint64_t(nx) * ny;
+  if (ne64 > 0xffffffffull) return fail(FOT_ERR_INVALID_ARGUMENT, "tri mesh too large");
+  const uint32_t nc = uint32_t(nc64), ne = uint32_t(ne64);
+  std::vector<uint32_t> cell_perm(nc), edge_perm(ne);
+  if (shuffle & 2u) permutation(nc, stream_seed(seed, 2), cell_perm.data());
+  else std::iota(cell_perm.begin(), cell_perm.end(), 0u);
+  if (shuffle & 4u) permutation(ne, stream_seed(seed, 3), edge_perm.data());
+  else std::iota(edge_perm.begin(), edge_perm.end(), 0u);
+  std::vector<Pt> X(size_t(nx + 1) * (ny + 1));
+  {
+    SplitMix64 g(stream_seed(seed, 4));
+    for (uint32_t j = 0; j <= ny; ++j)
+      for (uint32_t i = 0; i <= nx; ++i) {
+        const double dx = -0.2 + 0.4 * g.uniform();
+        const double dy = -0.2 + 0.4 * g.uniform();
+        X[size_t(j) * (nx + 1) + i] = {double(i) + dx, double(j) + dy};
+      }
+  }
+  std::vector<uint8_t> diag(size_t(nx) * ny);
+  {
+    SplitMix64 g(stream_seed(seed, 7));
+    for (auto &d : diag) d = uint8_t(g.next() >> 63);
+  }
+  auto node = [&](uint32_t i, uint32_t j) { return X[size_t(j) * (nx + 1) + i]; };
+  // triangle t of quad (i, j): vertices and centroid
+  auto tri = [&](uint32_t i, uint32_t j, int t, Pt v[3]) {
+    const Pt a = node(i, j), b = node(i + 1, j), c = node(i + 1, j + 1), d = node(i, j + 1);
+    if (diag[size_t(j) * nx + i] == 0) {
+      if (t == 0) v[0] = a, v[1] = b, v[2] = c;
+      else v[0] = a, v[1] = c, v[2] = d;
+    } else {
+      if (t == 0) v[0] = a, v[1] = b, v[2] = d;
+      else v[0] = b, v[1] = c, v[2] = d;
+    }
+  };
+  auto centroid = [&](uint32_t i, uint32_t j, int t) {
+    Pt v[3];
+    tri(i, j, t, v);
+    return Pt{(v[0].x + v[1].x + v[2].x) / 3.0, (v[0].y + v[1].y + v[2].y) / 3.0};
+  };
+  auto cid = [&](uint32_t i, uint32_t j, int t) { return cell_perm[2 * (j * nx + i) + t]; };
+  // which triangle of quad (i, j) holds its right / left edge
+  auto right_tri = [&](uint32_t i, uint32_t j) { return diag[size_t(j) * nx + i] == 0 ? 0 : 1; };
+  auto left_tri = [&](uint32_t i, uint32_t j) { return diag[size_t(j) * nx + i] == 0 ? 1 : 0; };
+  SplitMix64 gf(stream_seed(seed, 8)), gb(stream_seed(seed, 9));
+  uint32_t k = 0;
+  auto emit = [&](Pt p, Pt q, bool boundary, uint32_t lc, Pt lcen, uint32_t rc, Pt rcen) {
+    const uint32_t e = edge_perm[k++];
+    double nxv = q.y - p.y, nyv = -(q.x - p.x);
+    const double len = std::sqrt(nxv * nxv + nyv * nyv);
+    nxv /= len, nyv /= len;
+    // interior: towards the right cell; boundary: away from the cell
+    const Pt to = boundary ? Pt{0.5 * (p.x + q.x), 0.5 * (p.y + q.y)} : rcen;
+    if (nxv * (to.x - lcen.x) + nyv * (to.y - lcen.y) < 0) nxv = -nxv, nyv = -nyv;
+    if (edge_cells) {
+      edge_cells[2 * size_t(e)] = lc;
+      edge_cells[2 * size_t(e) + 1] = boundary ? lc : rc;
+    }
+    if (is_boundary) is_boundary[e] = boundary ? 1.0 : 0.0;
+    if (normals) normals[e] = nxv, normals[size_t(ne) + e] = nyv;
+    // fluxes and bathymetry source N(0, 0.1), drawn in generation order
+    for (int d = 0; d < 3; ++d) {
+      const double v = 0.1 * gf.normal();
+      if (fluxes) fluxes[size_t(d) * ne + e] = v;
+    }
+    for (int d = 0; d < 2; ++d) {
+      const double v = 0.1 * gb.normal();
+      if (bathy) bathy[size_t(d) * ne + e] = v;
+    }
+  };
+  const Pt none{0, 0};
+  for (uint32_t j = 0; j <= ny; ++j) // horizontal edges: below T1 of (i,j-1), above T0 of (i,j)
+    for (uint32_t i = 0; i < nx; ++i) {
+      const Pt p = node(i, j), q = node(i + 1, j);
+      if (j == 0) emit(p, q, true, cid(i, 0, 0), centroid(i, 0, 0), 0, none);
+      else if (j == ny) emit(p, q, true, cid(i, ny - 1, 1), centroid(i, ny - 1, 1), 0, none);
+      else
+        emit(p, q, false, cid(i, j - 1, 1), centroid(i, j - 1, 1), cid(i, j, 0),
+             centroid(i, j, 0));
+    }
+  for (uint32_t j = 0; j < ny; ++j) // vertical edges: left quad's right triangle first
+    for (uint32_t i = 0; i <= nx; ++i) {
+      const Pt p = node(i, j), q = node(i, j + 1);
+      if (i == 0) {
+        const int t = left_tri(0, j);
+        emit(p, q, true, cid(0, j, t), centroid(0, j, t), 0, none);
+      } else if (i == nx) {
+        const int t = right_tri(nx - 1, j);
+        emit(p, q, true, cid(nx - 1, j, t), centroid(nx - 1, j, t), 0, none);
+      } else {
+        const int tl = right_tri(i - 1, j), tr = left_tri(i, j);
+        emit(p, q, false, cid(i - 1, j, tl), centroid(i - 1, j, tl), cid(i, j, tr),
+             centroid(i, j, tr));
+      }
+    }
+  for (uint32_t j = 0; j < ny; ++j) // diagonals: T0 left, T1 right
+    for (uint32_t i = 0; i < nx; ++i) {
+      const bool ac = diag[size_t(j) * nx + i] == 0;
+      const Pt p = ac ? node(i, j) : node(i + 1, j), q = ac ? node(i + 1, j + 1) : node(i, j + 1);
+      emit(p, q, false, cid(i, j, 0), centroid(i, j, 0), cid(i, j, 1), centroid(i, j, 1));
+    }
+  if (volume) {
+    for (uint32_t j = 0; j < ny; ++j)
+      for (uint32_t i = 0; i < nx; ++i)
+        for (int t = 0; t < 2; ++t) {
+          Pt v[3];
+          tri(i, j, t, v);
+          const double a2 = (v[1].x - v[0].x) * (v[2].y - v[0].y) -
+                            (v[2].x - v[0].x) * (v[1].y - v[0].y);
+          volume[cid(i, j, t)] = 0.5 * std::fabs(a2);
+        }
+  }
+  if (state) {
+    SplitMix64 g(stream_seed(seed, 10));
+    for (uint32_t c = 0; c < nc; ++c) {
+      const uint32_t cp = cell_perm[c];
+      state[4 * size_t(cp) + 0] = 0.5 + 1.5 * g.uniform();
+      state[4 * size_t(cp) + 1] = 0.1 * g.normal();
+      state[4 * size_t(cp) + 2] = 0.1 * g.normal();
+      state[4 * size_t(cp) + 3] = -g.uniform();
+    }
+  }
+  return FOT_OK;
+}

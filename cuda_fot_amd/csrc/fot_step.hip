@@ -69,6 +69,20 @@ template <bool SOA> struct HierGather<HexScatterF, SOA> {
   }
 };
 
+template <bool SOA> struct HierGather<VolnaF, SOA> {
+  // volumes through the block's global ids (mapping 1 is the same map)
+  FOT_DEV static void run(const StepArgs &a, uint32_t ind, uint32_t n, const uint32_t *cached,
+                          uint32_t *loc, double *inc) {
+    uint32_t g[2];
+#pragma unroll
+    for (uint32_t k = 0; k < 2; ++k) {
+      loc[k] = a.c2n[0][ind + k * n];
+      g[k] = cached[loc[k]];
+    }
+    VolnaF::compute<SOA>(a, ind, g, inc);
+  }
+};
+
 __device__ uint32_t g_lds_overflow; // set when a block's cache exceeds cache_size
 
 // stepGPUHierarchical (kernels/res_calc.hpp:278-443 and its siblings): the
@@ -146,8 +160,8 @@ extern "C" fot_status fot_functor_init(fot_functor *f, fot_kernel kernel, uint32
   case FOT_KERNEL_MINI_AERO: // kernels/mini_aero.hpp:16-19
     f->mesh_dim = 2, f->point_dim = 5, f->cell_dim = 12;
     return FOT_OK;
-  case FOT_KERNEL_VOLNA: // kernels/volna.hpp:16-19
-    f->mesh_dim = 2, f->point_dim = 4, f->cell_dim = 4;
+  case FOT_KERNEL_VOLNA: // kernels/volna.hpp:16-19 (cell_dim: the fluxes)
+    f->mesh_dim = 2, f->point_dim = 4, f->cell_dim = 3;
     return FOT_OK;
   case FOT_KERNEL_MINE:
     if ((mesh_dim != 2 && mesh_dim != 4 && mesh_dim != 8) || point_dim == 0 ||

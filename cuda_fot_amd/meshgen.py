@@ -54,3 +54,19 @@ def airfoil(nx: int, ny: int, seed: int = 0, shuffle: int = 7) -> dict:
          ptr(adt))
     return {"pedge": pedge, "pecell": pecell, "x": x, "q": q, "adt": adt,
             "num_nodes": nn, "num_cells": nc, "num_edges": ne, "nx": nx, "ny": ny}
+
+
+def tri_mesh(nx: int, ny: int, seed: int = 0, shuffle: int = 6) -> dict:
+    """Triangle mesh and data of the shallow-water (volna) edge loop: an
+    nx x ny quad grid split on random diagonals, boundary edges included.
+    edge_cells (E,2) (cell, cell) on the boundary; is_boundary (E,) 0/1;
+    normals (2,E), fluxes (3,E), bathy (2,E) SOA; volume (C,); state (C,4)."""
+    nc = 2 * nx * ny
+    ne = nx * (ny + 1) + (nx + 1) * ny + nx * ny
+    out = {"edge_cells": np.empty((ne, 2), np.uint32), "is_boundary": np.empty(ne),
+           "normals": np.empty((2, ne)), "fluxes": np.empty((3, ne)), "bathy": np.empty((2, ne)),
+           "volume": np.empty(nc), "state": np.empty((nc, 4))}
+    call("fot_gen_tri_mesh", nx, ny, seed, shuffle, *(ptr(out[k]) for k in
+         ("edge_cells", "is_boundary", "normals", "fluxes", "bathy", "volume", "state")))
+    out.update(num_cells=nc, num_edges=ne, nx=nx, ny=ny)
+    return out

@@ -183,3 +183,15 @@ def res_calc_problem(mesh: dict, block_size: int = DEFAULT_BLOCK_SIZE, soa: bool
         res, x, q = res.T.copy(), x.T.copy(), q.T.copy()
     maps = [mesh["pecell"], mesh["pedge"], mesh["pecell"], mesh["pecell"]]
     return Problem(maps, [nc, nn, nc, nc], [res, x, q, adt], (), block_size, soa)
+
+
+def volna_problem(mesh: dict, block_size: int = DEFAULT_BLOCK_SIZE, soa: bool = False) -> Problem:
+    """The shallow-water edge loop as volna.cu:10-36 sets it up: two copies
+    of the edge->cell map; point data state (dim 4, the target) and cell
+    volume (dim 1); edge data fluxes, bathymetry source, normals and the
+    boundary flag."""
+    nc = mesh["num_cells"]
+    state = mesh["state"].T.copy() if soa else mesh["state"].copy()
+    maps = [mesh["edge_cells"], mesh["edge_cells"]]
+    cell = [mesh["fluxes"], mesh["bathy"], mesh["normals"], mesh["is_boundary"][None, :]]
+    return Problem(maps, [nc, nc], [state, mesh["volume"]], cell, block_size, soa)

@@ -105,7 +105,10 @@ typedef enum {
    * (dim 5, target), mapping 1 cell data (dim 28), 12 face doubles */
   FOT_KERNEL_MINI_AERO = 3,
   /* shallow-water edge increment in FP64, kernels/volna.hpp +
-   * volna_func.hpp */
+   * volna_func.hpp: mappings 0 and 1 the same edge->cell map, point data 0
+   * state (dim 4, target), point data 1 cell volume (dim 1); cell data
+   * fluxes (3), bathymetry source (2), edge normals (2), right-boundary flag
+   * (1, stored as 0.0 / 1.0 where the reference has an int) */
   FOT_KERNEL_VOLNA = 4
 } fot_kernel;
 
@@ -367,6 +370,20 @@ fot_status fot_gen_uniform(uint64_t n, uint64_t seed, double lo, double hi,
                            double *out);
 fot_status fot_gen_normal(uint64_t n, uint64_t seed, double mean, double sd,
                           double *out);
+
+/* Triangle mesh of the shallow-water edge loop (kernels/volna.hpp shapes):
+ * an (nx+1) x (ny+1) node grid perturbed by U(-0.2,0.2), each quad split on a
+ * random diagonal.  C = 2 nx ny cells, E = nx(ny+1) + (nx+1)ny + nx ny edges
+ * (boundary edges included).  edge_cells[2E]: (left, right), (cell, cell) on
+ * the boundary; is_boundary[E] 0.0 / 1.0; normals SOA [2][E], unit, left to
+ * right (outward on the boundary); fluxes SOA [3][E] and bathy SOA [2][E]
+ * N(0, 0.1); volume[C] triangle areas; state AOS [C][4]: h U[0.5,2),
+ * hu, hv N(0, 0.1), zb U(-1,0].  shuffle: bit 1 cells, bit 2 edges.  Any
+ * output may be NULL.  No reference counterpart (the reference reads Volna
+ * meshes from files, volna.cu:10-36). */
+fot_status fot_gen_tri_mesh(uint32_t nx, uint32_t ny, uint64_t seed, uint32_t shuffle,
+                            uint32_t *edge_cells, double *is_boundary, double *normals,
+                            double *fluxes, double *bathy, double *volume, double *state);
 
 /* ------------------------------------------------------------------ */
 /* Loop engine (MI355X-native fast path)                                 */

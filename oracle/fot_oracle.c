@@ -186,6 +186,29 @@ static void compute_inc(const oracle_problem *p, uint32_t ind, double *inc) {
         inc[j * 3 + i] = F[(size_t)(i * 8 + j) * cs];
     break;
   }
+  case ORACLE_VOLNA: {
+    /* volna::StepSeq (kernels/volna.hpp:27-78) with volna_func
+     * (kernels/volna_func.hpp:16-36), in FP64: volumes (point data 1)
+     * through mapping-0 ids; cell data fluxes (3), bathymetry source (2),
+     * edge normals (2) and the right-boundary flag (0.0 / 1.0).  The
+     * increments start at zero and take the reference's -= / +=. */
+    const uint32_t *m0 = p->cell_to_node[0];
+    const double *vol = p->point_data[1];
+    const uint32_t cs = p->num_cells;
+    const double *fl = p->cell_data[0] + ind, *bs = p->cell_data[1] + ind;
+    const double *nm = p->cell_data[2] + ind;
+    const double v0 = vol[m0[2 * ind]], v1 = vol[m0[2 * ind + 1]];
+    for (int k = 0; k < 8; ++k) inc[k] = 0.0;
+    inc[0] -= fl[0] / v0;
+    inc[1] -= (fl[cs] + bs[0] * nm[0]) / v0;
+    inc[2] -= (fl[2 * cs] + bs[0] * nm[cs]) / v0;
+    if (p->cell_data[3][ind] == 0.0) {
+      inc[4] += fl[0] / v1;
+      inc[5] += (fl[cs] + bs[cs] * nm[0]) / v1;
+      inc[6] += (fl[2 * cs] + bs[cs] * nm[cs]) / v1;
+    }
+    break;
+  }
   default:
     break;
   }
