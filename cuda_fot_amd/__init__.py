@@ -6,9 +6,9 @@ is a thin Python binding over that library; it has no CPU fallback.
 """
 from . import capi, meshgen, plan  # noqa: F401
 from .capi import FotError, InvalidInputFile, ColourOverflow  # noqa: F401
-from .device import DeviceArray, Loop, Stream, Timer, synchronize  # noqa: F401
-from .problem import Problem, res_calc_problem, volna_problem  # noqa: F401
+from .device import Batch, DeviceArray, Loop, Stream, Timer, synchronize  # noqa: F401
+from .problem import Problem, gpu_batch, res_calc_problem, volna_problem  # noqa: F401
 
 __all__ = ["capi", "meshgen", "plan", "FotError", "InvalidInputFile", "ColourOverflow",
-           "DeviceArray", "Loop", "Stream", "Timer", "synchronize", "Problem",
-           "res_calc_problem", "volna_problem"]
+           "Batch", "DeviceArray", "Loop", "Stream", "Timer", "synchronize", "Problem",
+           "gpu_batch", "res_calc_problem", "volna_problem"]

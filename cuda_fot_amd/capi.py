@@ -104,6 +104,12 @@ class LoopStats(C.Structure):
                 ("redundancy", C.c_double)]
 
 
+class BatchStats(C.Structure):
+    _fields_ = [("num_problems", C.c_uint32), ("num_work_items", C.c_uint32),
+                ("grid", C.c_uint32), ("threads", C.c_uint32), ("lds_bytes", C.c_uint32),
+                ("uses_graph", C.c_uint32)]
+
+
 class OplanView(C.Structure):
     _fields_ = [("mesh_dim", C.c_uint32), ("num_blocks", C.c_uint32),
                 ("num_positions", C.c_uint32), ("num_cached", C.c_uint32),
@@ -140,6 +146,7 @@ U32 = C.c_uint32
 U64 = C.c_uint64
 I32 = C.c_int
 D = C.c_double
+F32 = C.c_float
 
 # name -> argtypes; every entry point of include/fot_capi.h
 _PROTOS = {
@@ -201,6 +208,22 @@ _PROTOS = {
     "fot_loop_run": [P, U32, P],
     "fot_loop_check": [P],
     "fot_loop_stats_get": [P, P],
+    "fot_batch_create": [P, U32, P, P, U32, P],
+    "fot_batch_destroy": [P],
+    "fot_batch_step": [P, P],
+    "fot_batch_run": [P, U32, P],
+    "fot_batch_stats_get": [P, P],
+    "fot_loop_features": [P, P, P],
+    "fot_autotune_measure": [P, P, P, U32, U32, U32, P],
+    "fot_mlp_create": [U32, U32, U32, U64, P],
+    "fot_mlp_destroy": [P],
+    "fot_mlp_num_params": [P, P],
+    "fot_mlp_get_params": [P, P, U64],
+    "fot_mlp_set_params": [P, P, U64],
+    "fot_mlp_forward": [P, P, U32, P],
+    "fot_mlp_loss_grad": [P, P, P, P, U32, P, P],
+    "fot_mlp_train": [P, P, P, P, U32, U32, F32, F32, P],
+    "fot_gemm_f32": [P, P, P, U32, U32, U32, C.c_int, C.c_int, P],
 }
 
 _RESTYPES = {"fot_status_string": C.c_char_p, "fot_last_error_message": C.c_char_p,

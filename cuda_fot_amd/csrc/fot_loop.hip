@@ -9,21 +9,15 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <numeric>
 #include <type_traits>
 #include <vector>
 
 #include "fot_dispatch.hpp"
+#include "fot_hip.hpp"
 
 using namespace fot;
 
-#define FOT_HIP(call)                                                          \
-  do {                                                                         \
-    const hipError_t e_ = (call);                                              \
-    if (e_ != hipSuccess)                                                      \
-      return fail(e_ == hipErrorOutOfMemory ? FOT_ERR_OUT_OF_MEMORY            \
-                                            : FOT_ERR_HIP,                     \
-                  "%s: %s", #call, hipGetErrorString(e_));                     \
-  } while (0)
 
 namespace {
 
@@ -353,9 +347,8 @@ __global__ __launch_bounds__(1024) void res_calc_hier_staged_kernel(EngineArgs a
 // shares a written point with another, so there is no order between blocks.
 
 template <class F, bool SOA>
-__global__ __launch_bounds__(1024) void eng_owner_kernel(EngineArgs a, HierDev h, uint32_t nb) {
-  extern __shared__ double lds[];
-  const uint32_t b = xcd_block(blockIdx.x, nb);
+__device__ __forceinline__ void owner_block(const EngineArgs &a, const HierDev &h, uint32_t b,
+                                            double *lds) {
   const uint32_t tid = threadIdx.x;
   const uint32_t e0 = h.blk_cell_off[b], ne = h.blk_cell_off[b + 1] - e0;
   const uint32_t c0 = h.blk_cache_off[b];
@@ -391,6 +384,61 @@ __global__ __launch_bounds__(1024) void eng_owner_kernel(EngineArgs a, HierDev h
 #pragma unroll
     for (uint32_t d = 0; d < F::TD; ++d) a.out[pidx<SOA>(np, g, F::TD, d)] = cache[d * nw + i];
   }
+}
+
+template <class F, bool SOA>
+__global__ __launch_bounds__(1024) void eng_owner_kernel(EngineArgs a, HierDev h, uint32_t nb) {
+  extern __shared__ double lds[];
+  owner_block<F, SOA>(a, h, xcd_block(blockIdx.x, nb), lds);
+}
+
+// One problem of a batch: its engine arguments and owner-compute plan.
+struct BatchProblem {
+  EngineArgs args;
+  HierDev hier;
+};
+
+// Batched owner-compute step: a persistent grid whose workgroups take
+// (problem, block) items from a device queue until it runs dry.  queue[0] is
+// the next item, queue[1] counts finished workgroups; the last workgroup
+// resets both, so the next launch (or graph replay) starts from zero with no
+// memset.
+template <class F, bool SOA>
+__global__ __launch_bounds__(1024) void batch_owner_kernel(const BatchProblem *__restrict__ probs,
+                                                           const uint2 *__restrict__ work,
+                                                           uint32_t nwork,
+                                                           uint32_t *__restrict__ queue) {
+  extern __shared__ double lds[];
+  __shared__ uint32_t item;
+  for (;;) {
+    __syncthreads(); // the previous item's LDS and `item` are no longer read
+    if (threadIdx.x == 0) item = atomicAdd(&queue[0], 1u);
+    __syncthreads();
+    const uint32_t w = item;
+    if (w >= nwork) break;
+    const uint2 pb = work[w];
+    owner_block<F, SOA>(probs[pb.x].args, probs[pb.x].hier, pb.y, lds);
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
+      atomicExch(&queue[0], 0u);
+      atomicExch(&queue[1], 0u);
+    }
+  }
+}
+
+// Batched copies: item i copies count doubles (<= kCopyChunk) src -> dst.
+struct CopyItem {
+  double *dst;
+  const double *src;
+  uint32_t count, pad;
+};
+constexpr uint32_t kCopyChunk = 4096;
+
+__global__ __launch_bounds__(256) void batch_copy_kernel(const CopyItem *__restrict__ items) {
+  const CopyItem it = items[blockIdx.x];
+  for (uint32_t i = threadIdx.x; i < it.count; i += blockDim.x) it.dst[i] = it.src[i];
 }
 
 // Per-block record of the owner-compute res_calc kernel: one scalar load.
@@ -1208,4 +1256,227 @@ extern "C" fot_status fot_loop_stats_get(const fot_loop *loop, fot_loop_stats *s
   if (!loop || !stats) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
   *stats = loop->stats;
   return FOT_OK;
+}
+
+// ---------------------------------------------------------------------
+// Batched loops: many problems of one functor, one persistent launch per
+// step (no reference counterpart).
+// ---------------------------------------------------------------------
+struct fot_batch {
+  fot_functor f{};
+  int soa = 0;
+  std::vector<fot_loop *> loops;
+  std::vector<void *> owned;
+  BatchProblem *probs = nullptr;
+  uint2 *work = nullptr;
+  uint32_t nwork = 0;
+  uint32_t *queue = nullptr;
+  CopyItem *copy_in = nullptr, *copy_back = nullptr; // out <- in, in <- out
+  uint32_t ncopy = 0;
+  uint32_t grid = 0, threads = 0, lds_bytes = 0;
+  bool jacobi = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  ~fot_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (fot_loop *l : loops) delete l;
+    for (void *p : owned) (void)hipFree(p);
+  }
+  template <class T> fot_status upload(const std::vector<T> &v, T **dev) {
+    void *p = nullptr;
+    FOT_HIP(hipMalloc(&p, std::max<size_t>(1, v.size()) * sizeof(T)));
+    owned.push_back(p);
+    if (!v.empty()) FOT_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    *dev = static_cast<T *>(p);
+    return FOT_OK;
+  }
+  fot_status launch_step(hipStream_t s) {
+    if (nwork == 0) return FOT_OK;
+    return dispatch(&f, soa, [&]<class F, bool SOA>() -> fot_status {
+      hipLaunchKernelGGL((batch_owner_kernel<F, SOA>), dim3(grid), dim3(threads), lds_bytes, s,
+                         probs, work, nwork, queue);
+      FOT_HIP(hipGetLastError());
+      return FOT_OK;
+    });
+  }
+  fot_status launch_copy(const CopyItem *items, hipStream_t s) {
+    if (ncopy == 0) return FOT_OK;
+    hipLaunchKernelGGL(batch_copy_kernel, dim3(ncopy), dim3(256), 0, s, items);
+    FOT_HIP(hipGetLastError());
+    return FOT_OK;
+  }
+};
+
+extern "C" fot_status fot_batch_create(const fot_functor *f, uint32_t num_problems,
+                                       const fot_problem_desc *descs,
+                                       const fot_loop_config *cfgs, uint32_t use_graph,
+                                       fot_batch **out) {
+  if (!f || !descs || !cfgs || !out || num_problems == 0)
+    return fail(FOT_ERR_INVALID_ARGUMENT, "null argument or empty batch");
+  *out = nullptr;
+  std::unique_ptr<fot_batch> B(new fot_batch);
+  B->f = *f;
+  B->soa = descs[0].soa;
+  B->jacobi = reads_iterate(*f);
+  fot_status st;
+  std::vector<BatchProblem> host(num_problems);
+  std::vector<CopyItem> cin, cback;
+  for (uint32_t i = 0; i < num_problems; ++i) {
+    if (descs[i].soa != B->soa)
+      return fail(FOT_ERR_INVALID_ARGUMENT, "problem %u: every problem needs the same layout", i);
+    fot_loop_config c = cfgs[i];
+    c.strategy = FOT_STRATEGY_HIERARCHICAL;
+    c.flags = (c.flags & FOT_LOOP_POINT_PARTITION) | FOT_LOOP_OWNER_COMPUTE;
+    c.use_graph = 0;
+    fot_loop *l = nullptr;
+    if ((st = fot_loop_create(f, &descs[i], &c, &l)) != FOT_OK) return st;
+    B->loops.push_back(l);
+    host[i].args = l->plan_args;
+    host[i].hier = l->hier;
+    B->threads = std::max(B->threads, l->threads);
+    B->lds_bytes = std::max(B->lds_bytes, l->lds_bytes);
+    const size_t n = size_t(descs[i].num_points[0]) * descs[i].point_dim[0];
+    double *in = const_cast<double *>(descs[i].point_data[0]);
+    for (size_t o = 0; o < n; o += kCopyChunk) {
+      const uint32_t cnt = uint32_t(std::min<size_t>(kCopyChunk, n - o));
+      cin.push_back({descs[i].point_data_out + o, in + o, cnt, 0});
+      cback.push_back({in + o, descs[i].point_data_out + o, cnt, 0});
+    }
+  }
+  // work items, largest problems first (longest jobs first balances the tail)
+  std::vector<uint32_t> order(num_problems);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    return descs[a].num_cells > descs[b].num_cells;
+  });
+  std::vector<uint2> work;
+  for (uint32_t i : order) {
+    const uint32_t nb = B->loops[i]->colour_blk_off.back();
+    for (uint32_t b = 0; b < nb; ++b) work.push_back(make_uint2(i, b));
+  }
+  B->nwork = uint32_t(work.size());
+  B->ncopy = uint32_t(cin.size());
+  if ((st = B->upload(host, &B->probs)) != FOT_OK) return st;
+  if ((st = B->upload(work, &B->work)) != FOT_OK) return st;
+  if ((st = B->upload(cin, &B->copy_in)) != FOT_OK) return st;
+  if ((st = B->upload(cback, &B->copy_back)) != FOT_OK) return st;
+  if ((st = B->upload(std::vector<uint32_t>{0u, 0u}, &B->queue)) != FOT_OK) return st;
+  if (B->lds_bytes > 160u * 1024u)
+    return fail(FOT_ERR_UNSUPPORTED, "batch needs %u B of LDS", B->lds_bytes);
+  // persistent grid: as many workgroups as fit on the device at once
+  int dev = 0, cus = 0, per_cu = 0;
+  FOT_HIP(hipGetDevice(&dev));
+  FOT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  st = dispatch(f, B->soa, [&]<class F, bool SOA>() -> fot_status {
+    FOT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void *>(&batch_owner_kernel<F, SOA>), int(B->threads),
+        size_t(B->lds_bytes)));
+    return FOT_OK;
+  });
+  if (st != FOT_OK) return st;
+  B->grid = std::max<uint32_t>(1, std::min<uint32_t>(B->nwork, uint32_t(std::max(1, per_cu) * cus)));
+  FOT_HIP(hipDeviceSynchronize());
+  if (use_graph) {
+    // one timestep: the batched increments, then in <- out for every problem
+    hipStream_t cs = nullptr;
+    FOT_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      st = B->launch_step(cs);
+      if (st == FOT_OK) st = B->launch_copy(B->copy_back, cs);
+      e = hipStreamEndCapture(cs, &B->graph);
+    }
+    if (e == hipSuccess && st == FOT_OK) e = hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0);
+    (void)hipStreamDestroy(cs);
+    if (st != FOT_OK) return st;
+    if (e != hipSuccess) return fail(FOT_ERR_HIP, "batch graph: %s", hipGetErrorString(e));
+  }
+  *out = B.release();
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_batch_destroy(fot_batch *b) {
+  delete b;
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_batch_step(fot_batch *b, void *stream) {
+  if (!b) return fail(FOT_ERR_INVALID_ARGUMENT, "null batch");
+  return b->launch_step(static_cast<hipStream_t>(stream));
+}
+
+extern "C" fot_status fot_batch_run(fot_batch *b, uint32_t iters, void *stream) {
+  if (!b) return fail(FOT_ERR_INVALID_ARGUMENT, "null batch");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  fot_status st = b->launch_copy(b->copy_in, s);
+  for (uint32_t it = 0; it < iters && st == FOT_OK; ++it) {
+    if (b->exec) {
+      FOT_HIP(hipGraphLaunch(b->exec, s));
+    } else {
+      st = b->launch_step(s);
+      if (st == FOT_OK && (b->jacobi || it + 1 == iters)) st = b->launch_copy(b->copy_back, s);
+    }
+  }
+  if (st != FOT_OK) return st;
+  FOT_HIP(hipStreamSynchronize(s));
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_batch_stats_get(const fot_batch *b, fot_batch_stats *s) {
+  if (!b || !s) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
+  s->num_problems = uint32_t(b->loops.size());
+  s->num_work_items = b->nwork;
+  s->grid = b->grid;
+  s->threads = b->threads;
+  s->lds_bytes = b->lds_bytes;
+  s->uses_graph = b->exec ? 1u : 0u;
+  return FOT_OK;
+}
+
+// ---------------------------------------------------------------------
+// Autotuner measurement: each candidate config planned and timed on a
+// private stream with HIP events (no reference counterpart).
+// ---------------------------------------------------------------------
+extern "C" fot_status fot_autotune_measure(const fot_functor *f, const fot_problem_desc *desc,
+                                           const fot_loop_config *cands, uint32_t n,
+                                           uint32_t warmup, uint32_t steps, float *times_ms) {
+  if (!f || !desc || !cands || !times_ms || n == 0 || steps == 0)
+    return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  FOT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  std::unique_ptr<std::remove_pointer<hipStream_t>::type, hipError_t (*)(hipStream_t)> sg(
+      s, hipStreamDestroy);
+  FOT_HIP(hipEventCreate(&e0));
+  FOT_HIP(hipEventCreate(&e1));
+  fot_status result = FOT_OK;
+  for (uint32_t i = 0; i < n && result == FOT_OK; ++i) {
+    times_ms[i] = -1.f;
+    fot_loop_config c = cands[i];
+    c.use_graph = 0;
+    fot_loop *l = nullptr;
+    const fot_status st = fot_loop_create(f, desc, &c, &l);
+    if (st == FOT_ERR_COLOUR_OVERFLOW || st == FOT_ERR_UNSUPPORTED) continue; // not plannable
+    if (st != FOT_OK) {
+      result = st;
+      break;
+    }
+    std::unique_ptr<fot_loop> lg(l);
+    for (uint32_t w = 0; w < warmup && result == FOT_OK; ++w) result = l->enqueue(s);
+    hipError_t he = hipEventRecord(e0, s);
+    for (uint32_t k = 0; k < steps && result == FOT_OK && he == hipSuccess; ++k)
+      result = l->enqueue(s);
+    if (he == hipSuccess) he = hipEventRecord(e1, s);
+    if (he == hipSuccess) he = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+    if (he != hipSuccess) result = fail(FOT_ERR_HIP, "autotune timing: %s", hipGetErrorString(he));
+    if (result == FOT_OK) result = fot_loop_check(l);
+    times_ms[i] = ms / float(steps);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return result;
 }

@@ -7,6 +7,7 @@
 #include <string>
 
 #include "fot_internal.hpp"
+#include "fot_hip.hpp"
 
 namespace {
 thread_local std::string g_message;
@@ -26,14 +27,6 @@ fot_status fail(fot_status status, const char *fmt, ...) {
 
 using fot::fail;
 
-#define FOT_HIP(call)                                                          \
-  do {                                                                         \
-    const hipError_t e_ = (call);                                              \
-    if (e_ != hipSuccess)                                                      \
-      return fail(e_ == hipErrorOutOfMemory ? FOT_ERR_OUT_OF_MEMORY            \
-                                            : FOT_ERR_HIP,                     \
-                  "%s: %s", #call, hipGetErrorString(e_));                     \
-  } while (0)
 
 extern "C" const char *fot_status_string(fot_status s) {
   switch (s) {

@@ -98,6 +98,82 @@ class Timer:
             pass
 
 
+def make_desc(maps, num_points, point_data, out, cell_data=(), soa=False):
+    """A fot_problem_desc over host maps and device point / cell data;
+    returns (desc, keepalive)."""
+    maps = [capi.u32(m) for m in maps]
+    d = capi.ProblemDesc()
+    d.num_cells = maps[0].shape[0]
+    d.num_mappings = len(maps)
+    for k, m in enumerate(maps):
+        d.cell_to_node[k] = ptr(m)
+        d.mesh_dim[k] = m.shape[1]
+        d.num_points[k] = int(num_points[k])
+        pd = point_data[k]
+        d.point_data[k] = pd.ptr if pd is not None else None
+        d.point_dim[k] = pd.shape[1] if pd is not None and len(pd.shape) > 1 else 1
+        if soa and pd is not None and len(pd.shape) > 1:
+            d.point_dim[k] = pd.shape[0]
+    d.num_cell_data = len(cell_data)
+    for k, cd in enumerate(cell_data):
+        d.cell_data[k] = cd.ptr
+        d.cell_dim[k] = cd.shape[0] if len(cd.shape) > 1 else 1
+    d.point_data_out = out.ptr
+    d.soa = 1 if soa else 0
+    return d, maps
+
+
+class Batch:
+    """Many problems of one functor run by one persistent launch per step
+    (fot_batch).  Each problem: dict(maps, num_points, point_data, out,
+    cell_data, soa, block_size, point_partition=None) with device arrays."""
+
+    def __init__(self, func: capi.Functor, problems, use_graph=False):
+        n = len(problems)
+        descs = (capi.ProblemDesc * n)()
+        cfgs = (capi.LoopConfig * n)()
+        self._keep = []
+        for i, pr in enumerate(problems):
+            d, keep = make_desc(pr["maps"], pr["num_points"], pr["point_data"], pr["out"],
+                                pr.get("cell_data", ()), pr.get("soa", False))
+            descs[i] = d
+            self._keep.append(keep)
+            cfgs[i].strategy = capi.STRATEGY_HIERARCHICAL
+            cfgs[i].block_size = int(pr.get("block_size", 128))
+            part = pr.get("point_partition")
+            if part is not None:
+                part = capi.u32(part)
+                self._keep.append(part)
+                cfgs[i].partition = ptr(part)
+                cfgs[i].flags = capi.LOOP_POINT_PARTITION
+        h = C.c_void_p()
+        call("fot_batch_create", C.byref(func), n, descs, cfgs, 1 if use_graph else 0,
+             C.byref(h))
+        self.handle = h.value
+
+    def step(self, stream=None) -> None:
+        call("fot_batch_step", self.handle, stream)
+
+    def run(self, iters: int, stream=None) -> None:
+        call("fot_batch_run", self.handle, iters, stream)
+
+    def stats(self) -> dict:
+        s = capi.BatchStats()
+        call("fot_batch_stats_get", self.handle, C.byref(s))
+        return {f: getattr(s, f) for f, _ in capi.BatchStats._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib.fot_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Loop:
     """A planned loop (fot_loop): strategy, plan on the device, optional graph.
 

@@ -11,7 +11,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import capi, plan
-from .device import DeviceArray, Loop
+from .device import Batch, DeviceArray, Loop
 
 DEFAULT_BLOCK_SIZE = 128  # problem.hpp:16
 
@@ -195,3 +195,25 @@ def volna_problem(mesh: dict, block_size: int = DEFAULT_BLOCK_SIZE, soa: bool = 
     maps = [mesh["edge_cells"], mesh["edge_cells"]]
     cell = [mesh["fluxes"], mesh["bathy"], mesh["normals"], mesh["is_boundary"][None, :]]
     return Problem(maps, [nc, nc], [state, mesh["volume"]], cell, block_size, soa)
+
+
+def gpu_batch(problems, func: capi.Functor, iters: int, use_graph: bool = False) -> dict:
+    """Runs `iters` iterations of every Problem in one batch (one persistent
+    launch per step) and writes each point data 0 back.  Problems with a
+    point partition (partition_points) use it; the others the engine's."""
+    specs, keep = [], []
+    for p in problems:
+        dev = [None if d is None else DeviceArray.from_host(d) for d in p.point_data]
+        out = DeviceArray(p.point_data[0].shape, np.float64)
+        cdev = [DeviceArray.from_host(c) for c in p.cell_data]
+        keep.append((dev, out, cdev))
+        specs.append(dict(maps=p.maps, num_points=p.num_points, point_data=dev, out=out,
+                          cell_data=cdev, soa=p.soa, block_size=p.block_size,
+                          point_partition=p.point_partition))
+    b = Batch(func, specs, use_graph=use_graph)
+    b.run(iters)
+    for p, (dev, _, _) in zip(problems, keep):
+        p.point_data[0] = dev[0].download()
+    stats = b.stats()
+    b.close()
+    return stats

@@ -458,6 +458,85 @@ typedef struct {
 } fot_loop_stats;
 fot_status fot_loop_stats_get(const fot_loop *loop, fot_loop_stats *stats);
 
+/* ------------------------------------------------------------------ */
+/* Batched loops (no reference counterpart): many problems of one        */
+/* functor, each planned as an owner-compute hierarchical loop with its  */
+/* own config, run together by ONE persistent launch per step whose      */
+/* workgroups take (problem, block) items from a device work queue.      */
+/* ------------------------------------------------------------------ */
+typedef struct fot_batch fot_batch;
+/* descs/cfgs: one per problem (same functor and layout).  Each cfg's
+ * block_size / partition (+ FOT_LOOP_POINT_PARTITION) shape that problem's
+ * plan; the strategy is always owner-compute hierarchical.  use_graph != 0
+ * captures one timestep (the batched increments, then point_data[0] = out
+ * for every problem) in a hipGraph that fot_batch_run replays. */
+fot_status fot_batch_create(const fot_functor *f, uint32_t num_problems,
+                            const fot_problem_desc *descs, const fot_loop_config *cfgs,
+                            uint32_t use_graph, fot_batch **batch);
+fot_status fot_batch_destroy(fot_batch *batch);
+/* One step: every problem's increments into its point_data_out. */
+fot_status fot_batch_step(fot_batch *batch, void *stream);
+/* `iters` iterations with fot_loop_run's semantics for every problem. */
+fot_status fot_batch_run(fot_batch *batch, uint32_t iters, void *stream);
+typedef struct {
+  uint32_t num_problems, num_work_items, grid, threads, lds_bytes, uses_graph;
+} fot_batch_stats;
+fot_status fot_batch_stats_get(const fot_batch *batch, fot_batch_stats *stats);
+
+/* ------------------------------------------------------------------ */
+/* Autotuner support (no reference counterpart)                          */
+/* ------------------------------------------------------------------ */
+#define FOT_NUM_FEATURES 24
+/* Cheap features of a loop (no plan is built), out[FOT_NUM_FEATURES]:
+ *  0 log2 cells, 1 log2 points of mapping 0, 2 mean point degree,
+ *  3 mesh_dim, 4 point_dim, 5 cell-data doubles per cell, 6 point-data
+ *  doubles gathered per cell through mappings >= 1, 7 degree std / mean,
+ *  8 max degree / mean, 9-12 fractions of points with degree <= 2, 3-4,
+ *  5-8, > 8, 13 global colours (65 if 64 do not suffice), 14 mean cell
+ *  span / points, 15-16 fractions of cells with span < 256 / < 4096,
+ *  17 mean jump between consecutive cells' lowest ids / points, 18-19
+ *  fractions of jumps < 256 / < 4096, 20 log2 algorithmic bytes per
+ *  iteration, 21 res_calc, 22 reads its iterate (mine), 23 log2 target
+ *  bytes.  Host arrays only (point_data may be NULL). */
+fot_status fot_loop_features(const fot_functor *f, const fot_problem_desc *desc, float *out);
+/* Plans each candidate config and times `steps` steps after `warmup` on a
+ * private stream with HIP events: times_ms[i] = ms per step, or -1 when the
+ * config cannot be planned for this mesh (colour overflow, block too big). */
+fot_status fot_autotune_measure(const fot_functor *f, const fot_problem_desc *desc,
+                                const fot_loop_config *candidates, uint32_t n,
+                                uint32_t warmup, uint32_t steps, float *times_ms);
+
+/* ------------------------------------------------------------------ */
+/* Learned cost model (no reference counterpart): an FP32 MLP            */
+/* in -> hidden -> hidden -> out with ReLU, trained and evaluated on the */
+/* GPU by hand-written kernels (matrix products on MFMA).                */
+/* Parameters, flat: W1[hidden][in], b1, W2[hidden][hidden], b2,          */
+/* W3[out][hidden], b3.  Host arrays are row-major float.                 */
+/* ------------------------------------------------------------------ */
+typedef struct fot_mlp fot_mlp;
+fot_status fot_mlp_create(uint32_t in, uint32_t hidden, uint32_t out, uint64_t seed,
+                          fot_mlp **mlp);
+fot_status fot_mlp_destroy(fot_mlp *mlp);
+fot_status fot_mlp_num_params(const fot_mlp *mlp, uint64_t *n);
+fot_status fot_mlp_get_params(const fot_mlp *mlp, float *params, uint64_t n);
+/* also resets the optimiser state */
+fot_status fot_mlp_set_params(fot_mlp *mlp, const float *params, uint64_t n);
+/* y[rows][out] = MLP(x[rows][in]) */
+fot_status fot_mlp_forward(fot_mlp *mlp, const float *x, uint32_t rows, float *y);
+/* loss = sum(mask (y - t)^2) / sum(mask) (mask NULL: all ones) and its
+ * gradient with respect to the flat parameters */
+fot_status fot_mlp_loss_grad(fot_mlp *mlp, const float *x, const float *t, const float *mask,
+                             uint32_t rows, float *loss, float *grad);
+/* `epochs` full-batch Adam steps (beta 0.9 / 0.999, eps 1e-8, L2 weight
+ * decay); loss_per_epoch (nullable) gets the loss before each step */
+fot_status fot_mlp_train(fot_mlp *mlp, const float *x, const float *t, const float *mask,
+                         uint32_t rows, uint32_t epochs, float lr, float weight_decay,
+                         float *loss_per_epoch);
+/* The MFMA GEMM on DEVICE buffers: C[M][N] = op(A) op(B), A row-major M x K
+ * (K x M when trans_a), B row-major K x N (N x K when trans_b). */
+fot_status fot_gemm_f32(const float *A, const float *B, float *C, uint32_t M, uint32_t N,
+                        uint32_t K, int trans_a, int trans_b, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,83 @@
+"""Batched loops (fot_batch): many meshes of uneven size in one persistent
+launch per step, eager and replayed from a hipGraph.  The bar: every mesh's
+result is bit-identical to its own unbatched owner-compute loop with the
+same plan (same summation order), and so to the oracle's owner replay."""
+import copy
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cuda_fot_amd import capi, gpu_batch, meshgen, plan, volna_problem
+from helpers import airfoil_problem, assert_bitwise, assert_close_scaled, oracle_of
+
+pytestmark = pytest.mark.gpu
+
+VOLNA = capi.functor(capi.KERNEL_VOLNA)
+RES_CALC = capi.functor(capi.KERNEL_RES_CALC)
+SIZES = [(8, 6, 64), (30, 20, 128), (61, 41, 256), (17, 9, 96), (45, 33, 192)]
+
+
+def _volna_set(soa=False):
+    return [volna_problem(meshgen.tri_mesh(nx, ny, seed=10 + i), block_size=bs, soa=soa)
+            for i, (nx, ny, bs) in enumerate(SIZES)]
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_volna_batch_matches_unbatched(gpu, use_graph):
+    iters = 3
+    batch = _volna_set()
+    singles = _volna_set()
+    stats = gpu_batch(batch, VOLNA, iters, use_graph=use_graph)
+    assert stats["num_problems"] == len(SIZES) and stats["uses_graph"] == int(use_graph)
+    assert stats["grid"] <= stats["num_work_items"]
+    for i, (b, s) in enumerate(zip(batch, singles)):
+        ref = copy.deepcopy(s)
+        s.gpu_loop(VOLNA, capi.STRATEGY_HIERARCHICAL, iters, flags=capi.LOOP_OWNER_COMPUTE)
+        assert_bitwise(b.point_data[0], s.point_data[0], f"batch vs single, mesh {i}")
+        o = oracle_of(ref, O.VOLNA)
+        O.loop_owner_replay(o, plan.owner_plan(ref.maps[0], ref.num_points[0],
+                                               ref.block_size), iters)
+        assert_bitwise(b.point_data[0], o.target, f"batch vs owner replay, mesh {i}")
+
+
+def test_volna_batch_graph_replays_repeat(gpu):
+    # two runs of the same graph-backed batch: the work queue resets itself
+    from cuda_fot_amd import Batch, DeviceArray
+    probs = _volna_set(soa=True)
+    specs, dev0 = [], []
+    for p in probs:
+        dev = [DeviceArray.from_host(d) for d in p.point_data]
+        out = DeviceArray(p.point_data[0].shape, np.float64)
+        cdev = [DeviceArray.from_host(c) for c in p.cell_data]
+        specs.append(dict(maps=p.maps, num_points=p.num_points, point_data=dev, out=out,
+                          cell_data=cdev, soa=True, block_size=p.block_size))
+        dev0.append((dev, out, cdev))
+    b = Batch(VOLNA, specs, use_graph=True)
+    b.run(2)
+    first = [d[0][0].download() for d in dev0]
+    b.run(2)
+    second = [d[0][0].download() for d in dev0]
+    b.close()
+    ref = _volna_set(soa=True)
+    for i, (p, f, s) in enumerate(zip(ref, first, second)):
+        o = oracle_of(p, O.VOLNA)
+        S = o.loop_serial(4, with_scale=True)
+        assert_close_scaled(s, o.target, S, what=f"4 iterations over two runs, mesh {i}")
+        assert not np.array_equal(f, s)
+
+
+def test_res_calc_batch_matches_unbatched(gpu):
+    iters = 2
+    def make():
+        ps = [airfoil_problem(nx=48, ny=24, res0_seed=3, block_size=192),
+              airfoil_problem(nx=90, ny=40, seed=2, res0_seed=4, block_size=128)]
+        for p in ps:
+            p.partition_points(p.block_size)
+        return ps
+    batch, singles = make(), make()
+    gpu_batch(batch, RES_CALC, iters)
+    for i, (b, s) in enumerate(zip(batch, singles)):
+        s.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, iters,
+                   flags=capi.LOOP_OWNER_COMPUTE | capi.LOOP_TWO_PER_THREAD)
+        assert_bitwise(b.point_data[0], s.point_data[0], f"res_calc batch vs single, mesh {i}")
