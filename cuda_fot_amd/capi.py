@@ -202,6 +202,8 @@ _PROTOS = {
     "fot_gen_uniform": [U64, U64, D, D, P],
     "fot_gen_normal": [U64, U64, D, D, P],
     "fot_gen_tri_mesh": [U32, U32, U64, U32, P, P, P, P, P, P, P],
+    "fot_gen_hex_faces": [U32, U32, U32, U64, U32, P, P, P],
+    "fot_mini_aero_face_flux": [P, P, U32, P, U32, P],
     "fot_loop_create": [P, P, P, P],
     "fot_loop_destroy": [P],
     "fot_loop_step": [P, P],

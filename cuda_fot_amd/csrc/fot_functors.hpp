@@ -166,6 +166,191 @@ struct HexScatterF {
   }
 };
 
+// ---- mini_aero face flux (kernels/mini_aero_func.hpp:11-85 with
+// kernels/mini_aero_utils.hpp) ------------------------------------------------
+// Cell state reconstructed to the face with limited gradients, a Roe flux
+// with the entropy fix, minus the viscous flux of the face-averaged state.
+// Every operation is in the reference's order and association (the constants
+// (gamma - 1), 0.5 * a * b, the zero-initialised accumulators included), so
+// host and device give the same bits.  Host + device: libfot's CPU entry
+// point (fot_cpu_cell) runs the same code.
+#define FOT_HD __host__ __device__ __forceinline__
+namespace ma {
+
+FOT_HD void to_primitive(const double *U, uint32_t s, double *V) {
+  const double gamma = 1.4, rgas = 287.05;
+  const double r = U[0];
+  const double ri = 1.0 / r;
+  const double u = U[s] * ri, v = U[2 * s] * ri, w = U[3 * s] * ri;
+  const double k = 0.5 * (u * u + v * v + w * w);
+  const double e = U[4 * s] * ri - k;
+  V[0] = r, V[1] = u, V[2] = v, V[3] = w;
+  V[4] = e * (gamma - 1.0) / rgas;
+}
+
+// y = A x for a 5x5 row-major A, accumulated from zero
+FOT_HD void mat5(const double *A, const double *x, double *y) {
+  for (int i = 0; i < 5; ++i) {
+    double acc = 0.0 * 0.0;
+    for (int j = 0; j < 5; ++j) acc += 1.0 * A[5 * i + j] * x[j];
+    y[i] = acc;
+  }
+}
+
+FOT_HD void roe_flux(const double *pl, const double *pr, const double *n, const double *t,
+                     const double *b, double *flux) {
+  const double efix_u = 0.1, efix_c = 0.1, gm1 = 0.4;
+  const double rl = pl[0], ul = pl[1], vl = pl[2], wl = pl[3];
+  const double rr = pr[0], ur = pr[1], vr = pr[2], wr = pr[3];
+  const double pres_l = rl * 287.05 * pl[4], pres_r = rr * 287.05 * pr[4];
+  const double h_l = 1004.0 * pl[4], h_r = 1004.0 * pr[4];
+  const double H_l = h_l + 0.5 * (ul * ul + vl * vl + wl * wl);
+  const double H_r = h_r + 0.5 * (ur * ur + vr * vr + wr * wr);
+  const double mf_l = rl * (n[0] * ul + n[1] * vl + n[2] * wl);
+  const double mf_r = rr * (n[0] * ur + n[1] * vr + n[2] * wr);
+  const double psum = pres_l + pres_r;
+  flux[0] = 0.5 * (mf_l + mf_r);
+  flux[1] = 0.5 * (mf_l * ul + mf_r * ur + n[0] * psum);
+  flux[2] = 0.5 * (mf_l * vl + mf_r * vr + n[1] * psum);
+  flux[3] = 0.5 * (mf_l * wl + mf_r * wr + n[2] * psum);
+  flux[4] = 0.5 * (mf_l * H_l + mf_r * H_r);
+  const double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  const double tn = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+  const double bn = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  const double nu[3] = {n[0] / nn, n[1] / nn, n[2] / nn};
+  const double tu[3] = {t[0] / tn, t[1] / tn, t[2] / tn};
+  const double bu[3] = {b[0] / bn, b[1] / bn, b[2] / bn};
+  const double denom = 1.0 / (sqrt(rl) + sqrt(rr));
+  const double alpha = sqrt(rl) * denom;
+  const double beta = 1.0 - alpha;
+  const double u = alpha * ul + beta * ur, v = alpha * vl + beta * vr, w = alpha * wl + beta * wr;
+  const double h = alpha * h_l + beta * h_r +
+                   0.5 * alpha * beta *
+                       ((ur - ul) * (ur - ul) + (vr - vl) * (vr - vl) + (wr - wl) * (wr - wl));
+  const double c = sqrt(gm1 * h);
+  const double vn = u * nu[0] + v * nu[1] + w * nu[2];
+  const double vt = u * tu[0] + v * tu[1] + w * tu[2];
+  const double vb = u * bu[0] + v * bu[1] + w * bu[2];
+  const double ke = 0.5 * (u * u + v * v + w * w);
+  const double c2i = 1.0 / (c * c);
+  const double hc2i = 0.5 * c2i;
+  double dq[5];
+  dq[0] = rr - rl;
+  dq[1] = rr * ur - rl * ul;
+  dq[2] = rr * vr - rl * vl;
+  dq[3] = rr * wr - rl * wl;
+  dq[4] = (rr * H_r - pres_r) - (rl * H_l - pres_l);
+  const double cbar = c * nn;
+  const double ubar = u * n[0] + v * n[1] + w * n[2];
+  const double cfl = fabs(ubar) + cbar;
+  const double eig1 = ubar + cbar, eig2 = ubar - cbar, eig3 = ubar;
+  double a1 = fabs(eig1), a2 = fabs(eig2), a3 = fabs(eig3);
+  const double epuc = efix_u * cfl, epcc = efix_c * cfl;
+  if (a1 < epcc) a1 = 0.5 * (eig1 * eig1 + epcc * epcc) / epcc;
+  if (a2 < epcc) a2 = 0.5 * (eig2 * eig2 + epcc * epcc) / epcc;
+  if (a3 < epuc) a3 = 0.5 * (eig3 * eig3 + epuc * epuc) / epuc;
+  const double ep3 = 0.5 * (eig3 + a3), em3 = 0.5 * (eig3 - a3);
+  const double eigp[5] = {0.5 * (eig1 + a1), 0.5 * (eig2 + a2), ep3, ep3, ep3};
+  const double eigm[5] = {0.5 * (eig1 - a1), 0.5 * (eig2 - a2), em3, em3, em3};
+  const double kh = ke - h;
+  const double L[25] = {gm1 * kh + c * (c - vn), c * nu[0] - gm1 * u, c * nu[1] - gm1 * v,
+                        c * nu[2] - gm1 * w, gm1,
+                        gm1 * kh + c * (c + vn), -c * nu[0] - gm1 * u, -c * nu[1] - gm1 * v,
+                        -c * nu[2] - gm1 * w, gm1,
+                        kh, -u, -v, -w, 1.0,
+                        -vt, tu[0], tu[1], tu[2], 0.0,
+                        -vb, bu[0], bu[1], bu[2], 0.0};
+  double ldq[5];
+  mat5(L, dq, ldq);
+  for (int j = 0; j < 5; ++j) ldq[j] = (eigp[j] - eigm[j]) * ldq[j];
+  const double he = h + ke;
+  const double R[25] = {hc2i, hc2i, -gm1 * c2i, 0.0, 0.0,
+                        (u + nu[0] * c) * hc2i, (u - nu[0] * c) * hc2i, -gm1 * u * c2i, tu[0],
+                        bu[0],
+                        (v + nu[1] * c) * hc2i, (v - nu[1] * c) * hc2i, -gm1 * v * c2i, tu[1],
+                        bu[1],
+                        (w + nu[2] * c) * hc2i, (w - nu[2] * c) * hc2i, -gm1 * w * c2i, tu[2],
+                        bu[2],
+                        (he + c * vn) * hc2i, (he - c * vn) * hc2i, (c * c - gm1 * he) * c2i, vt,
+                        vb};
+  double rl_dq[5];
+  mat5(R, ldq, rl_dq);
+  for (int i = 0; i < 5; ++i) flux[i] -= 0.5 * rl_dq[i];
+}
+
+FOT_HD void viscous_flux(const double g[5][3], const double *prim, const double *a,
+                         double *vf) {
+  const double T = prim[4];
+  const double mu = 1.458e-6 * T * sqrt(T) / (T + 110.4);
+  const double kc = mu * 1006.0 / 0.71;
+  for (int i = 0; i < 5; ++i) vf[i] = 0.0;
+  double div = 0.0;
+  for (int i = 0; i < 3; ++i) div += g[i + 1][i];
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) {
+      const double delta = (i == j) ? 1.0 : 0.0;
+      const double S = 0.5 * (g[i + 1][j] + g[j + 1][i]);
+      const double tij = S - div * delta / 3.;
+      vf[1 + i] += (2 * mu * tij) * a[j];
+      vf[4] += (2 * mu * tij) * prim[i + 1] * a[j];
+    }
+    vf[4] += kc * g[4][i] * a[i];
+  }
+}
+
+// flux of face `f`: cell rows L / R of 28 values (5 conservatives, 3
+// coordinates, 15 gradients, 5 limiters) with element stride s; face row
+// `fd` of 12 values (coordinates, normal, tangent, binormal), stride fs.
+FOT_HD void face_flux(const double *L, const double *R, uint32_t s, const double *fd,
+                      uint32_t fs, double *flux) {
+  double cl[5], cr[5], pl[5], pr[5];
+  for (int k = 0; k < 5; ++k) cl[k] = L[k * s], cr[k] = R[k * s];
+  to_primitive(cl, 1, pl);
+  to_primitive(cr, 1, pr);
+  for (int k = 0; k < 5; ++k) {
+    double gl = 0, gr = 0;
+    for (int d = 0; d < 3; ++d) {
+      gl += (fd[d * fs] - L[(5 + d) * s]) * L[(8 + k * 3 + d) * s];
+      gr += (fd[d * fs] - R[(5 + d) * s]) * R[(8 + k * 3 + d) * s];
+    }
+    pl[k] += gl * L[(23 + k) * s];
+    pr[k] += gr * R[(23 + k) * s];
+  }
+  double n[3], t[3], b[3];
+  for (int d = 0; d < 3; ++d) n[d] = fd[(3 + d) * fs], t[d] = fd[(6 + d) * fs], b[d] = fd[(9 + d) * fs];
+  roe_flux(pl, pr, n, t, b, flux);
+  double pf[5], gf[5][3];
+  for (int k = 0; k < 5; ++k) {
+    pf[k] = 0.5 * (pl[k] + pr[k]);
+    for (int d = 0; d < 3; ++d) gf[k][d] = 0.5 * (L[(8 + k * 3 + d) * s] + R[(8 + k * 3 + d) * s]);
+  }
+  double vf[5];
+  viscous_flux(gf, pf, n, vf);
+  for (int k = 0; k < 5; ++k) flux[k] -= vf[k];
+}
+
+} // namespace ma
+
+// mini_aero (kernels/mini_aero.hpp:43-117): point data 0 the cell flux (dim
+// 5, target), point data 1 the 28 cell values (through mapping-0 ids as the
+// reference reads them), cell data 0 the 12 face values.  left -= flux,
+// right += flux: the increments are -flux and +flux (x - f == x + (-f)).
+struct MiniAeroF {
+  static constexpr uint32_t MD = 2, TD = 5;
+  static constexpr bool SLOT_OUTER = false;
+  template <bool SOA, class A>
+  FOT_DEV static void compute(const A &a, uint32_t ind, const uint32_t *tgt, double *inc) {
+    const double *cv = a.pd(1);
+    const uint32_t s = SOA ? a.ps(1) : 1;
+    const double *L = cv + (SOA ? size_t(tgt[0]) : size_t(28) * tgt[0]);
+    const double *R = cv + (SOA ? size_t(tgt[1]) : size_t(28) * tgt[1]);
+    double flux[5];
+    ma::face_flux(L, R, s, a.cd(0) + ind, a.cs(), flux);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) inc[k] = -flux[k], inc[5 + k] = flux[k];
+  }
+};
+
 // volna (kernels/volna.hpp:27-78, volna_func.hpp:16-36) in FP64: the cell
 // volumes are point data 1, read through mapping-0 ids as the reference
 // does; cell data 0..3 = edge fluxes (3), bathymetry source (2), edge

@@ -332,3 +332,66 @@ extern "C" fot_status fot_gen_tri_mesh(uint32_t nx, uint32_t ny, uint64_t seed, 
   }
   return FOT_OK;
 }
+
+// mini_aero-style hex face mesh (declared in fot_capi.h)
+extern "C" fot_status fot_gen_hex_faces(uint32_t n1, uint32_t n2, uint32_t n3, uint64_t seed,
+                                        uint32_t shuffle, uint32_t *face_cells, double *face,
+                                        double *cells) {
+  if (n1 < 1 || n2 < 1 || n3 < 1) return fail(FOT_ERR_INVALID_ARGUMENT, "hex faces need n >= 1");
+  const uint64_t nc64 = uint64_t(n1) * n2 * n3;
+  const uint64_t nf64 = uint64_t(n1 - 1) * n2 * n3 + uint64_t(n1) * (n2 - 1) * n3 +
+                        uint64_t(n1) * n2 * (n3 - 1);
+  if (nf64 > 0xffffffffull || nc64 > 0xffffffffull)
+    return fail(FOT_ERR_INVALID_ARGUMENT, "hex face mesh too large");
+  const uint32_t nc = uint32_t(nc64), nf = uint32_t(nf64);
+  std::vector<uint32_t> cell_perm(nc), face_perm(nf);
+  if (shuffle & 2u) permutation(nc, stream_seed(seed, 2), cell_perm.data());
+  else std::iota(cell_perm.begin(), cell_perm.end(), 0u);
+  if (shuffle & 4u) permutation(nf, stream_seed(seed, 3), face_perm.data());
+  else std::iota(face_perm.begin(), face_perm.end(), 0u);
+  auto cid = [&](uint32_t i, uint32_t j, uint32_t k) {
+    return cell_perm[(size_t(k) * n2 + j) * n1 + i];
+  };
+  uint32_t f = 0;
+  // axis a: faces between cell (i,j,k) and its +a neighbour
+  for (int a = 0; a < 3; ++a) {
+    const uint32_t m1 = n1 - (a == 0), m2 = n2 - (a == 1), m3 = n3 - (a == 2);
+    for (uint32_t k = 0; k < m3; ++k)
+      for (uint32_t j = 0; j < m2; ++j)
+        for (uint32_t i = 0; i < m1; ++i) {
+          const uint32_t e = face_perm[f++];
+          if (face_cells) {
+            face_cells[2 * size_t(e)] = cid(i, j, k);
+            face_cells[2 * size_t(e) + 1] = cid(i + (a == 0), j + (a == 1), k + (a == 2));
+          }
+          if (face) {
+            const double c[3] = {i + 0.5 + 0.5 * (a == 0), j + 0.5 + 0.5 * (a == 1),
+                                 k + 0.5 + 0.5 * (a == 2)};
+            // normal e_a, tangent e_{a+1}, binormal e_{a+2}
+            for (int d = 0; d < 3; ++d) {
+              face[size_t(d) * nf + e] = c[d];
+              face[size_t(3 + d) * nf + e] = d == a ? 1.0 : 0.0;
+              face[size_t(6 + d) * nf + e] = d == (a + 1) % 3 ? 1.0 : 0.0;
+              face[size_t(9 + d) * nf + e] = d == (a + 2) % 3 ? 1.0 : 0.0;
+            }
+          }
+        }
+  }
+  if (cells) {
+    const double rho = 1.0, u = 50.0, v = 10.0, w = 5.0, T = 300.0;
+    const double e_int = 287.05 * T / 0.4;
+    const double U0[5] = {rho, rho * u, rho * v, rho * w,
+                          rho * (e_int + 0.5 * (u * u + v * v + w * w))};
+    SplitMix64 gq(stream_seed(seed, 5)), gg(stream_seed(seed, 6)), gl(stream_seed(seed, 7));
+    for (uint32_t k = 0; k < n3; ++k)
+      for (uint32_t j = 0; j < n2; ++j)
+        for (uint32_t i = 0; i < n1; ++i) {
+          double *r = cells + size_t(cid(i, j, k)) * 28;
+          for (int d = 0; d < 5; ++d) r[d] = U0[d] * (1.0 + 1e-2 * gq.normal());
+          r[5] = i + 0.5, r[6] = j + 0.5, r[7] = k + 0.5;
+          for (int d = 0; d < 15; ++d) r[8 + d] = 1e-3 * gg.normal();
+          for (int d = 0; d < 5; ++d) r[23 + d] = gl.uniform();
+        }
+  }
+  return FOT_OK;
+}

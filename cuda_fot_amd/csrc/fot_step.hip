@@ -69,19 +69,23 @@ template <bool SOA> struct HierGather<HexScatterF, SOA> {
   }
 };
 
-template <bool SOA> struct HierGather<VolnaF, SOA> {
-  // volumes through the block's global ids (mapping 1 is the same map)
+// functors that read their other point data through the global ids of
+// mapping 0 (volna's volumes, mini_aero's cell values: mapping 1 is the same
+// map): the block's cached ids give them back
+template <class F, bool SOA> struct HierGatherGlobalIds {
   FOT_DEV static void run(const StepArgs &a, uint32_t ind, uint32_t n, const uint32_t *cached,
                           uint32_t *loc, double *inc) {
-    uint32_t g[2];
+    uint32_t g[F::MD];
 #pragma unroll
-    for (uint32_t k = 0; k < 2; ++k) {
+    for (uint32_t k = 0; k < F::MD; ++k) {
       loc[k] = a.c2n[0][ind + k * n];
       g[k] = cached[loc[k]];
     }
-    VolnaF::compute<SOA>(a, ind, g, inc);
+    F::template compute<SOA>(a, ind, g, inc);
   }
 };
+template <bool SOA> struct HierGather<VolnaF, SOA> : HierGatherGlobalIds<VolnaF, SOA> {};
+template <bool SOA> struct HierGather<MiniAeroF, SOA> : HierGatherGlobalIds<MiniAeroF, SOA> {};
 
 __device__ uint32_t g_lds_overflow; // set when a block's cache exceeds cache_size
 
@@ -240,5 +244,15 @@ extern "C" fot_status fot_lds_overflow_flag(uint32_t *flag, int clear) {
     e = hipMemcpyToSymbol(HIP_SYMBOL(g_lds_overflow), &zero, sizeof(uint32_t));
   }
   if (e != hipSuccess) return fail(FOT_ERR_HIP, "%s", hipGetErrorString(e));
+  return FOT_OK;
+}
+
+// The mini_aero user function on the host (the code the kernels run): the
+// C++ drop-in's mini_aero::StepSeq calls it per face.
+extern "C" fot_status fot_mini_aero_face_flux(const double *left, const double *right,
+                                              uint32_t cell_stride, const double *face,
+                                              uint32_t face_stride, double *flux) {
+  if (!left || !right || !face || !flux) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
+  fot::ma::face_flux(left, right, cell_stride, face, face_stride, flux);
   return FOT_OK;
 }

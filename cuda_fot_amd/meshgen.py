@@ -70,3 +70,16 @@ def tri_mesh(nx: int, ny: int, seed: int = 0, shuffle: int = 6) -> dict:
          ("edge_cells", "is_boundary", "normals", "fluxes", "bathy", "volume", "state")))
     out.update(num_cells=nc, num_edges=ne, nx=nx, ny=ny)
     return out
+
+
+def hex_faces(n1: int, n2: int, n3: int, seed: int = 0, shuffle: int = 6) -> dict:
+    """mini_aero-style hex mesh: interior faces -> 2 cells, face data (12, F)
+    SOA (centre, normal, tangent, binormal), cell values (C, 28)."""
+    nc = n1 * n2 * n3
+    nf = (n1 - 1) * n2 * n3 + n1 * (n2 - 1) * n3 + n1 * n2 * (n3 - 1)
+    out = {"face_cells": np.empty((nf, 2), np.uint32), "face": np.empty((12, nf)),
+           "cells": np.empty((nc, 28))}
+    call("fot_gen_hex_faces", n1, n2, n3, seed, shuffle, ptr(out["face_cells"]),
+         ptr(out["face"]), ptr(out["cells"]))
+    out.update(num_cells=nc, num_faces=nf)
+    return out

@@ -217,3 +217,15 @@ def gpu_batch(problems, func: capi.Functor, iters: int, use_graph: bool = False)
     stats = b.stats()
     b.close()
     return stats
+
+
+def mini_aero_problem(mesh: dict, block_size: int = DEFAULT_BLOCK_SIZE,
+                      soa: bool = False) -> Problem:
+    """The face loop as mini_aero.cu sets it up: two copies of the
+    face->cell map; point data the cell flux (dim 5, the target, zero) and the
+    28 cell values; face data 12 values."""
+    nc = mesh["num_cells"]
+    flux = np.zeros((5, nc)) if soa else np.zeros((nc, 5))
+    cells = mesh["cells"].T.copy() if soa else mesh["cells"].copy()
+    maps = [mesh["face_cells"], mesh["face_cells"]]
+    return Problem(maps, [nc, nc], [flux, cells], [mesh["face"]], block_size, soa)

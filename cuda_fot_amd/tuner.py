@@ -1,0 +1,252 @@
+"""Autotuner and learned cost model for the loop engine (no reference
+counterpart).
+
+Six candidate strategies per loop (the choices the engine offers):
+  0 atomic       FP64 global atomics, 256 threads
+  1 global       global colouring, 256 threads
+  2 hier256      hierarchical, per-block-colour launches, 256 cells per block
+  3 owner128     owner-compute hierarchical, parts of 128 points
+  4 owner256     owner-compute hierarchical, parts of 256 points
+  5 owner512     owner-compute hierarchical, parts of 512 points
+
+`measure` is the exhaustive search: every candidate planned and timed on the
+GPU (fot_autotune_measure).  `CostModel` is an FP32 MLP 24 -> 64 -> 64 -> 6
+(libfot's MFMA kernels) trained on the autotuner's own timings to predict
+log(ms per step) of every candidate from 24 cheap mesh features
+(fot_loop_features).  `Tuner.learned` takes the predicted-best candidate and
+measures only it; when that measurement exceeds the model's prediction for it
+by more than the tolerance (5 %), or the pick cannot be planned, it falls back
+to the measured search.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import capi
+from .capi import call, ptr
+from .device import DeviceArray, make_desc
+from .mlp import MLP
+
+NAMES = ["atomic", "global", "hier256", "owner128", "owner256", "owner512"]
+NUM_FEATURES = 24
+
+
+def candidate_configs(func: capi.Functor):
+    """The six fot_loop_config candidates for a functor."""
+    res_calc = func.kernel == capi.KERNEL_RES_CALC
+    own = capi.LOOP_OWNER_COMPUTE | (capi.LOOP_TWO_PER_THREAD if res_calc else 0)
+    spec = [(capi.STRATEGY_ATOMIC, 256, 0), (capi.STRATEGY_GLOBAL_COLOUR, 256, 0),
+            (capi.STRATEGY_HIERARCHICAL, 256, capi.LOOP_STAGE_INPUTS if res_calc else 0),
+            (capi.STRATEGY_HIERARCHICAL, 128, own), (capi.STRATEGY_HIERARCHICAL, 256, own),
+            (capi.STRATEGY_HIERARCHICAL, 512, own)]
+    cfgs = (capi.LoopConfig * len(spec))()
+    for i, (s, bs, fl) in enumerate(spec):
+        cfgs[i].strategy, cfgs[i].block_size, cfgs[i].flags = s, bs, fl
+    return cfgs
+
+
+def features(problem, func: capi.Functor) -> np.ndarray:
+    """fot_loop_features of a Problem (host arrays only)."""
+    maps = [capi.u32(m) for m in problem.maps]
+    d = capi.ProblemDesc()
+    d.num_cells = maps[0].shape[0]
+    d.num_mappings = len(maps)
+    for k, m in enumerate(maps):
+        d.cell_to_node[k] = ptr(m)
+        d.mesh_dim[k] = m.shape[1]
+        d.num_points[k] = int(problem.num_points[k])
+        pd = problem.point_data[k]
+        d.point_dim[k] = 1 if pd is None or pd.ndim == 1 else (pd.shape[0] if problem.soa
+                                                               else pd.shape[1])
+    d.num_cell_data = len(problem.cell_data)
+    for k, c in enumerate(problem.cell_data):
+        d.cell_dim[k] = c.shape[0] if c.ndim > 1 else 1
+    d.soa = int(problem.soa)
+    out = np.empty(NUM_FEATURES, np.float32)
+    call("fot_loop_features", C.byref(func), C.byref(d), ptr(out))
+    return out
+
+
+class _Uploaded:
+    """A Problem's data on the device and its fot_problem_desc."""
+
+    def __init__(self, problem):
+        self.dev = [None if x is None else DeviceArray.from_host(x) for x in problem.point_data]
+        self.out = DeviceArray.from_host(problem.point_data[0])
+        self.cdev = [DeviceArray.from_host(c) for c in problem.cell_data]
+        self.desc, self._keep = make_desc(problem.maps, problem.num_points, self.dev, self.out,
+                                          self.cdev, problem.soa)
+
+
+def measure(problem, func: capi.Functor, which=None, warmup: int = 3,
+            steps: int = 10) -> np.ndarray:
+    """ms per step of the candidates `which` (default all); -1 where a
+    candidate cannot be planned, NaN where it was not measured."""
+    up = _Uploaded(problem)
+    all_cfgs = candidate_configs(func)
+    idx = list(range(len(NAMES))) if which is None else list(which)
+    cfgs = (capi.LoopConfig * len(idx))(*[all_cfgs[i] for i in idx])
+    t = np.empty(len(idx), np.float32)
+    call("fot_autotune_measure", C.byref(func), C.byref(up.desc), cfgs, len(idx), warmup,
+         steps, ptr(t))
+    out = np.full(len(NAMES), np.nan, np.float32)
+    out[idx] = t
+    return out
+
+
+class CostModel:
+    """MLP on standardised features -> standardised log(ms) per candidate."""
+
+    def __init__(self, seed: int = 0, hidden: int = 64):
+        self.net = MLP(NUM_FEATURES, hidden, len(NAMES), seed=seed)
+        self.fmu = self.fsd = self.tmu = self.tsd = None
+
+    def fit(self, X, T, epochs: int = 3000, lr: float = 3e-3, weight_decay: float = 1e-4):
+        X = np.asarray(X, np.float64)
+        T = np.asarray(T, np.float64)
+        mask = np.isfinite(T) & (T > 0)
+        logT = np.where(mask, np.log(np.where(mask, T, 1.0)), 0.0)
+        self.fmu, self.fsd = X.mean(0), X.std(0) + 1e-6
+        self.tmu = (logT * mask).sum() / max(mask.sum(), 1)
+        self.tsd = math.sqrt(((logT - self.tmu) ** 2 * mask).sum() / max(mask.sum(), 1)) + 1e-6
+        xs = (X - self.fmu) / self.fsd
+        ts = np.where(mask, (logT - self.tmu) / self.tsd, 0.0)
+        return self.net.train(xs, ts, mask.astype(np.float32), epochs=epochs, lr=lr,
+                              weight_decay=weight_decay)
+
+    def predict(self, X) -> np.ndarray:
+        xs = (np.atleast_2d(np.asarray(X, np.float64)) - self.fmu) / self.fsd
+        return np.exp(self.net.forward(xs).astype(np.float64) * self.tsd + self.tmu)
+
+
+class Tuner:
+    def __init__(self, model: CostModel | None = None, tolerance: float = 0.05):
+        self.model, self.tolerance = model, tolerance
+
+    @staticmethod
+    def exhaustive(problem, func, **kw):
+        t = measure(problem, func, **kw)
+        ok = np.where(t > 0, t, np.inf)
+        return int(np.argmin(ok)), t
+
+    def learned(self, problem, func, **kw) -> dict:
+        pred = self.model.predict(features(problem, func))[0]
+        pick = int(np.argmin(pred))
+        t = measure(problem, func, which=[pick], **kw)
+        res = {"pick": pick, "predicted": pred, "measured_pick": float(t[pick]),
+               "fell_back": False, "measured": 1}
+        if not (t[pick] > 0) or t[pick] > (1.0 + self.tolerance) * pred[pick]:
+            best, times = self.exhaustive(problem, func, **kw)
+            res.update(pick=best, fell_back=True, measured=1 + len(NAMES), times=times)
+        return res
+
+
+def fallback_pick(pred_ms, measured_ms, tolerance: float = 0.05):
+    """The learned choice with the fallback rule, on known measurements:
+    take the predicted-best candidate; if its measured time exceeds the
+    model's prediction for it by more than `tolerance` (or it cannot be
+    planned), take the measured best instead.  Returns (choice, fell_back)."""
+    pick = int(np.argmin(pred_ms))
+    t = measured_ms[pick]
+    ok = np.where(measured_ms > 0, measured_ms, np.inf)
+    if not (t > 0) or t > (1.0 + tolerance) * pred_ms[pick]:
+        return int(np.argmin(ok)), True
+    return pick, False
+
+
+# ---- corpus of autotuner timings ---------------------------------------------
+def corpus_meshes(n: int, seed: int = 0, min_edges: float = 5e4, max_edges: float = 1.2e6):
+    """(family, ordering, size) specs: 2D quad (mine), airfoil quads
+    (res_calc), triangles (volna), 3D hexes (scatter), each in random, RCM
+    and point-partitioned orderings, edges log-uniform in [min, max]."""
+    rng = np.random.default_rng(seed)
+    fams = ["mine", "res_calc", "volna", "hex"]
+    orders = ["random", "rcm", "partitioned"]
+    out = []
+    for i in range(n):
+        e = float(np.exp(rng.uniform(np.log(min_edges), np.log(max_edges))))
+        out.append({"id": i, "family": fams[i % 4], "ordering": orders[(i // 4) % 3],
+                    "edges": int(e), "seed": int(rng.integers(1 << 30))})
+    return out
+
+
+def build_corpus_problem(spec):
+    """The Problem and functor of a corpus spec."""
+    from . import meshgen
+    from .problem import Problem, res_calc_problem, volna_problem
+    fam, e, s = spec["family"], spec["edges"], spec["seed"]
+    rng = np.random.default_rng(s)
+    aspect = float(rng.uniform(0.5, 2.0))
+    if fam == "mine":
+        # ~2 edges per point
+        N = max(8, int(math.sqrt(e / 2 * aspect)))
+        M = max(8, int(e / 2 / N))
+        edges = meshgen.grid_edges(N, M)
+        edges = edges[meshgen.permutation(edges.shape[0], s + 1)]
+        n = N * M
+        pd = meshgen.uniform(n * 4, s + 2).reshape(n, 4)
+        w = meshgen.uniform(edges.shape[0] * 4, s + 3).reshape(4, -1)
+        p, func = Problem([edges], [n], [pd], [w], 256), capi.functor(capi.KERNEL_MINE, 2, 4, 4)
+    elif fam == "res_calc":
+        nx = max(8, int(math.sqrt(e / 2 * aspect)))
+        ny = max(8, int(e / 2 / nx))
+        p, func = res_calc_problem(meshgen.airfoil(nx, ny, seed=s)), \
+            capi.functor(capi.KERNEL_RES_CALC)
+    elif fam == "volna":
+        nx = max(8, int(math.sqrt(e / 3 * aspect)))
+        ny = max(8, int(e / 3 / nx))
+        p, func = volna_problem(meshgen.tri_mesh(nx, ny, seed=s)), \
+            capi.functor(capi.KERNEL_VOLNA)
+    else:
+        m = max(4, int(round(e ** (1 / 3))))
+        cells = meshgen.hex_grid(m + 1, m + 1, m + 1)
+        cells = cells[meshgen.permutation(cells.shape[0], s + 5)]
+        nn = (m + 1) ** 3
+        forces = meshgen.normal(cells.shape[0] * 24, s + 6).reshape(24, -1)
+        p, func = Problem([cells], [nn], [np.zeros((nn, 3))], [forces], 256), \
+            capi.functor(capi.KERNEL_HEX_SCATTER)
+    if spec["ordering"] == "rcm":
+        p.reorder("rcm")
+    elif spec["ordering"] == "partitioned":
+        p.partition_points(256)
+        p.point_partition = None  # the ordering stays; candidates plan their own
+    return p, func
+
+
+def corpus_main(argv=None):
+    """python -m cuda_fot_amd.tuner corpus OUT.jsonl N [SEED] [FIRST]:
+    measures every candidate on corpus meshes FIRST..N-1, appending one JSON
+    line per mesh (features, ms per step) and printing progress."""
+    argv = sys.argv[1:] if argv is None else argv
+    out, n = argv[1], int(argv[2])
+    seed = int(argv[3]) if len(argv) > 3 else 0
+    first = int(argv[4]) if len(argv) > 4 else 0
+    done = set()
+    if os.path.exists(out):
+        with open(out) as f:
+            done = {json.loads(l)["id"] for l in f if l.strip()}
+    for spec in corpus_meshes(n, seed)[first:]:
+        if spec["id"] in done:
+            continue
+        t0 = time.perf_counter()
+        p, func = build_corpus_problem(spec)
+        x = features(p, func)
+        t = measure(p, func)
+        rec = dict(spec, cells=int(p.num_cells), features=[float(v) for v in x],
+                   ms=[float(v) for v in t], seconds=round(time.perf_counter() - t0, 2))
+        with open(out, "a") as f:
+            f.write(json.dumps(rec) + "\n")
+        print(f"mesh {spec['id']} {spec['family']}/{spec['ordering']} cells {p.num_cells}: "
+              + " ".join(f"{v:.3f}" for v in t) + f" ({rec['seconds']} s)", flush=True)
+
+
+if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "corpus":
+        corpus_main()

@@ -175,6 +175,15 @@ fot_status fot_step_gpu_hierarchical(
     uint32_t num_cells, const uint32_t *point_stride, uint32_t cell_stride,
     uint32_t num_blocks, uint32_t block_size, uint32_t cache_size, void *stream);
 
+/* mini_aero's user function (kernels/mini_aero_func.hpp:11-85) on the host,
+ * the same code the kernels run: flux[5] of a face from the left / right
+ * cells' 28 values (5 conservatives, 3 coordinates, 15 gradients,
+ * 5 limiters; element stride cell_stride) and the face's 12 values
+ * (coordinates, normal, tangent, binormal; stride face_stride). */
+fot_status fot_mini_aero_face_flux(const double *left, const double *right,
+                                   uint32_t cell_stride, const double *face,
+                                   uint32_t face_stride, double *flux);
+
 /* Reads (and with clear != 0 resets) the flag a hierarchical launch sets
  * when a block's points do not fit cache_size; such blocks are skipped.  No
  * reference counterpart (the reference writes past its shared memory). */
@@ -381,6 +390,17 @@ fot_status fot_gen_normal(uint64_t n, uint64_t seed, double mean, double sd,
  * hu, hv N(0, 0.1), zb U(-1,0].  shuffle: bit 1 cells, bit 2 edges.  Any
  * output may be NULL.  No reference counterpart (the reference reads Volna
  * meshes from files, volna.cu:10-36). */
+/* mini_aero-style 3D hex mesh of n1 x n2 x n3 unit cells: interior faces
+ * only, F = (n1-1)n2n3 + n1(n2-1)n3 + n1n2(n3-1), each mapping to its two
+ * cells (lower index side first).  face_cells[2F]; face SOA [12][F]:
+ * centre, normal (unit, lower -> upper), tangent, binormal; cells AOS
+ * [C][28]: conservatives freestream (rho 1, u 50, v 10, w 5, T 300) times
+ * (1 + 1e-2 N(0,1)) per component, centre, gradients N(0, 1e-3),
+ * limiters U[0,1).  shuffle: bit 1 cells, bit 2 faces.  Any output may be
+ * NULL.  No reference counterpart (mini_aero.cu reads its mesh). */
+fot_status fot_gen_hex_faces(uint32_t n1, uint32_t n2, uint32_t n3, uint64_t seed,
+                             uint32_t shuffle, uint32_t *face_cells, double *face,
+                             double *cells);
 fot_status fot_gen_tri_mesh(uint32_t nx, uint32_t ny, uint64_t seed, uint32_t shuffle,
                             uint32_t *edge_cells, double *is_boundary, double *normals,
                             double *fluxes, double *bathy, double *volume, double *state);

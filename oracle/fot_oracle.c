@@ -117,6 +117,209 @@ void oracle_res_calc_func(const double *x1, const double *x2, const double *q1,
   res2[3] = -f;
 }
 
+/* ---- mini_aero (kernels/mini_aero_func.hpp:11-85 and
+ * kernels/mini_aero_utils.hpp:1-375), restated step by step ---------------- */
+static double ma_pressure(const double *V) { /* mini_aero_utils.hpp:6-12 */
+  const double Rgas = 287.05;
+  return V[0] * Rgas * V[4];
+}
+
+static double ma_enthalpy(const double *V) { /* :14-18 */
+  const double Cp = 1004.0;
+  return Cp * V[4];
+}
+
+static void ma_matvec5(double alpha, const double *A, const double *x, double beta,
+                       double *y) { /* :20-29 */
+  for (int i = 0; i < 5; ++i) {
+    y[i] *= beta;
+    for (int j = 0; j < 5; ++j) y[i] += alpha * A[5 * i + j] * x[j];
+  }
+}
+
+static void ma_inviscid(const double *Pl, const double *Pr, double *flux, const double *n,
+                        const double *t, const double *b) { /* :31-242, unit strides */
+  const double efix_u = 0.1, efix_c = 0.1, gm1 = 0.4;
+  const double rho_l = Pl[0], u_l = Pl[1], v_l = Pl[2], w_l = Pl[3];
+  const double p_l = ma_pressure(Pl), h_l = ma_enthalpy(Pl);
+  const double ht_l = h_l + 0.5 * (u_l * u_l + v_l * v_l + w_l * w_l);
+  const double mf_l = rho_l * (n[0] * u_l + n[1] * v_l + n[2] * w_l);
+  const double rho_r = Pr[0], u_r = Pr[1], v_r = Pr[2], w_r = Pr[3];
+  const double p_r = ma_pressure(Pr), h_r = ma_enthalpy(Pr);
+  const double ht_r = h_r + 0.5 * (u_r * u_r + v_r * v_r + w_r * w_r);
+  const double mf_r = rho_r * (n[0] * u_r + n[1] * v_r + n[2] * w_r);
+  const double psum = p_l + p_r;
+  flux[0] = 0.5 * (mf_l + mf_r);
+  flux[1] = 0.5 * (mf_l * u_l + mf_r * u_r + n[0] * psum);
+  flux[2] = 0.5 * (mf_l * v_l + mf_r * v_r + n[1] * psum);
+  flux[3] = 0.5 * (mf_l * w_l + mf_r * w_r + n[2] * psum);
+  flux[4] = 0.5 * (mf_l * ht_l + mf_r * ht_r);
+  const double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  const double tn = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+  const double bn = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  const double nu[3] = {n[0] / nn, n[1] / nn, n[2] / nn};
+  const double tu[3] = {t[0] / tn, t[1] / tn, t[2] / tn};
+  const double bu[3] = {b[0] / bn, b[1] / bn, b[2] / bn};
+  const double denom = 1.0 / (sqrt(rho_l) + sqrt(rho_r));
+  const double alpha = sqrt(rho_l) * denom;
+  const double beta = 1.0 - alpha;
+  const double ur = alpha * u_l + beta * u_r;
+  const double vr = alpha * v_l + beta * v_r;
+  const double wr = alpha * w_l + beta * w_r;
+  const double hr = alpha * h_l + beta * h_r +
+                    0.5 * alpha * beta *
+                        ((u_r - u_l) * (u_r - u_l) + (v_r - v_l) * (v_r - v_l) +
+                         (w_r - w_l) * (w_r - w_l));
+  const double cr = sqrt(gm1 * hr);
+  double M[25];
+  const double vn = ur * nu[0] + vr * nu[1] + wr * nu[2];
+  const double vt = ur * tu[0] + vr * tu[1] + wr * tu[2];
+  const double vb = ur * bu[0] + vr * bu[1] + wr * bu[2];
+  const double ke = 0.5 * (ur * ur + vr * vr + wr * wr);
+  const double c2i = 1.0 / (cr * cr);
+  const double hc2i = 0.5 * c2i;
+  double dq[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  dq[0] = rho_r - rho_l;
+  dq[1] = rho_r * u_r - rho_l * u_l;
+  dq[2] = rho_r * v_r - rho_l * v_l;
+  dq[3] = rho_r * w_r - rho_l * w_l;
+  dq[4] = (rho_r * ht_r - p_r) - (rho_l * ht_l - p_l);
+  const double cbar = cr * nn;
+  const double ubar = ur * n[0] + vr * n[1] + wr * n[2];
+  const double cfl = fabs(ubar) + cbar;
+  const double e1 = ubar + cbar, e2 = ubar - cbar, e3 = ubar;
+  double a1 = fabs(e1), a2 = fabs(e2), a3 = fabs(e3);
+  const double epuc = efix_u * cfl, epcc = efix_c * cfl;
+  if (a1 < epcc) a1 = 0.5 * (e1 * e1 + epcc * epcc) / epcc;
+  if (a2 < epcc) a2 = 0.5 * (e2 * e2 + epcc * epcc) / epcc;
+  if (a3 < epuc) a3 = 0.5 * (e3 * e3 + epuc * epuc) / epuc;
+  double eigp[5] = {0.5 * (e1 + a1), 0.5 * (e2 + a2), 0.5 * (e3 + a3), 0.0, 0.0};
+  eigp[3] = eigp[4] = eigp[2];
+  double eigm[5] = {0.5 * (e1 - a1), 0.5 * (e2 - a2), 0.5 * (e3 - a3), 0.0, 0.0};
+  eigm[3] = eigm[4] = eigm[2];
+  double ldq[5] = {0, 0, 0, 0, 0}, rlldq[5] = {0, 0, 0, 0, 0};
+  M[0] = gm1 * (ke - hr) + cr * (cr - vn);
+  M[1] = cr * nu[0] - gm1 * ur;
+  M[2] = cr * nu[1] - gm1 * vr;
+  M[3] = cr * nu[2] - gm1 * wr;
+  M[4] = gm1;
+  M[5] = gm1 * (ke - hr) + cr * (cr + vn);
+  M[6] = -cr * nu[0] - gm1 * ur;
+  M[7] = -cr * nu[1] - gm1 * vr;
+  M[8] = -cr * nu[2] - gm1 * wr;
+  M[9] = gm1;
+  M[10] = ke - hr;
+  M[11] = -ur;
+  M[12] = -vr;
+  M[13] = -wr;
+  M[14] = 1.0;
+  M[15] = -vt;
+  M[16] = tu[0];
+  M[17] = tu[1];
+  M[18] = tu[2];
+  M[19] = 0.0;
+  M[20] = -vb;
+  M[21] = bu[0];
+  M[22] = bu[1];
+  M[23] = bu[2];
+  M[24] = 0.0;
+  ma_matvec5(1.0, M, dq, 0.0, ldq);
+  for (int j = 0; j < 5; ++j) ldq[j] = (eigp[j] - eigm[j]) * ldq[j];
+  M[0] = hc2i;
+  M[1] = hc2i;
+  M[2] = -gm1 * c2i;
+  M[3] = 0.0;
+  M[4] = 0.0;
+  M[5] = (ur + nu[0] * cr) * hc2i;
+  M[6] = (ur - nu[0] * cr) * hc2i;
+  M[7] = -gm1 * ur * c2i;
+  M[8] = tu[0];
+  M[9] = bu[0];
+  M[10] = (vr + nu[1] * cr) * hc2i;
+  M[11] = (vr - nu[1] * cr) * hc2i;
+  M[12] = -gm1 * vr * c2i;
+  M[13] = tu[1];
+  M[14] = bu[1];
+  M[15] = (wr + nu[2] * cr) * hc2i;
+  M[16] = (wr - nu[2] * cr) * hc2i;
+  M[17] = -gm1 * wr * c2i;
+  M[18] = tu[2];
+  M[19] = bu[2];
+  M[20] = (hr + ke + cr * vn) * hc2i;
+  M[21] = (hr + ke - cr * vn) * hc2i;
+  M[22] = (cr * cr - gm1 * (hr + ke)) * c2i;
+  M[23] = vt;
+  M[24] = vb;
+  ma_matvec5(1.0, M, ldq, 0.0, rlldq);
+  for (int i = 0; i < 5; ++i) flux[i] -= 0.5 * rlldq[i];
+}
+
+static void ma_viscous(double g[5][3], const double *P, const double *a,
+                       double *vf) { /* :244-300 */
+  const double s0 = 1.458e-6, s1 = 110.4;
+  const double T = P[4];
+  const double mu = s0 * T * sqrt(T) / (T + s1);
+  const double Pr = 0.71, Cp = 1006.0;
+  const double kc = mu * Cp / Pr;
+  double div = 0;
+  for (int c = 0; c < 5; ++c) vf[c] = 0.0;
+  for (int i = 0; i < 3; i++) div += g[i + 1][i];
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) {
+      const double dij = (i == j) ? 1 : 0;
+      const double Sij = 0.5 * (g[i + 1][j] + g[j + 1][i]);
+      const double tij = Sij - div * dij / 3.;
+      vf[1 + i] += (2 * mu * tij) * a[j];
+      vf[4] += (2 * mu * tij) * P[i + 1] * a[j];
+    }
+    vf[4] += kc * g[4][i] * a[i];
+  }
+}
+
+static void ma_primitives(const double *U, double *V) { /* :302-322 */
+  double gamma = 1.4, Rgas = 287.05;
+  double r = U[0], ri = 1.0 / r;
+  double u = U[1] * ri, v = U[2] * ri, w = U[3] * ri;
+  double k = 0.5 * (u * u + v * v + w * w);
+  double e = U[4] * ri - k;
+  double T = e * (gamma - 1.0) / Rgas;
+  V[0] = r, V[1] = u, V[2] = v, V[3] = w, V[4] = T;
+}
+
+/* mini_aero_func.hpp:11-85: L, R cell rows of 28 (stride s), face row of 12
+ * (stride fs) */
+static void ma_face(const double *L, const double *R, uint32_t s, const double *F, uint32_t fs,
+                    double *flux) {
+  double cl[5], cr[5], pl[5], pr[5];
+  for (int c = 0; c < 5; ++c) cl[c] = L[c * s], cr[c] = R[c * s];
+  ma_primitives(cl, pl);
+  ma_primitives(cr, pr);
+  for (int c = 0; c < 5; ++c) {
+    double gl = 0, gr = 0;
+    for (int d = 0; d < 3; ++d) {
+      gl += (F[d * fs] - L[(5 + d) * s]) * L[(8 + c * 3 + d) * s];
+      gr += (F[d * fs] - R[(5 + d) * s]) * R[(8 + c * 3 + d) * s];
+    }
+    pl[c] += gl * L[(23 + c) * s];
+    pr[c] += gr * R[(23 + c) * s];
+  }
+  double n[3], t[3], b[3];
+  for (int d = 0; d < 3; ++d) {
+    n[d] = F[(3 + d) * fs];
+    t[d] = F[(6 + d) * fs];
+    b[d] = F[(9 + d) * fs];
+  }
+  ma_inviscid(pl, pr, flux, n, t, b);
+  double pf[5], gf[5][3];
+  for (int c = 0; c < 5; ++c) {
+    pf[c] = 0.5 * (pl[c] + pr[c]);
+    for (int d = 0; d < 3; ++d) gf[c][d] = 0.5 * (L[(8 + c * 3 + d) * s] + R[(8 + c * 3 + d) * s]);
+  }
+  double vf[5];
+  ma_viscous(gf, pf, n, vf);
+  for (int c = 0; c < 5; ++c) flux[c] -= vf[c];
+}
+
 /* Index of element (point i, component d) of a point set (data_t.hpp:104-115). */
 static inline size_t pidx(int soa, uint32_t n, uint32_t i, uint32_t dim,
                           uint32_t d) {
@@ -184,6 +387,19 @@ static void compute_inc(const oracle_problem *p, uint32_t ind, double *inc) {
     for (uint32_t j = 0; j < 8; ++j)
       for (uint32_t i = 0; i < 3; ++i)
         inc[j * 3 + i] = F[(size_t)(i * 8 + j) * cs];
+    break;
+  }
+  case ORACLE_MINI_AERO: {
+    /* mini_aero::StepSeq (kernels/mini_aero.hpp:43-117): cell values (28)
+     * through mapping-0 ids, the face's 12 values as SOA cell data; left
+     * -= flux, right += flux, i.e. increments -flux and +flux. */
+    const uint32_t *m0 = p->cell_to_node[0];
+    const double *cv = p->point_data[1];
+    const uint32_t s = soa ? p->num_points[1] : 1, u = soa ? 1 : 28;
+    double flux[5];
+    ma_face(cv + (size_t)u * m0[2 * ind], cv + (size_t)u * m0[2 * ind + 1], s,
+            p->cell_data[0] + ind, p->num_cells, flux);
+    for (int c = 0; c < 5; ++c) inc[c] = -flux[c], inc[5 + c] = flux[c];
     break;
   }
   case ORACLE_VOLNA: {

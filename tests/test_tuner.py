@@ -1,0 +1,85 @@
+"""The autotuner: loop features (known answers on a tiny mesh), the
+fallback rule of the learned choice (closed form), and on the GPU the
+measured search and the learned tuner end to end."""
+import numpy as np
+import pytest
+
+from cuda_fot_amd import Problem, capi, tuner
+
+MINE = capi.functor(capi.KERNEL_MINE, 2, 1, 1)
+
+
+def _path_problem():
+    # path 0-1-2-3: degrees 1,2,2,1; two colours; spans 1; jumps 1
+    e = np.array([[0, 1], [1, 2], [2, 3]], np.uint32)
+    return Problem([e], [4], [np.zeros((4, 1))], [np.ones((1, 3))], 128)
+
+
+def test_features_known_answer():
+    x = tuner.features(_path_problem(), MINE)
+    assert x.shape == (24,)
+    assert x[0] == pytest.approx(np.log2(3)) and x[1] == pytest.approx(2.0)
+    assert x[2] == pytest.approx(1.5)                        # mean degree
+    assert x[3] == 2 and x[4] == 1 and x[5] == 1 and x[6] == 0
+    assert x[7] == pytest.approx(np.sqrt(0.25) / 1.5)        # std / mean
+    assert x[8] == pytest.approx(2 / 1.5)                    # max / mean
+    assert list(x[9:13]) == [1.0, 0.0, 0.0, 0.0]             # all degrees <= 2
+    assert x[13] == 2                                        # colours
+    assert x[14] == pytest.approx(0.25) and x[15] == 1 and x[16] == 1
+    assert x[17] == pytest.approx(0.25) and x[18] == 1 and x[19] == 1
+    # bytes: target 16*4*1 + cell data 8*3 + map 4*3*2 = 64 + 24 + 24
+    assert x[20] == pytest.approx(np.log2(112))
+    assert x[21] == 0 and x[22] == 1 and x[23] == pytest.approx(np.log2(32))
+
+
+def test_features_see_ordering():
+    from cuda_fot_amd import meshgen, res_calc_problem
+    shuffled = res_calc_problem(meshgen.airfoil(40, 30, seed=1))
+    ordered = res_calc_problem(meshgen.airfoil(40, 30, seed=1))
+    ordered.reorder("rcm")
+    f = capi.functor(capi.KERNEL_RES_CALC)
+    a, b = tuner.features(shuffled, f), tuner.features(ordered, f)
+    assert b[17] < 0.2 * a[17] and b[18] > a[18]             # jumps shrink
+    assert a[:4].tolist() == b[:4].tolist()                  # sizes do not change
+
+
+def test_fallback_rule_closed_form():
+    meas = np.array([5.0, 4.0, 3.0, 2.0, 2.1, 2.5])
+    # pick 3 predicted 2.0, measured 2.0 <= 1.05 * 2.0: kept
+    assert tuner.fallback_pick(np.array([9, 9, 9, 2.0, 3, 3]), meas) == (3, False)
+    # pick 4 predicted 1.9, measured 2.1 > 1.05 * 1.9: measured search -> 3
+    assert tuner.fallback_pick(np.array([9, 9, 9, 3, 1.9, 3]), meas) == (3, True)
+    # exactly at the tolerance stays
+    assert tuner.fallback_pick(np.array([9, 9, 9, 9, 2.0, 9]), meas) == (4, False)
+    # an unplannable pick (-1) always falls back
+    m2 = meas.copy()
+    m2[0] = -1
+    assert tuner.fallback_pick(np.array([1.0, 9, 9, 9, 9, 9]), m2) == (3, True)
+
+
+def test_corpus_specs_cover_families_and_orderings():
+    specs = tuner.corpus_meshes(24, seed=0)
+    fams = {(s["family"], s["ordering"]) for s in specs}
+    assert len(fams) == 12
+    assert all(5e4 <= s["edges"] <= 1.2e6 for s in specs)
+
+
+@pytest.mark.gpu
+def test_measure_and_learned_tuner(gpu):
+    from cuda_fot_amd import meshgen, res_calc_problem
+    f = capi.functor(capi.KERNEL_RES_CALC)
+    p = res_calc_problem(meshgen.airfoil(120, 80, seed=2))
+    best, t = tuner.Tuner.exhaustive(p, f, warmup=2, steps=5)
+    assert np.all(t > 0), t
+    assert t[best] == t.min()
+    # a model that predicts the measured times exactly keeps its pick
+    model = tuner.CostModel(seed=0)
+    x = tuner.features(p, f)[None, :]
+    X = np.repeat(x, 8, 0) + np.random.default_rng(0).normal(0, 1e-3, (8, 24))
+    losses = model.fit(X, np.repeat(t[None, :], 8, 0), epochs=1500, lr=3e-3)
+    assert losses[-1] < 0.05 * losses[0]
+    pred = model.predict(x)[0]
+    np.testing.assert_allclose(pred, t, rtol=0.1)
+    res = tuner.Tuner(model, tolerance=0.5).learned(p, f, warmup=2, steps=5)
+    assert res["pick"] == int(np.argmin(pred))
+    assert res["measured"] in (1, 7)
