@@ -39,6 +39,11 @@ def unique_bytes(num_nodes: int, num_cells: int, num_edges: int) -> int:
 def parse():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="res_calc",
+                    choices=["res_calc", "volna-batch", "mini_aero", "hex"],
+                    help="res_calc: the airfoil edge loop (the headline line); the others "
+                         "print their own line: 64 batched Volna meshes, mini_aero 96^3, "
+                         "hex scatter 128^3")
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nx", type=int, default=1200)
@@ -155,6 +160,144 @@ def cpu_baseline(mesh, seconds):
             "serial_edges_per_s": mesh["num_edges"] / serial_per}
 
 
+def _line(metric, value, unit, steps, warmup, ms, workload, **extra):
+    d = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
+         "warmup": warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+         "config": {"workload": workload}}
+    d.update(extra)
+    return d
+
+
+def bench_volna_batch(args):
+    """64 shallow-water triangle meshes, cells log-uniform in [2e4, 4e5], one
+    persistent owner-compute launch per step; timed eagerly and as hipGraph
+    replays (increments + copy back), beside one owner loop per mesh."""
+    from cuda_fot_amd import Batch, DeviceArray, Timer, capi, meshgen, synchronize, volna_problem
+    from cuda_fot_amd.device import Loop
+    rng = np.random.default_rng(0)
+    probs = []
+    for i in range(64):
+        cells = float(np.exp(rng.uniform(np.log(2e4), np.log(4e5))))
+        aspect = float(rng.uniform(0.5, 2.0))
+        nx = max(4, int(round(np.sqrt(cells / 2 * aspect))))
+        ny = max(4, int(round(cells / 2 / nx)))
+        probs.append(volna_problem(meshgen.tri_mesh(nx, ny, seed=100 + i),
+                                   block_size=args.points_per_block))
+    f = capi.functor(capi.KERNEL_VOLNA)
+    specs, keep = [], []
+    for p in probs:
+        dev = [DeviceArray.from_host(d) for d in p.point_data]
+        out = DeviceArray(p.point_data[0].shape, np.float64)
+        cdev = [DeviceArray.from_host(c) for c in p.cell_data]
+        keep.append((dev, out, cdev))
+        specs.append(dict(maps=p.maps, num_points=p.num_points, point_data=dev, out=out,
+                          cell_data=cdev, block_size=p.block_size))
+    edges = sum(p.num_cells for p in probs)
+    cells = sum(p.num_points[0] for p in probs)
+    ubytes = 72 * cells + 72 * edges  # state r+w 64 + volume 8 / cell; 64 data + 8 map / edge
+    res = {}
+    for graph in (False, True):
+        b = Batch(f, specs, use_graph=graph)
+        b.run(args.warmup)
+        synchronize()
+        t = Timer()
+        t0 = time.perf_counter()
+        t.start()
+        if graph:
+            b.run(args.steps)          # steps graph replays (increments + copy back)
+        else:
+            for _ in range(args.steps):
+                b.step()
+        t.stop()
+        synchronize()
+        res[graph] = (time.perf_counter() - t0, t.elapsed_ms() / 1e3, b.stats())
+        b.close()
+    # the same meshes one owner loop (one launch) each
+    loops = [Loop(f, sp["maps"], p.num_points, sp["point_data"], sp["out"], sp["cell_data"],
+                  strategy=capi.STRATEGY_HIERARCHICAL, block_size=p.block_size,
+                  flags=capi.LOOP_OWNER_COMPUTE) for sp, p in zip(specs, probs)]
+    for _ in range(args.warmup):
+        for l in loops:
+            l.step()
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for l in loops:
+            l.step()
+    synchronize()
+    seq = time.perf_counter() - t0
+    for l in loops:
+        l.close()
+    wall, ev, st = res[False]
+    gwall, gev, gst = res[True]
+    return _line("volna edges/s (64 batched meshes, one persistent launch per step)",
+                 edges * args.steps / wall, "edges/s", args.steps, args.warmup,
+                 wall / args.steps * 1e3,
+                 f"volna batch: 64 triangle meshes, {cells} cells, {edges} edges, seed 0",
+                 roofline={"bound": "hbm", "achieved": ubytes * args.steps / ev / 1e9,
+                           "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                           "frac": ubytes * args.steps / ev / PEAK_HBM, "traffic": None,
+                           "algorithmic_bytes_per_step": ubytes},
+                 graph_edges_per_s=edges * args.steps / gwall,
+                 graph_ms_per_step=gwall / args.steps * 1e3,
+                 per_mesh_launch_edges_per_s=edges * args.steps / seq,
+                 batch={"grid": st["grid"], "work_items": st["num_work_items"],
+                        "threads": st["threads"], "lds_bytes": st["lds_bytes"]})
+
+
+def bench_single(args, workload):
+    """mini_aero 96^3 faces or hex scatter 128^3 elements: owner compute on a
+    point-partitioned mesh beside plain atomics."""
+    from cuda_fot_amd import Problem, capi, meshgen, mini_aero_problem
+    if workload == "mini_aero":
+        n = 96
+        m = meshgen.hex_faces(n, n, n, seed=0)
+        p = mini_aero_problem(m, block_size=512)
+        f = capi.functor(capi.KERNEL_MINI_AERO)
+        units, ubytes = m["num_faces"], 304 * m["num_cells"] + 104 * m["num_faces"]
+        name = f"mini_aero {n}^3 hex cells, {m['num_faces']} interior faces, shuffled, seed 0"
+    else:
+        n = 128
+        cells = meshgen.hex_grid(n + 1, n + 1, n + 1)
+        cells = cells[meshgen.permutation(cells.shape[0], 5)]
+        nn = (n + 1) ** 3
+        forces = meshgen.normal(cells.shape[0] * 24, 6).reshape(24, -1)
+        p = Problem([cells], [nn], [np.zeros((nn, 3))], [forces], 512)
+        f = capi.functor(capi.KERNEL_HEX_SCATTER)
+        units, ubytes = cells.shape[0], 48 * nn + 224 * cells.shape[0]
+        name = f"hex scatter {n}^3 elements, 8 corners x 3 forces, shuffled, seed 0"
+    from cuda_fot_amd import DeviceArray
+    from cuda_fot_amd.device import Loop
+
+    def run(prob, strategy, flags, bs):
+        dev = [DeviceArray.from_host(d) for d in prob.point_data]
+        out = DeviceArray.from_host(prob.point_data[0])
+        cdev = [DeviceArray.from_host(c) for c in prob.cell_data]
+        part = prob.point_partition if flags & capi.LOOP_OWNER_COMPUTE else None
+        if part is not None:
+            flags |= capi.LOOP_POINT_PARTITION
+        l = Loop(f, prob.maps, prob.num_points, dev, out, cdev, soa=prob.soa, strategy=strategy,
+                 block_size=bs, partition=part, flags=flags)
+        wall, ev = time_loop(l, args.steps, args.warmup)
+        l.close()
+        return wall, ev
+
+    wa, _ = run(p, capi.STRATEGY_ATOMIC, 0, 256)
+    p.partition_points(args.points_per_block)
+    wall, ev = run(p, capi.STRATEGY_HIERARCHICAL, capi.LOOP_OWNER_COMPUTE,
+                   args.points_per_block)
+    unit = "faces/s" if workload == "mini_aero" else "elements/s"
+    return _line(f"{workload} {unit}", units * args.steps / wall, unit, args.steps, args.warmup,
+                 wall / args.steps * 1e3, name,
+                 roofline={"bound": "hbm", "achieved": ubytes * args.steps / ev / 1e9,
+                           "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                           "frac": ubytes * args.steps / ev / PEAK_HBM, "traffic": None,
+                           "algorithmic_bytes_per_step": ubytes},
+                 atomic_per_s=units * args.steps / wa,
+                 speedup_vs_atomic=wa / wall)
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -165,6 +308,12 @@ def main():
     if capi.device_count() < 1:
         print("no HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "volna-batch":
+        print(json.dumps(bench_volna_batch(args)), flush=True)
+        return
+    if args.workload in ("mini_aero", "hex"):
+        print(json.dumps(bench_single(args, args.workload)), flush=True)
+        return
     t_plan = time.perf_counter()
     mesh, p = build_problem(args, args.partition)
     loop, dev, out = make_loop(p, args.strategy, args.block_size, not args.no_stage,

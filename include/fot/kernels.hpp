@@ -223,4 +223,86 @@ using StepGPUAtomic = fot::GPUAtomic<Desc>;
 using StepGPUHierarchical = fot::GPUHierarchical<Desc>;
 }  // namespace hex_scatter
 
+// ---- volna (kernels/volna.hpp, volna_func.hpp) in FP64 ---------------------
+// point data 0 the state (dim 4, target), point data 1 the cell volume;
+// cell data fluxes (3), bathymetry source (2), normals (2) and the
+// right-boundary flag stored as 0.0 / 1.0.
+namespace volna {
+constexpr unsigned MESH_DIM = 2, INC_DIM = 4, FLUX_DIM = 3, CELL_DIM = 2;
+struct Desc {
+  static fot_functor functor() { return fot::make_functor(FOT_KERNEL_VOLNA, 2, 4, 3); }
+};
+struct StepSeq : Desc {
+  template <bool SOA>
+  static void call(const void **point_data, void *point_data_out, void **cell_data,
+                   const MY_SIZE **cell_to_node, MY_SIZE ind, const unsigned *point_stride,
+                   unsigned cell_stride) {
+    const double *vol = static_cast<const double *>(point_data[1]);
+    double *out = static_cast<double *>(point_data_out);
+    const double *fl = static_cast<const double *>(cell_data[0]) + ind;
+    const double *bs = static_cast<const double *>(cell_data[1]) + ind;
+    const double *nm = static_cast<const double *>(cell_data[2]) + ind;
+    const double boundary = static_cast<const double *>(cell_data[3])[ind];
+    const MY_SIZE l = cell_to_node[0][2 * ind], r = cell_to_node[0][2 * ind + 1];
+    const unsigned cs = cell_stride, s = SOA ? point_stride[0] : 1;
+    double inc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    inc[0] -= fl[0] / vol[l];
+    inc[1] -= (fl[cs] + bs[0] * nm[0]) / vol[l];
+    inc[2] -= (fl[2 * cs] + bs[0] * nm[cs]) / vol[l];
+    if (boundary == 0.0) {
+      inc[4] += fl[0] / vol[r];
+      inc[5] += (fl[cs] + bs[cs] * nm[0]) / vol[r];
+      inc[6] += (fl[2 * cs] + bs[cs] * nm[cs]) / vol[r];
+    }
+    double *ol = out + (SOA ? l : INC_DIM * l), *orr = out + (SOA ? r : INC_DIM * r);
+    for (unsigned i = 0; i < INC_DIM; ++i) {
+      ol[i * s] += inc[i];
+      orr[i * s] += inc[INC_DIM + i];
+    }
+  }
+};
+using StepOMP = StepSeq;
+using StepGPUGlobal = fot::GPUGlobal<Desc>;
+using StepGPUAtomic = fot::GPUAtomic<Desc>;
+using StepGPUHierarchical = fot::GPUHierarchical<Desc>;
+}  // namespace volna
+
+// ---- mini_aero (kernels/mini_aero.hpp) -----------------------------------------
+// point data 0 the cell flux (dim 5, target), point data 1 the 28 cell values
+// (5 conservatives, 3 coordinates, 15 gradients, 5 limiters); cell data 0 the
+// 12 face values.  The CPU struct runs libfot's host copy of the user
+// function, the same code as the kernels.
+namespace mini_aero {
+constexpr unsigned MESH_DIM = 2, POINT_DIM0 = 5, POINT_DIM1 = 28, CELL_DIM = 12;
+struct Desc {
+  static fot_functor functor() { return fot::make_functor(FOT_KERNEL_MINI_AERO, 2, 5, 12); }
+};
+struct StepSeq : Desc {
+  template <bool SOA>
+  static void call(const void **point_data, void *point_data_out, void **cell_data,
+                   const MY_SIZE **cell_to_node, MY_SIZE ind, const unsigned *point_stride,
+                   unsigned cell_stride) {
+    const double *cv = static_cast<const double *>(point_data[1]);
+    double *out = static_cast<double *>(point_data_out);
+    const double *face = static_cast<const double *>(cell_data[0]) + ind;
+    const MY_SIZE l = cell_to_node[0][2 * ind], r = cell_to_node[0][2 * ind + 1];
+    const unsigned s1 = SOA ? point_stride[1] : 1, s0 = SOA ? point_stride[0] : 1;
+    double flux[5];
+    fot::check(fot_mini_aero_face_flux(cv + (SOA ? l : POINT_DIM1 * l),
+                                       cv + (SOA ? r : POINT_DIM1 * r), s1, face, cell_stride,
+                                       flux),
+               "fot_mini_aero_face_flux");
+    double *ol = out + (SOA ? l : POINT_DIM0 * l), *orr = out + (SOA ? r : POINT_DIM0 * r);
+    for (unsigned i = 0; i < POINT_DIM0; ++i) {
+      ol[i * s0] -= flux[i];
+      orr[i * s0] += flux[i];
+    }
+  }
+};
+using StepOMP = StepSeq;
+using StepGPUGlobal = fot::GPUGlobal<Desc>;
+using StepGPUAtomic = fot::GPUAtomic<Desc>;
+using StepGPUHierarchical = fot::GPUHierarchical<Desc>;
+}  // namespace mini_aero
+
 #endif  // FOT_KERNELS_HPP

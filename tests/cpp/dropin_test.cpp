@@ -105,6 +105,49 @@ template <bool SOA> Problem<SOA> hex_grid() {
   return p;
 }
 
+// volna on a 14 x 9 triangle mesh (seed 4), mini_aero on 5 x 4 x 3 hexes (seed 2)
+template <bool SOA> Problem<SOA> volna_mesh() {
+  const std::uint32_t nx = 14, ny = 9, nc = 2 * nx * ny;
+  const std::uint32_t ne = nx * (ny + 1) + (nx + 1) * ny + nx * ny;
+  std::vector<std::uint32_t> ec(2 * ne);
+  std::vector<double> bnd(ne), nrm(2 * ne), fl(3 * ne), ba(2 * ne), vol(nc), st(4 * nc);
+  fot::check(fot_gen_tri_mesh(nx, ny, 4, 6, ec.data(), bnd.data(), nrm.data(), fl.data(),
+                              ba.data(), vol.data(), st.data()),
+             "fot_gen_tri_mesh");
+  Problem<SOA> p(Mesh({nc, nc}, ne, {ec.data(), ec.data()}, {2, 2}),
+                 {{4, sizeof(double)}, {1, sizeof(double)}},
+                 {{3, sizeof(double)}, {2, sizeof(double)}, {2, sizeof(double)},
+                  {1, sizeof(double)}},
+                 128);
+  for (MY_SIZE i = 0; i < nc; ++i) {
+    for (MY_SIZE j = 0; j < 4; ++j)
+      p.point_weights[0].template operator[]<double>(index<SOA>(nc, i, 4, j)) = st[4 * i + j];
+    p.point_weights[1].template operator[]<double>(i) = vol[i];
+  }
+  std::copy(fl.begin(), fl.end(), p.cell_weights[0].template begin<double>());
+  std::copy(ba.begin(), ba.end(), p.cell_weights[1].template begin<double>());
+  std::copy(nrm.begin(), nrm.end(), p.cell_weights[2].template begin<double>());
+  std::copy(bnd.begin(), bnd.end(), p.cell_weights[3].template begin<double>());
+  return p;
+}
+
+template <bool SOA> Problem<SOA> mini_aero_mesh() {
+  const std::uint32_t n1 = 5, n2 = 4, n3 = 3, nc = n1 * n2 * n3;
+  const std::uint32_t nf = (n1 - 1) * n2 * n3 + n1 * (n2 - 1) * n3 + n1 * n2 * (n3 - 1);
+  std::vector<std::uint32_t> fc(2 * nf);
+  std::vector<double> face(12 * nf), cells(28 * nc);
+  fot::check(fot_gen_hex_faces(n1, n2, n3, 2, 6, fc.data(), face.data(), cells.data()),
+             "fot_gen_hex_faces");
+  Problem<SOA> p(Mesh({nc, nc}, nf, {fc.data(), fc.data()}, {2, 2}),
+                 {{5, sizeof(double)}, {28, sizeof(double)}}, {{12, sizeof(double)}}, 128);
+  for (MY_SIZE i = 0; i < nc; ++i)
+    for (MY_SIZE j = 0; j < 28; ++j)
+      p.point_weights[1].template operator[]<double>(index<SOA>(nc, i, 28, j)) =
+          cells[28 * i + j];
+  std::copy(face.begin(), face.end(), p.cell_weights[0].template begin<double>());
+  return p;
+}
+
 void io_round_trips() {
   // Mesh text format and writeCellList (mesh.hpp:58-98, 184-197)
   std::istringstream m0("4 3\n0 1\n1 2\n2 3\n"), m1("5 3\n4 0\n0 1\n1 3\n");
@@ -193,6 +236,16 @@ template <bool SOA> void cpu_cases(const char *tag) {
     p.template loopCPUCellCentred<hex_scatter::StepSeq>(2);
     dump(p, std::string("hex_serial_") + tag);
   }
+  {
+    auto p = volna_mesh<SOA>();
+    p.template loopCPUCellCentred<volna::StepSeq>(2);
+    dump(p, std::string("volna_serial_") + tag);
+  }
+  {
+    auto p = mini_aero_mesh<SOA>();
+    p.template loopCPUCellCentred<mini_aero::StepSeq>(2);
+    dump(p, std::string("ma_serial_") + tag);
+  }
 }
 
 template <bool SOA> void gpu_cases(const char *tag) {
@@ -235,6 +288,16 @@ template <bool SOA> void gpu_cases(const char *tag) {
     auto p = hex_grid<SOA>();
     p.template loopGPUCellCentred<hex_scatter::StepGPUGlobal>(2);
     dump(p, "hex_global_" + t);
+  }
+  {
+    auto p = volna_mesh<SOA>();
+    p.template loopGPUHierarchical<volna::StepGPUHierarchical>(2);
+    dump(p, "volna_owner_" + t);
+  }
+  {
+    auto p = mini_aero_mesh<SOA>();
+    p.template loopGPUHierarchical<mini_aero::StepGPUHierarchical>(2);
+    dump(p, "ma_owner_" + t);
   }
 }
 

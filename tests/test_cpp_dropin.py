@@ -51,6 +51,19 @@ def _hex_oracle():
     return lambda: O.OracleProblem(O.HEX_SCATTER, [c], [60], [np.zeros((60, 3))], [F])
 
 
+def _volna_oracle():
+    from cuda_fot_amd import volna_problem
+    m = meshgen.tri_mesh(14, 9, seed=4)
+    p = volna_problem(m)
+    return lambda: O.OracleProblem(O.VOLNA, p.maps, p.num_points, p.point_data, p.cell_data)
+
+
+def _mini_aero_oracle():
+    from cuda_fot_amd import mini_aero_problem
+    p = mini_aero_problem(meshgen.hex_faces(5, 4, 3, seed=2))
+    return lambda: O.OracleProblem(O.MINI_AERO, p.maps, p.num_points, p.point_data, p.cell_data)
+
+
 def _serial(make, iters):
     o = make()
     S = o.loop_serial(iters, with_scale=True)
@@ -88,9 +101,13 @@ def test_cpp_dropin_cpu_loops(built, tmp_path):
                             what="C++ partitionOurs " + tag)
     ms, _ = _serial(mine, ITERS_MINE)
     hs, _ = _serial(hexp, ITERS_HEX)
+    vs, _ = _serial(_volna_oracle(), 2)
+    mas, _ = _serial(_mini_aero_oracle(), 2)
     for tag in ("aos", "soa"):
         assert_bitwise(_load(tmp_path, "mine_serial_" + tag, 4), ms, "C++ mine " + tag)
         assert_bitwise(_load(tmp_path, "hex_serial_" + tag, 3), hs, "C++ hex " + tag)
+        assert_bitwise(_load(tmp_path, "volna_serial_" + tag, 4), vs, "C++ volna " + tag)
+        assert_bitwise(_load(tmp_path, "ma_serial_" + tag, 5), mas, "C++ mini_aero " + tag)
 
 
 @pytest.mark.gpu
@@ -103,6 +120,8 @@ def test_cpp_dropin_gpu_loops(gpu, tmp_path):
     ms, mS = _serial(mine, ITERS_MINE)
     hs, hS = _serial(hexp, ITERS_HEX)
     hc = _colour(hexp, ITERS_HEX)
+    vs, vS = _serial(_volna_oracle(), 2)
+    mas, maS = _serial(_mini_aero_oracle(), 2)
     for tag in ("aos", "soa"):
         assert_bitwise(_load(tmp_path, "res_global_" + tag, 4), colour, "global " + tag)
         for name in ("res_atomic_", "res_owner_", "res_hier_"):
@@ -110,3 +129,7 @@ def test_cpp_dropin_gpu_loops(gpu, tmp_path):
         assert_close_scaled(_load(tmp_path, "mine_owner_" + tag, 4), ms, mS, what="mine " + tag)
         assert_close_scaled(_load(tmp_path, "hex_atomic_" + tag, 3), hs, hS, what="hex " + tag)
         assert_bitwise(_load(tmp_path, "hex_global_" + tag, 3), hc, "hex global " + tag)
+        assert_close_scaled(_load(tmp_path, "volna_owner_" + tag, 4), vs, vS,
+                            what="volna owner " + tag)
+        assert_close_scaled(_load(tmp_path, "ma_owner_" + tag, 5), mas, maS,
+                            what="mini_aero owner " + tag)
