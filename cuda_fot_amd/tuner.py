@@ -161,6 +161,50 @@ def fallback_pick(pred_ms, measured_ms, tolerance: float = 0.05):
     return pick, False
 
 
+# ---- evaluation ---------------------------------------------------------------
+DEFAULT_CHOICE = 4  # the method without the learned part: owner256 everywhere
+
+
+def evaluate(model: CostModel, X, T, tolerance: float = 0.05) -> dict:
+    """The learned choice with its fallback against the exhaustive search on
+    rows whose six times are all measured (T, ms; -1 = cannot be planned).
+    Reports, over the rows: how often the predicted-best pick is within the
+    tolerance of the best; the mean slowdown of the pick, of the choice after
+    the fallback rule and of the fixed default; how many candidates the
+    learned tuner measured against the six of the exhaustive search."""
+    X, T = np.asarray(X, np.float64), np.asarray(T, np.float64)
+    pred = model.predict(X)
+    best = np.where(T > 0, T, np.inf).min(1)
+    pick_t, final_t, measured, within, fell = [], [], 0, 0, 0
+    for i in range(len(X)):
+        pick = int(np.argmin(pred[i]))
+        choice, fb = fallback_pick(pred[i], T[i], tolerance)
+        t_pick = T[i, pick] if T[i, pick] > 0 else np.inf
+        pick_t.append(t_pick / best[i])
+        final_t.append(T[i, choice] / best[i])
+        within += t_pick <= (1 + tolerance) * best[i]
+        fell += fb
+        measured += 1 + (len(NAMES) if fb else 0)
+    d = T[:, DEFAULT_CHOICE]
+    default = np.where(d > 0, d, np.inf) / best
+    n = len(X)
+    return {"rows": n, "pick_within_tolerance": within / n,
+            "pick_mean_slowdown": float(np.mean(pick_t)),
+            "final_mean_slowdown": float(np.mean(final_t)),
+            "final_max_slowdown": float(np.max(final_t)),
+            "fallbacks": fell / n,
+            "measured_per_mesh": measured / n, "exhaustive_measured_per_mesh": len(NAMES),
+            "default_mean_slowdown": float(np.mean(default)),
+            "default_within_tolerance": float(np.mean(default <= 1 + tolerance))}
+
+
+def load_corpus(path):
+    rows = [json.loads(l) for l in open(path) if l.strip()]
+    X = np.array([r["features"] for r in rows], np.float64)
+    T = np.array([r["ms"] for r in rows], np.float64)
+    return rows, X, T
+
+
 # ---- corpus of autotuner timings ---------------------------------------------
 def corpus_meshes(n: int, seed: int = 0, min_edges: float = 5e4, max_edges: float = 1.2e6):
     """(family, ordering, size) specs: 2D quad (mine), airfoil quads
@@ -247,6 +291,30 @@ def corpus_main(argv=None):
               + " ".join(f"{v:.3f}" for v in t) + f" ({rec['seconds']} s)", flush=True)
 
 
+def eval_main(argv=None):
+    """python -m cuda_fot_amd.tuner eval CORPUS.jsonl [HELDOUT_FAMILY|-] [SEED]:
+    trains the cost model on the GPU on every mesh but the held-out family
+    (or a seeded 20 % split when '-') and prints the evaluation as JSON."""
+    argv = sys.argv[1:] if argv is None else argv
+    rows, X, T = load_corpus(argv[1])
+    held = argv[2] if len(argv) > 2 else "-"
+    seed = int(argv[3]) if len(argv) > 3 else 0
+    if held == "-":
+        test = np.random.default_rng(seed).random(len(rows)) < 0.2
+    else:
+        test = np.array([r["family"] == held for r in rows])
+    full = np.all(np.isfinite(T), 1)
+    model = CostModel(seed=seed)
+    losses = model.fit(X[~test], T[~test])
+    out = evaluate(model, X[test & full], T[test & full])
+    out.update(train_rows=int((~test).sum()), held_out=held,
+               final_train_loss=float(losses[-1]), first_train_loss=float(losses[0]))
+    print(json.dumps(out))
+    return out
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "corpus":
         corpus_main()
+    elif len(sys.argv) > 1 and sys.argv[1] == "eval":
+        eval_main()

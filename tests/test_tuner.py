@@ -57,6 +57,27 @@ def test_fallback_rule_closed_form():
     assert tuner.fallback_pick(np.array([1.0, 9, 9, 9, 9, 9]), m2) == (3, True)
 
 
+class _FixedModel:
+    def __init__(self, pred):
+        self.pred = np.asarray(pred, np.float64)
+
+    def predict(self, X):
+        return self.pred[:len(np.atleast_2d(X))]
+
+
+def test_evaluate_counts_closed_form():
+    T = np.array([[5.0, 4.0, 3.0, 2.0, 2.1, 2.5],     # best 3
+                  [1.0, 4.0, 3.0, 2.0, 2.1, 2.5]])    # best 0
+    pred = np.array([[9, 9, 9, 2.0, 3, 3],             # kept, exact
+                     [9, 9, 9, 9, 1.5, 9]])            # 2.1 > 1.05*1.5: falls back to 0
+    r = tuner.evaluate(_FixedModel(pred), np.zeros((2, 24)), T)
+    assert r["rows"] == 2 and r["pick_within_tolerance"] == 0.5
+    assert r["pick_mean_slowdown"] == pytest.approx((1.0 + 2.1) / 2)
+    assert r["final_mean_slowdown"] == 1.0 and r["fallbacks"] == 0.5
+    assert r["measured_per_mesh"] == pytest.approx((1 + 7) / 2)
+    assert r["default_mean_slowdown"] == pytest.approx((2.1 / 2.0 + 2.1) / 2)
+
+
 def test_corpus_specs_cover_families_and_orderings():
     specs = tuner.corpus_meshes(24, seed=0)
     fams = {(s["family"], s["ordering"]) for s in specs}
