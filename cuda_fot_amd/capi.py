@@ -101,7 +101,7 @@ class LoopStats(C.Structure):
                 ("lds_bytes", C.c_uint32), ("max_thread_colours", C.c_uint32),
                 ("plan_device_bytes", C.c_uint64),
                 ("avg_thread_colours", C.c_double), ("reuse_factor", C.c_double),
-                ("redundancy", C.c_double)]
+                ("redundancy", C.c_double), ("avg_cache_lines", C.c_double)]
 
 
 class BatchStats(C.Structure):
@@ -185,6 +185,7 @@ _PROTOS = {
     "fot_oplan_build": [P, U32, U32, U32, P, U32, P, P],
     "fot_oplan_get": [P, P],
     "fot_oplan_free": [P],
+    "fot_count_cache_lines": [P, U32, U32, U32, U32, C.c_int, P],
     "fot_reorder_to_partition": [P, U32, P],
     "fot_point_renumbering_by_partition": [P, U32, U32, U32, P, P],
     "fot_point_renumbering_first_touch": [P, U32, U32, U32, P],

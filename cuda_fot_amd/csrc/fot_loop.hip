@@ -1047,8 +1047,19 @@ fot_status fot_loop::build_hierarchical() {
   stats.lds_bytes = lds_bytes;
   stats.max_thread_colours = max_tc;
   stats.avg_thread_colours = nb_total ? tc_sum / nb_total : 0.0;
-  stats.reuse_factor = double(cached.size()) / (double(md) * desc.num_cells);
+  // (cached may carry padding at its end: count the plan's entries only)
+  stats.reuse_factor = double(blk_cache_off.back()) / (double(md) * desc.num_cells);
   if (owner) stats.redundancy = double(plan_cell.size()) / desc.num_cells;
+  double lines = 0;
+  for (size_t b = 0; b + 1 < blk_cache_off.size(); ++b) {
+    uint64_t l = 0;
+    if ((st = fot_count_cache_lines(cached.data() + blk_cache_off[b],
+                                    blk_cache_off[b + 1] - blk_cache_off[b], td, 8, 128,
+                                    desc.soa, &l)) != FOT_OK)
+      return st;
+    lines += double(l);
+  }
+  stats.avg_cache_lines = nb_total ? lines / nb_total : 0.0;
   return FOT_OK;
 }
 

@@ -557,3 +557,27 @@ extern "C" fot_status fot_cell_to_cell_csr(const uint32_t *const *c2n,
   *num_arcs = arcs;
   return FOT_OK;
 }
+
+// countCacheLinesForBlock (problem.hpp:654-672) with the line size as a
+// parameter: distinct lines of `line_bytes` holding the points' data (AOS:
+// the dim values of each point; SOA: one line set per component, so dim x
+// the lines of the ids).  The reference counts 32-byte sectors; CDNA's L2
+// line is 128 bytes.
+extern "C" fot_status fot_count_cache_lines(const uint32_t *points, uint32_t n, uint32_t dim,
+                                            uint32_t type_size, uint32_t line_bytes, int soa,
+                                            uint64_t *lines) {
+  if ((!points && n) || !lines || dim == 0 || type_size == 0 || line_bytes < type_size)
+    return fail(FOT_ERR_INVALID_ARGUMENT, "bad cache-line query");
+  const uint64_t per_line = line_bytes / type_size;
+  std::vector<uint64_t> ids;
+  ids.reserve(size_t(n) * (soa ? 1 : dim));
+  for (uint32_t i = 0; i < n; ++i) {
+    if (soa) ids.push_back(points[i] / per_line);
+    else
+      for (uint32_t d = 0; d < dim; ++d) ids.push_back((uint64_t(points[i]) * dim + d) / per_line);
+  }
+  std::sort(ids.begin(), ids.end());
+  const uint64_t distinct = uint64_t(std::unique(ids.begin(), ids.end()) - ids.begin());
+  *lines = (soa ? dim : 1) * distinct;
+  return FOT_OK;
+}

@@ -279,6 +279,14 @@ fot_status fot_oplan_build(const uint32_t *cell_to_node0, uint32_t num_cells,
 fot_status fot_oplan_get(const fot_oplan *plan, fot_oplan_view *view);
 void fot_oplan_free(fot_oplan *plan);
 
+/* countCacheLinesForBlock (problem.hpp:654-672) with a line size parameter
+ * (the reference's 32-byte sectors, or CDNA's 128-byte lines): distinct
+ * lines holding the data of `points` (dim values of type_size bytes each,
+ * AOS, or SOA where each component counts its own lines). */
+fot_status fot_count_cache_lines(const uint32_t *points, uint32_t n, uint32_t dim,
+                                 uint32_t type_size, uint32_t line_bytes, int soa,
+                                 uint64_t *lines);
+
 /* Mesh::reorderToPartition (mesh.hpp:227-245): sorts partition in place,
  * writes the inverse cell permutation (new cell i = old inv_perm[i]). */
 fot_status fot_reorder_to_partition(uint32_t *partition, uint32_t num_cells,
@@ -475,6 +483,9 @@ typedef struct {
   uint64_t plan_device_bytes;
   double avg_thread_colours, reuse_factor;
   double redundancy; /* owner compute: cell runs per cell (>= 1) */
+  /* hierarchical: 128-byte lines of the target data per block, the
+   * reference's "average cache_line / block" (problem.hpp:321-334) */
+  double avg_cache_lines;
 } fot_loop_stats;
 fot_status fot_loop_stats_get(const fot_loop *loop, fot_loop_stats *stats);
 

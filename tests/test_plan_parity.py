@@ -277,3 +277,29 @@ def test_generators():
     assert np.all(np.sum(step * normal, axis=1) > 0)
     assert np.all((m["adt"] >= 0.5) & (m["adt"] < 1.5))
     assert np.all(m["q"][:, 2] == 0) and abs(m["q"][:, 0].mean() - 1) < 1e-2
+
+
+def _cache_lines_reference(points, dim, type_size, line, soa):
+    # countCacheLinesForBlock (problem.hpp:654-672) with a line-size parameter
+    per = line // type_size
+    s = set()
+    for p in points:
+        if soa:
+            s.add(p // per)
+        else:
+            for d in range(dim):
+                s.add((p * dim + d) // per)
+    return (dim if soa else 1) * len(s)
+
+
+@pytest.mark.parametrize("soa", [False, True])
+def test_count_cache_lines_matches_reference_rule(soa):
+    import ctypes as C
+    from cuda_fot_amd import capi
+    rng = np.random.default_rng(7)
+    for dim, line in [(4, 32), (4, 128), (5, 128), (1, 64)]:
+        pts = rng.integers(0, 5000, size=300).astype(np.uint32)
+        out = C.c_uint64()
+        capi.call("fot_count_cache_lines", capi.ptr(pts), pts.size, dim, 8, line, int(soa),
+                  C.byref(out))
+        assert out.value == _cache_lines_reference(pts.tolist(), dim, 8, line, soa)
