@@ -13,7 +13,7 @@ import re
 import numpy as np
 
 MAX_MAPPINGS = 8
-MAX_CELL_DATA = 4
+MAX_CELL_DATA = 8
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libfot.so")
@@ -36,6 +36,7 @@ KERNEL_MINE = 1
 KERNEL_HEX_SCATTER = 2
 KERNEL_MINI_AERO = 3
 KERNEL_VOLNA = 4
+KERNEL_GETACC = 5
 
 # fot_strategy
 STRATEGY_ATOMIC = 0
@@ -204,6 +205,7 @@ _PROTOS = {
     "fot_gen_normal": [U64, U64, D, D, P],
     "fot_gen_tri_mesh": [U32, U32, U64, U32, P, P, P, P, P, P, P],
     "fot_gen_hex_faces": [U32, U32, U32, U64, U32, P, P, P],
+    "fot_gen_quad_grid": [U32, U32, P],
     "fot_mini_aero_face_flux": [P, P, U32, P, U32, P],
     "fot_loop_create": [P, P, P, P],
     "fot_loop_destroy": [P],

@@ -380,6 +380,35 @@ struct VolnaF {
   }
 };
 
+// getacc_scatter (kernels/getacc_scatter.hpp:27-78, getacc_scatter_func.hpp:
+// 17-94): corner jj of a quad adds its mass (cnmass[jj], else the previous
+// corner's, else rho * cnwt[jj], with the 1e-40 cut), cnwt, cnfx and cnfy.
+// Increments start at zero and are added with +=, as the reference does.
+struct GetaccF {
+  static constexpr uint32_t MD = 4, TD = 4;
+  static constexpr bool SLOT_OUTER = false;
+  FOT_DEV static void user(const double *cnmass, const double *rho, const double *cnwt,
+                           const double *cnfx, const double *cnfy, uint32_t cs, double *inc) {
+    const double zerocut = 1.0e-40;
+#pragma unroll
+    for (uint32_t jj = 0; jj < 4; ++jj) {
+      const uint32_t ii = (jj + 3) & 3u;
+      double m = 0.0;
+      if (cnmass[jj * cs] > zerocut) m += cnmass[jj * cs];
+      else if (cnmass[ii * cs] > zerocut) m += cnmass[ii * cs];
+      else m += *rho * cnwt[jj * cs];
+      inc[4 * jj + 0] = m;
+      inc[4 * jj + 1] = 0.0 + cnwt[jj * cs];
+      inc[4 * jj + 2] = 0.0 + cnfx[jj * cs];
+      inc[4 * jj + 3] = 0.0 + cnfy[jj * cs];
+    }
+  }
+  template <bool SOA, class A>
+  FOT_DEV static void compute(const A &a, uint32_t ind, const uint32_t *, double *inc) {
+    user(a.cd(0) + ind, a.cd(1) + ind, a.cd(2) + ind, a.cd(3) + ind, a.cd(4) + ind, a.cs(), inc);
+  }
+};
+
 // Mapping-0 ids of cell `ind` (AOS row).
 template <class F, class A>
 FOT_DEV void targets(const A &a, uint32_t ind, uint32_t *tgt) {

@@ -1169,6 +1169,11 @@ extern "C" fot_status fot_loop_create(const fot_functor *f, const fot_problem_de
   if ((f->kernel == FOT_KERNEL_MINE || f->kernel == FOT_KERNEL_HEX_SCATTER) &&
       (desc->num_cell_data < 1 || desc->cell_dim[0] != f->cell_dim))
     return fail(FOT_ERR_INVALID_ARGUMENT, "functor needs cell data of dim %u", f->cell_dim);
+  if (f->kernel == FOT_KERNEL_GETACC &&
+      (desc->num_cell_data < 5 || desc->cell_dim[0] != 4 || desc->cell_dim[1] != 1 ||
+       desc->cell_dim[2] != 4 || desc->cell_dim[3] != 4 || desc->cell_dim[4] != 4))
+    return fail(FOT_ERR_INVALID_ARGUMENT,
+                "getacc needs cnmass(4), rho(1), cnwt(4), cnfx(4), cnfy(4) cell data");
   if (f->kernel == FOT_KERNEL_MINI_AERO &&
       (desc->num_mappings < 2 || desc->point_dim[1] != 28 ||
        desc->num_points[1] < desc->num_points[0] || desc->num_cell_data < 1 ||

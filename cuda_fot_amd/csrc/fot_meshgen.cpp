@@ -117,6 +117,21 @@ extern "C" fot_status fot_gen_hex_grid(uint32_t N1, uint32_t N2, uint32_t N3,
   return FOT_OK;
 }
 
+// Grid::fillCellList (grid.hpp:386-411): quads of an N x M point grid
+extern "C" fot_status fot_gen_quad_grid(uint32_t N, uint32_t M, uint32_t *cells) {
+  if (!cells || N < 2 || M < 2) return fail(FOT_ERR_INVALID_ARGUMENT, "quad grid needs N,M >= 2");
+  size_t k = 0;
+  for (uint32_t r = 0; r + 1 < N; ++r)
+    for (uint32_t c = 0; c + 1 < M; ++c) {
+      const uint32_t top_left = r * M + c, bottom_left = (r + 1) * M + c;
+      cells[k++] = top_left;
+      cells[k++] = top_left + 1;
+      cells[k++] = bottom_left;
+      cells[k++] = bottom_left + 1;
+    }
+  return FOT_OK;
+}
+
 // Airfoil-like structured quad mesh with interior edges only.  Edge order
 // before shuffling follows the cells row by row: for cell (i, j) the edge
 // below it (j > 0), then the edge left of it (i > 0).

@@ -86,6 +86,7 @@ template <class F, bool SOA> struct HierGatherGlobalIds {
 };
 template <bool SOA> struct HierGather<VolnaF, SOA> : HierGatherGlobalIds<VolnaF, SOA> {};
 template <bool SOA> struct HierGather<MiniAeroF, SOA> : HierGatherGlobalIds<MiniAeroF, SOA> {};
+template <bool SOA> struct HierGather<GetaccF, SOA> : HierGatherGlobalIds<GetaccF, SOA> {};
 
 __device__ uint32_t g_lds_overflow; // set when a block's cache exceeds cache_size
 
@@ -163,6 +164,9 @@ extern "C" fot_status fot_functor_init(fot_functor *f, fot_kernel kernel, uint32
     return FOT_OK;
   case FOT_KERNEL_MINI_AERO: // kernels/mini_aero.hpp:16-19
     f->mesh_dim = 2, f->point_dim = 5, f->cell_dim = 12;
+    return FOT_OK;
+  case FOT_KERNEL_GETACC: // kernels/getacc_scatter.hpp:22-23 (cell_dim: cnmass)
+    f->mesh_dim = 4, f->point_dim = 4, f->cell_dim = 4;
     return FOT_OK;
   case FOT_KERNEL_VOLNA: // kernels/volna.hpp:16-19 (cell_dim: the fluxes)
     f->mesh_dim = 2, f->point_dim = 4, f->cell_dim = 3;

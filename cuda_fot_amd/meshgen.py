@@ -83,3 +83,11 @@ def hex_faces(n1: int, n2: int, n3: int, seed: int = 0, shuffle: int = 6) -> dic
          ptr(out["face"]), ptr(out["cells"]))
     out.update(num_cells=nc, num_faces=nf)
     return out
+
+
+def quad_grid(N: int, M: int) -> np.ndarray:
+    """Grid::fillCellList (grid.hpp:386-411): (C, 4) u32 quads of an N x M
+    point grid."""
+    c = np.empty(((N - 1) * (M - 1), 4), dtype=np.uint32)
+    call("fot_gen_quad_grid", N, M, ptr(c))
+    return c

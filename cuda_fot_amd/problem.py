@@ -229,3 +229,25 @@ def mini_aero_problem(mesh: dict, block_size: int = DEFAULT_BLOCK_SIZE,
     cells = mesh["cells"].T.copy() if soa else mesh["cells"].copy()
     maps = [mesh["face_cells"], mesh["face_cells"]]
     return Problem(maps, [nc, nc], [flux, cells], [mesh["face"]], block_size, soa)
+
+
+def getacc_problem(N: int, M: int, seed: int = 0, block_size: int = DEFAULT_BLOCK_SIZE,
+                   soa: bool = False, shuffle: bool = True) -> Problem:
+    """The corner scatter as getacc.cu:10-42 sets it up, on synthetic data:
+    fillCellList quads (cells shuffled); point data ndmass / ndarea / ndub /
+    ndvb (dim 4, zero); cell data cnmass U[0,1) with a third of the corners
+    zero (so both fallbacks run), rho U[0.5,1.5), cnwt U[0,1), cnfx and cnfy
+    N(0,1)."""
+    from . import meshgen
+    cells = meshgen.quad_grid(N, M)
+    if shuffle:
+        cells = cells[meshgen.permutation(cells.shape[0], seed + 1)]
+    nc, np_ = cells.shape[0], N * M
+    cnmass = meshgen.uniform(4 * nc, seed + 2).reshape(4, nc)
+    cnmass[meshgen.uniform(4 * nc, seed + 3).reshape(4, nc) < 1 / 3] = 0.0
+    rho = meshgen.uniform(nc, seed + 4, 0.5, 1.5).reshape(1, nc)
+    cnwt = meshgen.uniform(4 * nc, seed + 5).reshape(4, nc)
+    cnfx = meshgen.normal(4 * nc, seed + 6).reshape(4, nc)
+    cnfy = meshgen.normal(4 * nc, seed + 7).reshape(4, nc)
+    out = np.zeros((4, np_)) if soa else np.zeros((np_, 4))
+    return Problem([cells], [np_], [out], [cnmass, rho, cnwt, cnfx, cnfy], block_size, soa)

@@ -30,7 +30,7 @@ extern "C" {
 #endif
 
 #define FOT_MAX_MAPPINGS 8
-#define FOT_MAX_CELL_DATA 4
+#define FOT_MAX_CELL_DATA 8
 
 typedef enum {
   FOT_OK = 0,
@@ -109,7 +109,11 @@ typedef enum {
    * state (dim 4, target), point data 1 cell volume (dim 1); cell data
    * fluxes (3), bathymetry source (2), edge normals (2), right-boundary flag
    * (1, stored as 0.0 / 1.0 where the reference has an int) */
-  FOT_KERNEL_VOLNA = 4
+  FOT_KERNEL_VOLNA = 4,
+  /* corner scatter of a 2D quad mesh, kernels/getacc_scatter.hpp +
+   * getacc_scatter_func.hpp: 4 points of dim 4 (ndmass, ndarea, ndub, ndvb);
+   * cell data cnmass (4), rho (1), cnwt (4), cnfx (4), cnfy (4) */
+  FOT_KERNEL_GETACC = 5
 } fot_kernel;
 
 typedef struct {
@@ -369,6 +373,9 @@ fot_status fot_gen_grid_edges(uint32_t N, uint32_t M, uint32_t *edges);
 /* Grid::fillCellList3D (grid.hpp:415-460): N1 x N2 x N3 points, hexes with
  * 8 corners each; cells[8*(N1-1)*(N2-1)*(N3-1)]. */
 fot_status fot_gen_hex_grid(uint32_t N1, uint32_t N2, uint32_t N3, uint32_t *cells);
+/* Grid::fillCellList (grid.hpp:386-411): N x M points, quads with corners
+ * (top_left, top_left + 1, bottom_left, bottom_left + 1); cells[4(N-1)(M-1)]. */
+fot_status fot_gen_quad_grid(uint32_t N, uint32_t M, uint32_t *cells);
 
 /* Airfoil-like quad mesh of nx x ny cells: (nx+1)(ny+1) nodes perturbed by
  * U(-0.2,0.2), interior edges only, pedge/pecell oriented so that (dy,-dx)

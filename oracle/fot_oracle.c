@@ -389,6 +389,33 @@ static void compute_inc(const oracle_problem *p, uint32_t ind, double *inc) {
         inc[j * 3 + i] = F[(size_t)(i * 8 + j) * cs];
     break;
   }
+  case ORACLE_GETACC: {
+    /* getacc::StepSeq (kernels/getacc_scatter.hpp:27-78) with the user
+     * function (getacc_scatter_func.hpp:17-94): cell data cnmass, rho, cnwt,
+     * cnfx, cnfy (SOA, stride num_cells); corner jj's mass falls back to the
+     * previous corner's (3 before 0), then to rho * cnwt[jj]. */
+    const uint32_t cs = p->num_cells;
+    const double *cnmass = p->cell_data[0] + ind, *rho = p->cell_data[1] + ind;
+    const double *cnwt = p->cell_data[2] + ind, *cnfx = p->cell_data[3] + ind;
+    const double *cnfy = p->cell_data[4] + ind;
+    const double zerocut = 1.0e-40;
+    for (int k = 0; k < 16; ++k) inc[k] = 0;
+    for (int jj = 0; jj < 4; ++jj) {
+      double *pt = inc + 4 * jj;
+      if (cnmass[jj * cs] > zerocut) {
+        pt[0] += cnmass[jj * cs];
+      } else {
+        int ii = jj - 1;
+        if (ii == -1) ii = 3;
+        if (cnmass[ii * cs] > zerocut) pt[0] += cnmass[ii * cs];
+        else pt[0] += *rho * cnwt[jj * cs];
+      }
+      pt[1] += cnwt[jj * cs];
+      pt[2] += cnfx[jj * cs];
+      pt[3] += cnfy[jj * cs];
+    }
+    break;
+  }
   case ORACLE_MINI_AERO: {
     /* mini_aero::StepSeq (kernels/mini_aero.hpp:43-117): cell values (28)
      * through mapping-0 ids, the face's 12 values as SOA cell data; left

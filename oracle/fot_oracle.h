@@ -25,7 +25,7 @@ extern "C" {
 #endif
 
 #define ORACLE_MAX_MAPPINGS 8
-#define ORACLE_MAX_CELL_DATA 4
+#define ORACLE_MAX_CELL_DATA 8
 
 /* kernels (user functions) restated by the oracle */
 enum {
@@ -35,7 +35,8 @@ enum {
   ORACLE_MINE8 = 3,     /* kernels/mine_func.hpp:58-99, MeshDim 8     */
   ORACLE_HEX_SCATTER = 4, /* lulesh struct API, scatter-only functor  */
   ORACLE_MINI_AERO = 5, /* kernels/mini_aero_func.hpp:12-85           */
-  ORACLE_VOLNA = 6      /* kernels/volna_func.hpp:16-36 in FP64       */
+  ORACLE_VOLNA = 6,     /* kernels/volna_func.hpp:16-36 in FP64       */
+  ORACLE_GETACC = 7     /* kernels/getacc_scatter_func.hpp:17-94      */
 };
 
 /*

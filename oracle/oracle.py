@@ -14,8 +14,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "liboracle.so")
 
-RES_CALC, MINE2, MINE4, MINE8, HEX_SCATTER, MINI_AERO, VOLNA = range(7)
-MAXM, MAXCD = 8, 4
+RES_CALC, MINE2, MINE4, MINE8, HEX_SCATTER, MINI_AERO, VOLNA, GETACC = range(8)
+MAXM, MAXCD = 8, 8
 
 
 def build() -> None:
