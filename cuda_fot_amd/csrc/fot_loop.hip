@@ -441,11 +441,11 @@ __global__ __launch_bounds__(256) void batch_copy_kernel(const CopyItem *__restr
   for (uint32_t i = threadIdx.x; i < it.count; i += blockDim.x) it.dst[i] = it.src[i];
 }
 
-// Per-block record of the owner-compute res_calc kernel: one scalar load.
-// entries appended to the owner res_calc plan arrays: covers reads up to
-// threads * max(cells, staged points) per thread past a block's start
+// Entries appended to the owner res_calc plan arrays: covers reads up to
+// threads * max(cells, staged points) per thread past a block's start.
 constexpr uint32_t kOwnerPad = 4096;
 
+// Per-block record of the owner-compute res_calc kernel: one scalar load.
 struct OwnerMeta {
   uint32_t cell_off, ncells, cache_off, ncached;
   uint32_t nowned, own_start, ncol, contiguous; // owned ids own_start + i when contiguous

@@ -93,14 +93,16 @@ def test_measure_and_learned_tuner(gpu):
     best, t = tuner.Tuner.exhaustive(p, f, warmup=2, steps=5)
     assert np.all(t > 0), t
     assert t[best] == t.min()
-    # a model that predicts the measured times exactly keeps its pick
+    # a model fitted to this mesh's own timings (identical rows: the
+    # standardised features are zero, so it only has to learn its output
+    # biases) predicts them and keeps its pick
     model = tuner.CostModel(seed=0)
     x = tuner.features(p, f)[None, :]
-    X = np.repeat(x, 8, 0) + np.random.default_rng(0).normal(0, 1e-3, (8, 24))
-    losses = model.fit(X, np.repeat(t[None, :], 8, 0), epochs=1500, lr=3e-3)
-    assert losses[-1] < 0.05 * losses[0]
+    losses = model.fit(np.repeat(x, 8, 0), np.repeat(t[None, :], 8, 0), epochs=3000, lr=3e-3,
+                       weight_decay=0.0)
+    assert losses[-1] < 1e-3 * losses[0]
     pred = model.predict(x)[0]
-    np.testing.assert_allclose(pred, t, rtol=0.1)
+    np.testing.assert_allclose(pred, t, rtol=0.05)
     res = tuner.Tuner(model, tolerance=0.5).learned(p, f, warmup=2, steps=5)
     assert res["pick"] == int(np.argmin(pred))
     assert res["measured"] in (1, 7)
