@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--ept", type=int, default=2, choices=[1, 2],
                     help="owner compute: cells per thread")
     ap.add_argument("--no-stage", action="store_true")
+    ap.add_argument("--point-terms", action="store_true",
+                    help="owner compute: 1/rho and pressure once per staged cell")
     ap.add_argument("--nodes", default="gather", choices=["gather", "stage"],
                     help="owner compute: gather node coordinates per edge, or stage each "
                          "block's nodes in LDS first")
@@ -92,7 +94,8 @@ def build_problem(args, partition_kind):
     return mesh, p
 
 
-def make_loop(p, strategy, block_size, stage, graph, order="launch", ept=1, gather=False):
+def make_loop(p, strategy, block_size, stage, graph, order="launch", ept=1, gather=False,
+              point_terms=False):
     from cuda_fot_amd import DeviceArray, Loop, capi
     strat = {"atomic": capi.STRATEGY_ATOMIC, "global": capi.STRATEGY_GLOBAL_COLOUR,
              "hierarchical": capi.STRATEGY_HIERARCHICAL}[strategy]
@@ -104,6 +107,8 @@ def make_loop(p, strategy, block_size, stage, graph, order="launch", ept=1, gath
         flags |= capi.LOOP_TWO_PER_THREAD
     if gather:
         flags |= capi.LOOP_GATHER_NODES
+    if point_terms:
+        flags |= capi.LOOP_POINT_TERMS
     part = p.partition_vector if strat == capi.STRATEGY_HIERARCHICAL else None
     if strat == capi.STRATEGY_HIERARCHICAL and order == "owner" and p.point_partition is not None:
         part = p.point_partition
@@ -129,6 +134,39 @@ def time_loop(loop, steps, warmup):
     synchronize()
     wall = time.perf_counter() - t0
     return wall, t.elapsed_ms() / 1e3
+
+
+def launch_durations(loop, steps):
+    """Mean and median duration of one launch: a HIP event pair recorded
+    around each of `steps` launches on the loop's stream (the kernel's own
+    time, without the gaps between back-to-back launches)."""
+    from cuda_fot_amd import Timer, synchronize
+    timers = [Timer() for _ in range(steps)]
+    synchronize()
+    for t in timers:
+        t.start()
+        loop.step()
+        t.stop()
+    synchronize()
+    ms = np.array([t.elapsed_ms() for t in timers])
+    return float(ms.mean()), float(np.median(ms))
+
+
+def load_traffic(args):
+    """HBM bytes per launch from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE, scaled by a copy-kernel calibration) when it
+    was taken on this configuration; else None."""
+    path = os.path.join(ROOT, "profiles", "res_calc_owner2_airfoil_1200x600_traffic.json")
+    if not os.path.exists(path):
+        return None
+    t = json.load(open(path))
+    c = t["config"]
+    same = (args.strategy == "hierarchical" and args.order == "owner" and
+            args.partition == c["partition"] and args.nx == c["nx"] and args.ny == c["ny"] and
+            args.points_per_block == c["points_per_block"] and
+            args.ept == c["cells_per_thread"] and args.nodes == c["nodes"] and
+            args.point_terms == c.get("point_terms", False))
+    return t["traffic_bytes"] if same else None
 
 
 def cpu_baseline(mesh, seconds):
@@ -317,14 +355,18 @@ def main():
     t_plan = time.perf_counter()
     mesh, p = build_problem(args, args.partition)
     loop, dev, out = make_loop(p, args.strategy, args.block_size, not args.no_stage,
-                               args.graph, args.order, args.ept, args.nodes == "gather")
+                               args.graph, args.order, args.ept, args.nodes == "gather", args.point_terms)
     plan_s = time.perf_counter() - t_plan
     stats = loop.stats()
     wall, ev = time_loop(loop, args.steps, args.warmup)
+    launch_ms, launch_med = launch_durations(loop, args.steps)
     capi.call("fot_loop_check", loop.handle)
     ne = mesh["num_edges"]
     ubytes = unique_bytes(mesh["num_nodes"], mesh["num_cells"], ne)
-    achieved = ubytes * args.steps / ev
+    # the dominant kernel's bandwidth: algorithmic bytes over its mean launch
+    # duration (event pairs around each launch)
+    achieved = ubytes / (launch_ms * 1e-3)
+    traffic = load_traffic(args)
     result = {
         "metric": "res_calc edges/s, achieved HBM GB/s (% of peak); learned-tuned vs plain atomics",
         "value": ne * args.steps / wall,
@@ -345,7 +387,7 @@ def main():
                    "points_per_block": args.points_per_block if args.partition == "points" else None,
                    "stage_inputs": not args.no_stage, "graph": args.graph,
                    "block_order": args.order, "cells_per_thread": args.ept,
-                   "nodes": args.nodes,
+                   "nodes": args.nodes, "point_terms": args.point_terms,
                    "launches_per_step": stats["num_launches_per_iter"],
                    "block_colours": stats["num_colours"],
                    "avg_thread_colours": round(stats["avg_thread_colours"], 3),
@@ -353,8 +395,9 @@ def main():
                    "redundancy": round(stats["redundancy"], 4),
                    "plan_seconds": round(plan_s, 2)},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
-                     "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": None,
-                     "algorithmic_bytes_per_step": ubytes,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": ubytes,
+                     "launch_ms_mean": launch_ms, "launch_ms_median": launch_med,
                      "event_ms_per_step": ev / args.steps * 1e3},
     }
     loop.close()

@@ -49,6 +49,7 @@ LOOP_OWNER_COMPUTE = 4
 LOOP_POINT_PARTITION = 8
 LOOP_TWO_PER_THREAD = 16
 LOOP_GATHER_NODES = 32
+LOOP_POINT_TERMS = 64
 
 
 class FotError(RuntimeError):

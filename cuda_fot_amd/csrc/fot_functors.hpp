@@ -56,6 +56,42 @@ FOT_DEV void res_calc_user(const double *__restrict__ x1, const double *__restri
   res2[3] = -f;
 }
 
+// The per-point terms of res_calc_user (1/rho and the pressure), with the
+// same operations, so a kernel can compute them once per cell of its block.
+FOT_DEV void res_calc_point_terms(const double *q, uint32_t qs, double *ri, double *p) {
+  const double gm1 = 0.4;
+  const double r = 1.0 / q[0];
+  *ri = r;
+  *p = gm1 * (q[3 * qs] - 0.5 * r * (q[qs] * q[qs] + q[2 * qs] * q[2 * qs]));
+}
+
+// res_calc_user with the per-point terms given: the same operations on the
+// same values, so the same bits.
+FOT_DEV void res_calc_user_terms(const double *__restrict__ x1, const double *__restrict__ x2,
+                                 const double *__restrict__ q1, const double *__restrict__ q2,
+                                 double ri1, double ri2, double p1, double p2, double adt1,
+                                 double adt2, double *__restrict__ res1,
+                                 double *__restrict__ res2) {
+  const double eps = 0.05;
+  const double dx = x1[0] - x2[0];
+  const double dy = x1[1] - x2[1];
+  const double vol1 = ri1 * (q1[1] * dy - q1[2] * dx);
+  const double vol2 = ri2 * (q2[1] * dy - q2[2] * dx);
+  const double mu = 0.5 * (adt1 + adt2) * eps;
+  double f = 0.5 * (vol1 * q1[0] + vol2 * q2[0]) + mu * (q1[0] - q2[0]);
+  res1[0] = f;
+  res2[0] = -f;
+  f = 0.5 * (vol1 * q1[1] + p1 * dy + vol2 * q2[1] + p2 * dy) + mu * (q1[1] - q2[1]);
+  res1[1] = f;
+  res2[1] = -f;
+  f = 0.5 * (vol1 * q1[2] - p1 * dx + vol2 * q2[2] - p2 * dx) + mu * (q1[2] - q2[2]);
+  res1[2] = f;
+  res2[2] = -f;
+  f = 0.5 * (vol1 * (q1[3] + p1) + vol2 * (q2[3] + p2)) + mu * (q1[3] - q2[3]);
+  res1[3] = f;
+  res2[3] = -f;
+}
+
 // Arguments of the reference-shaped launches: device arrays of device
 // pointers / strides exactly as Problem builds them (problem.hpp:158-176,
 // 258-265).

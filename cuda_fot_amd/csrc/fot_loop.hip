@@ -456,7 +456,7 @@ struct OwnerMeta {
 // res_calc owner compute, latency-oriented: block record in one scalar load,
 // owned points addressed directly when the mesh numbers them contiguously,
 // EPT edges per thread so each wave keeps more gathers in flight.
-template <bool SOA, int EPT, bool SX, int KS>
+template <bool SOA, int EPT, bool SX, int KS, bool PRE>
 __global__ __launch_bounds__(1024) void res_calc_owner2_kernel(EngineArgs a,
                                                                const OwnerMeta *__restrict__ meta,
                                                                const uint32_t *__restrict__ cached,
@@ -474,7 +474,9 @@ __global__ __launch_bounds__(1024) void res_calc_owner2_kernel(EngineArgs a,
   // staged nodes first: dynamic LDS starts 16-byte aligned
   double2 *x_s = reinterpret_cast<double2 *>(lds);
   double *res_s = lds + 2 * nx_b, *q_s = res_s + 4 * nw, *adt_s = q_s + 4 * m;
-  uint32_t *g_s = reinterpret_cast<uint32_t *>(adt_s + m);
+  // with PRE, 1/rho and the pressure of every staged cell follow adt
+  double *ri_s = adt_s + m, *p_s = ri_s + (PRE ? m : 0);
+  uint32_t *g_s = reinterpret_cast<uint32_t *>(p_s + (PRE ? m : 0));
   const double *x = a.point_data[1], *q = a.point_data[2], *adt = a.point_data[3];
   const uint32_t np = a.num_points[0];
   const uint32_t *pcached = cached + mt.cache_off;
@@ -548,6 +550,10 @@ __global__ __launch_bounds__(1024) void res_calc_owner2_kernel(EngineArgs a,
       q_s[0 * m + i] = vq0[k].x, q_s[1 * m + i] = vq0[k].y;
       q_s[2 * m + i] = vq1[k].x, q_s[3 * m + i] = vq1[k].y;
       adt_s[i] = vadt[k];
+      if (PRE) {
+        const double qq[4] = {vq0[k].x, vq0[k].y, vq1[k].x, vq1[k].y};
+        res_calc_point_terms(qq, 1, &ri_s[i], &p_s[i]);
+      }
       if (i < nw) {
         res_s[0 * nw + i] = vr0[k].x, res_s[1 * nw + i] = vr0[k].y;
         res_s[2 * nw + i] = vr1[k].x, res_s[3 * nw + i] = vr1[k].y;
@@ -586,7 +592,11 @@ __global__ __launch_bounds__(1024) void res_calc_owner2_kernel(EngineArgs a,
 #pragma unroll
       for (uint32_t d = 0; d < 4; ++d) q1[d] = q_s[d * m + l0], q2[d] = q_s[d * m + l1];
       const double a1 = adt_s[l0], a2 = adt_s[l1];
-      res_calc_user(x1[e], x2[e], q1, q2, &a1, &a2, inc[e], inc[e] + 4, 1, 1);
+      if (PRE)
+        res_calc_user_terms(x1[e], x2[e], q1, q2, ri_s[l0], ri_s[l1], p_s[l0], p_s[l1], a1, a2,
+                            inc[e], inc[e] + 4);
+      else
+        res_calc_user(x1[e], x2[e], q1, q2, &a1, &a2, inc[e], inc[e] + 4, 1, 1);
     }
   }
   for (uint32_t c = 0; c < mt.ncol; ++c) {
@@ -656,6 +666,7 @@ struct fot_loop {
   const OwnerMeta *owner_meta = nullptr;
   const uint32_t *owner_nodes = nullptr, *owner_nloc = nullptr;
   bool stage_x = false;
+  bool point_terms = false;      // owner res_calc: per-cell 1/rho and pressure
   uint32_t stage_per_thread = 1; // owner res_calc: staged points per thread
   uint32_t ept = 1;
   DepDev dep{};
@@ -880,6 +891,7 @@ fot_status fot_loop::build_hierarchical() {
   // owner res_calc loads plan arrays without bounds branches (clamped reads
   // past a block's end land in the next block or this padding)
   const bool owner_rc = owner && f.kernel == FOT_KERNEL_RES_CALC;
+  point_terms = owner_rc && (cfg.flags & FOT_LOOP_POINT_TERMS);
   if (owner_rc) cached.resize(cached.size() + kOwnerPad, 0u);
   if ((st = upload(cached.data(), cached.size(), &d32)) != FOT_OK) return st;
   hier.cached = d32;
@@ -1015,6 +1027,12 @@ fot_status fot_loop::build_hierarchical() {
       lds_bytes = std::max(lds_bytes, bytes);
     }
     if (f.kernel == FOT_KERNEL_RES_CALC && stage_x) lds_bytes += 16u * max_nodes; // staged x
+    if (f.kernel == FOT_KERNEL_RES_CALC && point_terms) {
+      uint32_t mm = 0; // + 1/rho and pressure of the largest block's cells
+      for (size_t b = 0; b + 1 < blk_cache_off.size(); ++b)
+        mm = std::max(mm, blk_cache_off[b + 1] - blk_cache_off[b]);
+      lds_bytes += 16u * mm;
+    }
   }
   if (lds_bytes > 160u * 1024u)
     return fail(FOT_ERR_UNSUPPORTED, "block cache needs %u B of LDS (max 160 KiB)", lds_bytes);
@@ -1083,10 +1101,15 @@ fot_status fot_loop::enqueue(hipStream_t s) {
       // no cells: nothing to add (and the staging reads assume point 0 exists)
       if (nb == 0 || desc.num_cells == 0) return FOT_OK;
       if constexpr (std::is_same<F, ResCalcF>::value) {
-#define FOT_OWNER2(E_, X_, K_)                                                          \
-  hipLaunchKernelGGL((res_calc_owner2_kernel<SOA, E_, X_, K_>), dim3(nb), dim3(threads),     \
+#define FOT_OWNER2P(E_, X_, K_, P_)                                                     \
+  hipLaunchKernelGGL((res_calc_owner2_kernel<SOA, E_, X_, K_, P_>), dim3(nb), dim3(threads), \
                      lds_bytes, s, plan_args, owner_meta, hier.cached, hier.packed,          \
                      owner_nodes, owner_nloc, nb)
+#define FOT_OWNER2(E_, X_, K_)                                                          \
+  do {                                                                                  \
+    if (point_terms) FOT_OWNER2P(E_, X_, K_, true);                                     \
+    else FOT_OWNER2P(E_, X_, K_, false);                                                \
+  } while (0)
 #define FOT_OWNER2_K(E_, X_)                                                            \
   switch (stage_per_thread) {                                                           \
   case 1: FOT_OWNER2(E_, X_, 1); break;                                                 \
@@ -1103,6 +1126,7 @@ fot_status fot_loop::enqueue(hipStream_t s) {
         }
 #undef FOT_OWNER2_K
 #undef FOT_OWNER2
+#undef FOT_OWNER2P
       } else
         hipLaunchKernelGGL((eng_owner_kernel<F, SOA>), dim3(nb), dim3(threads), lds_bytes, s,
                            plan_args, hier, nb);

@@ -469,6 +469,10 @@ typedef struct {
 /* owner compute, res_calc: gather mapping-1 (node) data straight from HBM per
  * cell instead of staging each block's nodes in LDS first */
 #define FOT_LOOP_GATHER_NODES 32u
+/* owner compute, res_calc: 1/rho and the pressure computed once per staged
+ * cell (LDS) instead of once per edge side; the same operations, the same
+ * bits */
+#define FOT_LOOP_POINT_TERMS 64u
 
 typedef struct fot_loop fot_loop;
 /* Builds the plan (host), uploads it, permutes cell data. */

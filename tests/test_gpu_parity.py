@@ -61,6 +61,7 @@ STAGE = capi.LOOP_STAGE_INPUTS
 OWN = capi.LOOP_OWNER_COMPUTE
 TWO = capi.LOOP_TWO_PER_THREAD
 GATHER = capi.LOOP_GATHER_NODES
+TERMS = capi.LOOP_POINT_TERMS
 
 
 def _owner_replay(p, kernel, block_size, partition=None, iters=ITERS, point_partition=None):
@@ -80,6 +81,9 @@ def _owner_replay(p, kernel, block_size, partition=None, iters=ITERS, point_part
                                                  (False, 128, OWN | TWO, "points"),
                                                  (True, 200, OWN | TWO, "points"),
                                                  (False, 192, OWN | TWO | GATHER, "points"),
+                                                 (False, 192, OWN | TWO | GATHER | TERMS,
+                                                  "points"),
+                                                 (True, 160, OWN | TERMS, "points"),
                                                  (True, 128, OWN | GATHER, "points"),
                                                  (False, 256, OWN | TWO, None)])
 def test_res_calc_owner_compute_bitwise(gpu, soa, bs, flags, method):
