@@ -54,10 +54,13 @@ def parse():
     ap.add_argument("--partition", default="points",
                     choices=["none", "kway", "ours", "gps", "points"],
                     help="points: owner partition of mapping-0 points (owner compute)")
-    ap.add_argument("--points-per-block", type=int, default=192)
+    ap.add_argument("--points-per-block", type=int, default=224)
     ap.add_argument("--ept", type=int, default=2, choices=[1, 2],
                     help="owner compute: cells per thread")
     ap.add_argument("--no-stage", action="store_true")
+    ap.add_argument("--accumulate", default="gather", choices=["gather", "colours"],
+                    help="owner compute: owned cells gather their edges' fluxes from LDS "
+                         "(three barriers), or add them colour by colour (a barrier each)")
     ap.add_argument("--point-terms", action="store_true",
                     help="owner compute: 1/rho and pressure once per staged cell")
     ap.add_argument("--nodes", default="gather", choices=["gather", "stage"],
@@ -95,7 +98,7 @@ def build_problem(args, partition_kind):
 
 
 def make_loop(p, strategy, block_size, stage, graph, order="launch", ept=1, gather=False,
-              point_terms=False):
+              point_terms=False, owner_gather=False):
     from cuda_fot_amd import DeviceArray, Loop, capi
     strat = {"atomic": capi.STRATEGY_ATOMIC, "global": capi.STRATEGY_GLOBAL_COLOUR,
              "hierarchical": capi.STRATEGY_HIERARCHICAL}[strategy]
@@ -109,6 +112,8 @@ def make_loop(p, strategy, block_size, stage, graph, order="launch", ept=1, gath
         flags |= capi.LOOP_GATHER_NODES
     if point_terms:
         flags |= capi.LOOP_POINT_TERMS
+    if owner_gather:
+        flags |= capi.LOOP_OWNER_GATHER
     part = p.partition_vector if strat == capi.STRATEGY_HIERARCHICAL else None
     if strat == capi.STRATEGY_HIERARCHICAL and order == "owner" and p.point_partition is not None:
         part = p.point_partition
@@ -156,7 +161,7 @@ def load_traffic(args):
     """HBM bytes per launch from the committed rocprofv3 counter summary
     (FETCH_SIZE and WRITE_SIZE, scaled by a copy-kernel calibration) when it
     was taken on this configuration; else None."""
-    path = os.path.join(ROOT, "profiles", "res_calc_owner2_airfoil_1200x600_traffic.json")
+    path = os.path.join(ROOT, "profiles", "res_calc_owner_airfoil_1200x600_traffic.json")
     if not os.path.exists(path):
         return None
     t = json.load(open(path))
@@ -165,7 +170,8 @@ def load_traffic(args):
             args.partition == c["partition"] and args.nx == c["nx"] and args.ny == c["ny"] and
             args.points_per_block == c["points_per_block"] and
             args.ept == c["cells_per_thread"] and args.nodes == c["nodes"] and
-            args.point_terms == c.get("point_terms", False))
+            args.point_terms == c.get("point_terms", False) and
+            args.accumulate == c.get("accumulate", "colours"))
     return t["traffic_bytes"] if same else None
 
 
@@ -355,7 +361,8 @@ def main():
     t_plan = time.perf_counter()
     mesh, p = build_problem(args, args.partition)
     loop, dev, out = make_loop(p, args.strategy, args.block_size, not args.no_stage,
-                               args.graph, args.order, args.ept, args.nodes == "gather", args.point_terms)
+                               args.graph, args.order, args.ept, args.nodes == "gather", args.point_terms,
+                               args.accumulate == "gather")
     plan_s = time.perf_counter() - t_plan
     stats = loop.stats()
     wall, ev = time_loop(loop, args.steps, args.warmup)
@@ -363,9 +370,10 @@ def main():
     capi.call("fot_loop_check", loop.handle)
     ne = mesh["num_edges"]
     ubytes = unique_bytes(mesh["num_nodes"], mesh["num_cells"], ne)
-    # the dominant kernel's bandwidth: algorithmic bytes over its mean launch
-    # duration (event pairs around each launch)
-    achieved = ubytes / (launch_ms * 1e-3)
+    # the dominant kernel's bandwidth: algorithmic bytes over its average
+    # launch duration, from HIP events bracketing the K timed launches (the
+    # per-launch event pairs are reported too; their own packets add ~1 us)
+    achieved = ubytes * args.steps / ev
     traffic = load_traffic(args)
     result = {
         "metric": "res_calc edges/s, achieved HBM GB/s (% of peak); learned-tuned vs plain atomics",
@@ -388,6 +396,7 @@ def main():
                    "stage_inputs": not args.no_stage, "graph": args.graph,
                    "block_order": args.order, "cells_per_thread": args.ept,
                    "nodes": args.nodes, "point_terms": args.point_terms,
+                   "accumulate": args.accumulate,
                    "launches_per_step": stats["num_launches_per_iter"],
                    "block_colours": stats["num_colours"],
                    "avg_thread_colours": round(stats["avg_thread_colours"], 3),

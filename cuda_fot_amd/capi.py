@@ -50,6 +50,7 @@ LOOP_POINT_PARTITION = 8
 LOOP_TWO_PER_THREAD = 16
 LOOP_GATHER_NODES = 32
 LOOP_POINT_TERMS = 64
+LOOP_OWNER_GATHER = 128
 
 
 class FotError(RuntimeError):

@@ -449,7 +449,8 @@ constexpr uint32_t kOwnerPad = 4096;
 struct OwnerMeta {
   uint32_t cell_off, ncells, cache_off, ncached;
   uint32_t nowned, own_start, ncol, contiguous; // owned ids own_start + i when contiguous
-  uint32_t node_off, nnodes, pad0, pad1;        // the block's mapping-1 points (x)
+  uint32_t node_off, nnodes, pad0, pad1;        // the block's mapping-1 points (x);
+                                                // pad0: first owned-cell list entry
   uint32_t pad2, pad3, pad4, pad5;
 };
 
@@ -626,6 +627,153 @@ __global__ __launch_bounds__(1024) void res_calc_owner2_kernel(EngineArgs a,
   }
 }
 
+// Owner-compute res_calc with a gather accumulation: each edge's flux f is
+// computed, written to LDS (over the dead q / adt region), and each owned
+// cell adds the +-f of its incident edges in thread-colour order, starting
+// from its res held in registers.  Every owned cell receives at most one
+// increment per colour, so this is the colour-ordered sum of the other
+// owner kernels, bit for bit, with three barriers instead of one per
+// colour, no res staging in LDS and a smaller LDS footprint.  plist holds
+// per owned cell up to 8 entries (edge position | slot << 15), sorted by
+// colour, 0xffff-terminated.
+template <bool SOA, int EPT, int KS>
+__global__ __launch_bounds__(1024) void res_calc_owner3_kernel(EngineArgs a,
+                                                               const OwnerMeta *__restrict__ meta,
+                                                               const uint32_t *__restrict__ cached,
+                                                               const uint32_t *__restrict__ packed,
+                                                               const uint4 *__restrict__ plist,
+                                                               uint32_t nb) {
+  extern __shared__ double lds[];
+  const uint32_t b = xcd_block(blockIdx.x, nb);
+  const OwnerMeta mt = meta[b];
+  const uint32_t tid = threadIdx.x, bd = blockDim.x;
+  const uint32_t m = mt.ncached, nw = mt.nowned, ne = mt.ncells;
+  const bool contig = mt.contiguous != 0;
+  double *q_s = lds, *adt_s = q_s + 4 * m;
+  double *f_s = lds; // after the second barrier: 4 doubles per edge
+  const double *x = a.point_data[1], *q = a.point_data[2], *adt = a.point_data[3];
+  const uint32_t np = a.num_points[0];
+  const uint32_t *pcached = cached + mt.cache_off;
+  // plan arrays are padded: branch-free loads
+  uint32_t pk[EPT];
+  uint2 nn[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const uint32_t t = tid + e * bd;
+    const uint32_t pv = packed[mt.cell_off + t];
+    pk[e] = t < ne ? pv : 0xff000000u;
+    nn[e] = reinterpret_cast<const uint2 *>(a.maps[1])[mt.cell_off + t];
+  }
+  uint32_t gq[KS];
+  uint4 lst[KS];
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const uint32_t i = tid + k * bd;
+    const uint32_t pc = pcached[i];
+    gq[k] = (contig && i < nw) ? mt.own_start + i : (i < m ? pc : 0u);
+    lst[k] = plist[mt.pad0 + i];
+  }
+  double2 vq0[KS], vq1[KS], vr0[KS], vr1[KS];
+  double vadt[KS];
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const uint32_t g = gq[k];
+    if (SOA) {
+      const uint32_t nq = a.num_points[2];
+      vq0[k] = make_double2(q[g], q[size_t(nq) + g]);
+      vq1[k] = make_double2(q[2 * size_t(nq) + g], q[3 * size_t(nq) + g]);
+      vr0[k] = make_double2(a.out[g], a.out[size_t(np) + g]);
+      vr1[k] = make_double2(a.out[2 * size_t(np) + g], a.out[3 * size_t(np) + g]);
+    } else {
+      const double2 *q2p = reinterpret_cast<const double2 *>(q);
+      vq0[k] = q2p[2 * size_t(g)], vq1[k] = q2p[2 * size_t(g) + 1];
+      const double2 *o = reinterpret_cast<const double2 *>(a.out) + 2 * size_t(g);
+      vr0[k] = o[0], vr1[k] = o[1];
+    }
+    vadt[k] = adt[g];
+  }
+  double x1[EPT][2], x2[EPT][2];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) { // nn is a valid node pair for every thread (padded map)
+    if (SOA) {
+      const uint32_t nx = a.num_points[1];
+      x1[e][0] = x[nn[e].x], x1[e][1] = x[nx + nn[e].x];
+      x2[e][0] = x[nn[e].y], x2[e][1] = x[nx + nn[e].y];
+    } else {
+      const double2 xa = reinterpret_cast<const double2 *>(x)[nn[e].x];
+      const double2 xb = reinterpret_cast<const double2 *>(x)[nn[e].y];
+      x1[e][0] = xa.x, x1[e][1] = xa.y, x2[e][0] = xb.x, x2[e][1] = xb.y;
+    }
+  }
+  // one wait for all of them (see res_calc_owner2_kernel)
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    asm volatile("" ::"v"(vq0[k].x), "v"(vq0[k].y), "v"(vq1[k].x), "v"(vq1[k].y), "v"(vadt[k]));
+  }
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const uint32_t i = tid + k * bd;
+    if (i < m) {
+      q_s[0 * m + i] = vq0[k].x, q_s[1 * m + i] = vq0[k].y;
+      q_s[2 * m + i] = vq1[k].x, q_s[3 * m + i] = vq1[k].y;
+      adt_s[i] = vadt[k];
+    }
+  }
+  __syncthreads();
+  double fl[EPT][4];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    if (tid + e * bd < ne) {
+      const uint32_t l0 = pk[e] & 0xfffu, l1 = (pk[e] >> 12) & 0xfffu;
+      double q1[4], q2[4], r2[4];
+#pragma unroll
+      for (uint32_t d = 0; d < 4; ++d) q1[d] = q_s[d * m + l0], q2[d] = q_s[d * m + l1];
+      const double a1 = adt_s[l0], a2 = adt_s[l1];
+      res_calc_user(x1[e], x2[e], q1, q2, &a1, &a2, fl[e], r2, 1, 1);
+    }
+  }
+  __syncthreads(); // q_s / adt_s are dead: f_s takes their place
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const uint32_t t = tid + e * bd;
+    if (t < ne) {
+      double2 *fp = reinterpret_cast<double2 *>(f_s + 4 * size_t(t));
+      fp[0] = make_double2(fl[e][0], fl[e][1]);
+      fp[1] = make_double2(fl[e][2], fl[e][3]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const uint32_t i = tid + k * bd;
+    if (i < nw) {
+      double r[4] = {vr0[k].x, vr0[k].y, vr1[k].x, vr1[k].y};
+      const uint32_t w4[4] = {lst[k].x, lst[k].y, lst[k].z, lst[k].w};
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const uint32_t ent = (w4[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+        if (ent == 0xffffu) break;
+        const double2 *fp = reinterpret_cast<const double2 *>(f_s + 4 * size_t(ent & 0x7fffu));
+        const double2 f01 = fp[0], f23 = fp[1];
+        if (ent & 0x8000u) { // the edge's second cell: res2 = -f, and r + (-f) == r - f
+          r[0] -= f01.x, r[1] -= f01.y, r[2] -= f23.x, r[3] -= f23.y;
+        } else {
+          r[0] += f01.x, r[1] += f01.y, r[2] += f23.x, r[3] += f23.y;
+        }
+      }
+      const uint32_t g = gq[k];
+      if (SOA) {
+#pragma unroll
+        for (uint32_t d = 0; d < 4; ++d) a.out[size_t(d) * np + g] = r[d];
+      } else {
+        double2 *o = reinterpret_cast<double2 *>(a.out) + 2 * size_t(g);
+        o[0] = make_double2(r[0], r[1]);
+        o[1] = make_double2(r[2], r[3]);
+      }
+    }
+  }
+}
+
 // dst[d * n_dst + i] = src[d * n_src + idx[i]] (SOA gather of cell data)
 __global__ __launch_bounds__(256) void gather_soa_kernel(double *__restrict__ dst,
                                                          const double *__restrict__ src,
@@ -667,6 +815,8 @@ struct fot_loop {
   const uint32_t *owner_nodes = nullptr, *owner_nloc = nullptr;
   bool stage_x = false;
   bool point_terms = false;      // owner res_calc: per-cell 1/rho and pressure
+  bool owner_gather = false;     // owner res_calc: gather accumulation kernel
+  const uint4 *owner_plist = nullptr;
   uint32_t stage_per_thread = 1; // owner res_calc: staged points per thread
   uint32_t ept = 1;
   DepDev dep{};
@@ -932,10 +1082,48 @@ fot_status fot_loop::build_hierarchical() {
             break;
           }
       }
+      // owned-cell edge lists for the gather kernel: (position | slot << 15)
+      // sorted by the edge's thread colour, at most 8, 0xffff-terminated
+      owner_gather = (cfg.flags & FOT_LOOP_OWNER_GATHER) != 0;
+      if (owner_gather) {
+        uint32_t total_owned = 0;
+        for (const OwnerMeta &mt : meta) total_owned += mt.nowned;
+        std::vector<uint16_t> plist((size_t(total_owned) + kOwnerPad) * 8, 0xffffu);
+        std::vector<std::vector<std::pair<uint32_t, uint16_t>>> lists;
+        uint32_t own_off = 0;
+        for (size_t b = 0; b < meta.size() && owner_gather; ++b) {
+          OwnerMeta &mt = meta[b];
+          mt.pad0 = own_off;
+          lists.assign(mt.nowned, {});
+          for (uint32_t t = 0; t < mt.ncells; ++t) {
+            const uint32_t pv = packed[mt.cell_off + t];
+            const uint32_t l0 = pv & 0xfffu, l1 = (pv >> 12) & 0xfffu, col = pv >> 24;
+            if (l0 < mt.nowned) lists[l0].push_back({col, uint16_t(t)});
+            if (l1 < mt.nowned) lists[l1].push_back({col, uint16_t(t | 0x8000u)});
+          }
+          for (uint32_t i = 0; i < mt.nowned && owner_gather; ++i) {
+            if (lists[i].size() > 8 || mt.ncells > 0x8000u) {
+              owner_gather = false;
+              break;
+            }
+            std::sort(lists[i].begin(), lists[i].end());
+            for (size_t j = 0; j < lists[i].size(); ++j)
+              plist[(size_t(own_off) + i) * 8 + j] = lists[i][j].second;
+          }
+          own_off += mt.nowned;
+        }
+        if (owner_gather) {
+          void *pp = nullptr;
+          if ((st = alloc(plist.size() * sizeof(uint16_t), &pp)) != FOT_OK) return st;
+          FOT_HIP(hipMemcpy(pp, plist.data(), plist.size() * sizeof(uint16_t),
+                            hipMemcpyHostToDevice));
+          owner_plist = static_cast<const uint4 *>(pp);
+        }
+      }
       // each block's mapping-1 points (x) for staging: ascending unique node
       // ids and per-cell local pairs
       std::vector<uint32_t> nodes_all, nloc(plan_cell.size()), tmp;
-      stage_x = (cfg.flags & FOT_LOOP_GATHER_NODES) == 0;
+      stage_x = (cfg.flags & FOT_LOOP_GATHER_NODES) == 0 && !owner_gather;
       for (size_t b = 0; b < meta.size() && stage_x; ++b) {
         OwnerMeta &mt = meta[b];
         tmp.clear();
@@ -1027,7 +1215,16 @@ fot_status fot_loop::build_hierarchical() {
       lds_bytes = std::max(lds_bytes, bytes);
     }
     if (f.kernel == FOT_KERNEL_RES_CALC && stage_x) lds_bytes += 16u * max_nodes; // staged x
-    if (f.kernel == FOT_KERNEL_RES_CALC && point_terms) {
+    if (f.kernel == FOT_KERNEL_RES_CALC && owner_gather) {
+      // q / adt of the staged cells, later the edges' fluxes in the same place
+      lds_bytes = 0;
+      for (size_t b = 0; b + 1 < blk_cache_off.size(); ++b) {
+        const uint32_t m = blk_cache_off[b + 1] - blk_cache_off[b];
+        const uint32_t ne = blk_cell_off[b + 1] - blk_cell_off[b];
+        lds_bytes = std::max(lds_bytes, 8u * std::max(5u * m, 4u * ne));
+      }
+    }
+    if (f.kernel == FOT_KERNEL_RES_CALC && point_terms && !owner_gather) {
       uint32_t mm = 0; // + 1/rho and pressure of the largest block's cells
       for (size_t b = 0; b + 1 < blk_cache_off.size(); ++b)
         mm = std::max(mm, blk_cache_off[b + 1] - blk_cache_off[b]);
@@ -1117,7 +1314,27 @@ fot_status fot_loop::enqueue(hipStream_t s) {
   case 3: FOT_OWNER2(E_, X_, 3); break;                                                 \
   default: FOT_OWNER2(E_, X_, 4); break;                                                \
   }
-        if (ept == 2) {
+        if (owner_gather) {
+#define FOT_OWNER3(E_, K_)                                                              \
+  hipLaunchKernelGGL((res_calc_owner3_kernel<SOA, E_, K_>), dim3(nb), dim3(threads), lds_bytes, \
+                     s, plan_args, owner_meta, hier.cached, hier.packed, owner_plist, nb)
+          if (ept == 2) {
+            switch (stage_per_thread) {
+            case 1: FOT_OWNER3(2, 1); break;
+            case 2: FOT_OWNER3(2, 2); break;
+            case 3: FOT_OWNER3(2, 3); break;
+            default: FOT_OWNER3(2, 4); break;
+            }
+          } else {
+            switch (stage_per_thread) {
+            case 1: FOT_OWNER3(1, 1); break;
+            case 2: FOT_OWNER3(1, 2); break;
+            case 3: FOT_OWNER3(1, 3); break;
+            default: FOT_OWNER3(1, 4); break;
+            }
+          }
+#undef FOT_OWNER3
+        } else if (ept == 2) {
           if (stage_x) FOT_OWNER2_K(2, true)
           else FOT_OWNER2_K(2, false)
         } else {

@@ -473,6 +473,11 @@ typedef struct {
  * cell (LDS) instead of once per edge side; the same operations, the same
  * bits */
 #define FOT_LOOP_POINT_TERMS 64u
+/* owner compute, res_calc: each owned cell gathers its edges' fluxes in
+ * thread-colour order from LDS (three barriers, no per-colour barriers);
+ * the same sums in the same order, the same bits.  Falls back to the
+ * colour loop when a cell has more than 8 edges. */
+#define FOT_LOOP_OWNER_GATHER 128u
 
 typedef struct fot_loop fot_loop;
 /* Builds the plan (host), uploads it, permutes cell data. */

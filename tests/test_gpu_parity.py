@@ -62,6 +62,7 @@ OWN = capi.LOOP_OWNER_COMPUTE
 TWO = capi.LOOP_TWO_PER_THREAD
 GATHER = capi.LOOP_GATHER_NODES
 TERMS = capi.LOOP_POINT_TERMS
+OG = capi.LOOP_OWNER_GATHER
 
 
 def _owner_replay(p, kernel, block_size, partition=None, iters=ITERS, point_partition=None):
@@ -84,6 +85,11 @@ def _owner_replay(p, kernel, block_size, partition=None, iters=ITERS, point_part
                                                  (False, 192, OWN | TWO | GATHER | TERMS,
                                                   "points"),
                                                  (True, 160, OWN | TERMS, "points"),
+                                                 (False, 192, OWN | TWO | OG, "points"),
+                                                 (True, 224, OWN | TWO | OG, "points"),
+                                                 (False, 128, OWN | OG, "points"),
+                                                 (False, 256, OWN | TWO | OG, None),
+                                                 (True, 512, OWN | OG, "kway"),
                                                  (True, 128, OWN | GATHER, "points"),
                                                  (False, 256, OWN | TWO, None)])
 def test_res_calc_owner_compute_bitwise(gpu, soa, bs, flags, method):
