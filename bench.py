@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--partition", default="points",
                     choices=["none", "kway", "ours", "gps", "points"],
                     help="points: owner partition of mapping-0 points (owner compute)")
-    ap.add_argument("--points-per-block", type=int, default=224)
+    ap.add_argument("--points-per-block", type=int, default=240)
     ap.add_argument("--ept", type=int, default=2, choices=[1, 2],
                     help="owner compute: cells per thread")
     ap.add_argument("--no-stage", action="store_true")

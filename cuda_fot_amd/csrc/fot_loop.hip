@@ -634,14 +634,14 @@ __global__ __launch_bounds__(1024) void res_calc_owner2_kernel(EngineArgs a,
 // increment per colour, so this is the colour-ordered sum of the other
 // owner kernels, bit for bit, with three barriers instead of one per
 // colour, no res staging in LDS and a smaller LDS footprint.  plist holds
-// per owned cell up to 8 entries (edge position | slot << 15), sorted by
-// colour, 0xffff-terminated.
-template <bool SOA, int EPT, int KS>
+// per owned cell up to 2 * LW entries (edge position | slot << 15; LW 32-bit
+// words, 2 or 4), sorted by colour, 0xffff-terminated.
+template <bool SOA, int EPT, int KS, int LW>
 __global__ __launch_bounds__(1024) void res_calc_owner3_kernel(EngineArgs a,
                                                                const OwnerMeta *__restrict__ meta,
                                                                const uint32_t *__restrict__ cached,
                                                                const uint32_t *__restrict__ packed,
-                                                               const uint4 *__restrict__ plist,
+                                                               const uint32_t *__restrict__ plist,
                                                                uint32_t nb) {
   extern __shared__ double lds[];
   const uint32_t b = xcd_block(blockIdx.x, nb);
@@ -649,6 +649,8 @@ __global__ __launch_bounds__(1024) void res_calc_owner3_kernel(EngineArgs a,
   const uint32_t tid = threadIdx.x, bd = blockDim.x;
   const uint32_t m = mt.ncached, nw = mt.nowned, ne = mt.ncells;
   const bool contig = mt.contiguous != 0;
+  typedef typename std::conditional<LW == 4, uint4, uint2>::type list_t;
+  const list_t *lists = reinterpret_cast<const list_t *>(plist);
   double *q_s = lds, *adt_s = q_s + 4 * m;
   double *f_s = lds; // after the second barrier: 4 doubles per edge
   const double *x = a.point_data[1], *q = a.point_data[2], *adt = a.point_data[3];
@@ -665,13 +667,15 @@ __global__ __launch_bounds__(1024) void res_calc_owner3_kernel(EngineArgs a,
     nn[e] = reinterpret_cast<const uint2 *>(a.maps[1])[mt.cell_off + t];
   }
   uint32_t gq[KS];
-  uint4 lst[KS];
+  uint32_t lst[KS][4];
 #pragma unroll
   for (int k = 0; k < KS; ++k) {
     const uint32_t i = tid + k * bd;
     const uint32_t pc = pcached[i];
     gq[k] = (contig && i < nw) ? mt.own_start + i : (i < m ? pc : 0u);
-    lst[k] = plist[mt.pad0 + i];
+    const list_t l = lists[mt.pad0 + i];
+    lst[k][0] = l.x, lst[k][1] = l.y;
+    if constexpr (LW == 4) lst[k][2] = l.z, lst[k][3] = l.w;
   }
   double2 vq0[KS], vq1[KS], vr0[KS], vr1[KS];
   double vadt[KS];
@@ -748,10 +752,9 @@ __global__ __launch_bounds__(1024) void res_calc_owner3_kernel(EngineArgs a,
     const uint32_t i = tid + k * bd;
     if (i < nw) {
       double r[4] = {vr0[k].x, vr0[k].y, vr1[k].x, vr1[k].y};
-      const uint32_t w4[4] = {lst[k].x, lst[k].y, lst[k].z, lst[k].w};
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const uint32_t ent = (w4[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+      for (int j = 0; j < 2 * LW; ++j) {
+        const uint32_t ent = (lst[k][j >> 1] >> (16 * (j & 1))) & 0xffffu;
         if (ent == 0xffffu) break;
         const double2 *fp = reinterpret_cast<const double2 *>(f_s + 4 * size_t(ent & 0x7fffu));
         const double2 f01 = fp[0], f23 = fp[1];
@@ -816,7 +819,8 @@ struct fot_loop {
   bool stage_x = false;
   bool point_terms = false;      // owner res_calc: per-cell 1/rho and pressure
   bool owner_gather = false;     // owner res_calc: gather accumulation kernel
-  const uint4 *owner_plist = nullptr;
+  const uint32_t *owner_plist = nullptr;
+  uint32_t list_words = 4;       // owner gather: 32-bit words per owned-cell list
   uint32_t stage_per_thread = 1; // owner res_calc: staged points per thread
   uint32_t ept = 1;
   DepDev dep{};
@@ -1089,6 +1093,7 @@ fot_status fot_loop::build_hierarchical() {
         uint32_t total_owned = 0;
         for (const OwnerMeta &mt : meta) total_owned += mt.nowned;
         std::vector<uint16_t> plist((size_t(total_owned) + kOwnerPad) * 8, 0xffffu);
+        uint32_t longest = 0;
         std::vector<std::vector<std::pair<uint32_t, uint16_t>>> lists;
         uint32_t own_off = 0;
         for (size_t b = 0; b < meta.size() && owner_gather; ++b) {
@@ -1107,17 +1112,26 @@ fot_status fot_loop::build_hierarchical() {
               break;
             }
             std::sort(lists[i].begin(), lists[i].end());
+            longest = std::max(longest, uint32_t(lists[i].size()));
             for (size_t j = 0; j < lists[i].size(); ++j)
               plist[(size_t(own_off) + i) * 8 + j] = lists[i][j].second;
           }
           own_off += mt.nowned;
         }
         if (owner_gather) {
+          // 4 entries (8 bytes) per owned cell when every list fits, else 8
+          list_words = longest <= 4 ? 2 : 4;
+          if (list_words == 2) {
+            const size_t n = plist.size() / 8;
+            for (size_t i = 0; i < n; ++i)
+              for (int j = 0; j < 4; ++j) plist[i * 4 + j] = plist[i * 8 + j];
+            plist.resize(n * 4);
+          }
           void *pp = nullptr;
           if ((st = alloc(plist.size() * sizeof(uint16_t), &pp)) != FOT_OK) return st;
           FOT_HIP(hipMemcpy(pp, plist.data(), plist.size() * sizeof(uint16_t),
                             hipMemcpyHostToDevice));
-          owner_plist = static_cast<const uint4 *>(pp);
+          owner_plist = static_cast<const uint32_t *>(pp);
         }
       }
       // each block's mapping-1 points (x) for staging: ascending unique node
@@ -1315,9 +1329,15 @@ fot_status fot_loop::enqueue(hipStream_t s) {
   default: FOT_OWNER2(E_, X_, 4); break;                                                \
   }
         if (owner_gather) {
+#define FOT_OWNER3W(E_, K_, W_)                                                         \
+  hipLaunchKernelGGL((res_calc_owner3_kernel<SOA, E_, K_, W_>), dim3(nb), dim3(threads),     \
+                     lds_bytes, s, plan_args, owner_meta, hier.cached, hier.packed,          \
+                     owner_plist, nb)
 #define FOT_OWNER3(E_, K_)                                                              \
-  hipLaunchKernelGGL((res_calc_owner3_kernel<SOA, E_, K_>), dim3(nb), dim3(threads), lds_bytes, \
-                     s, plan_args, owner_meta, hier.cached, hier.packed, owner_plist, nb)
+  do {                                                                                  \
+    if (list_words == 2) FOT_OWNER3W(E_, K_, 2);                                        \
+    else FOT_OWNER3W(E_, K_, 4);                                                        \
+  } while (0)
           if (ept == 2) {
             switch (stage_per_thread) {
             case 1: FOT_OWNER3(2, 1); break;
@@ -1334,6 +1354,7 @@ fot_status fot_loop::enqueue(hipStream_t s) {
             }
           }
 #undef FOT_OWNER3
+#undef FOT_OWNER3W
         } else if (ept == 2) {
           if (stage_x) FOT_OWNER2_K(2, true)
           else FOT_OWNER2_K(2, false)

@@ -226,3 +226,25 @@ def test_empty_and_single_cell(gpu):
         p = airfoil_problem(nx=2, ny=2, res0_seed=9)
         p.gpu_loop(RES_CALC, strategy, ITERS)
         assert_close_scaled(p.point_data[0], ref, S, what=f"tiny mesh strategy {strategy}")
+
+
+@pytest.mark.parametrize("copies", [2, 3])
+def test_res_calc_owner_gather_wide_lists(gpu, copies):
+    # repeated edges raise cell degrees to 8 (8-entry lists) or 12 (more than
+    # the gather kernel's lists hold: the engine falls back to the colour loop)
+    from cuda_fot_amd import meshgen, res_calc_problem
+    m = meshgen.airfoil(40, 24, seed=6)
+    rep = {k: np.concatenate([m[k]] * copies) if k in ("pedge", "pecell") else v
+           for k, v in m.items()}
+    def make():
+        p = res_calc_problem(rep, block_size=128,
+                             res0=meshgen.uniform(m["num_cells"] * 4, 13, -1, 1).reshape(-1, 4))
+        p.partition_points(128)
+        return p
+    p = make()
+    ref = _owner_replay(make(), O.RES_CALC, 128, point_partition=p.point_partition)
+    serial, S = _serial(make(), O.RES_CALC)
+    p.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, ITERS, flags=OWN | TWO | OG)
+    assert_bitwise(p.point_data[0], ref, f"owner gather, {copies} copies of each edge")
+    # the serial reference ran on the same (renumbered) problem
+    assert_close_scaled(p.point_data[0], serial, S, what="owner gather vs serial")
