@@ -55,6 +55,8 @@ def parse():
                     choices=["none", "kway", "ours", "gps", "points"],
                     help="points: owner partition of mapping-0 points (owner compute)")
     ap.add_argument("--points-per-block", type=int, default=240)
+    ap.add_argument("--tune", default="none", choices=["none", "exhaustive", "learned"],
+                    help="volna-batch: per-mesh partition size from the autotuner")
     ap.add_argument("--ept", type=int, default=2, choices=[1, 2],
                     help="owner compute: cells per thread")
     ap.add_argument("--no-stage", action="store_true")
@@ -229,6 +231,37 @@ def bench_volna_batch(args):
         probs.append(volna_problem(meshgen.tri_mesh(nx, ny, seed=100 + i),
                                    block_size=args.points_per_block))
     f = capi.functor(capi.KERNEL_VOLNA)
+    tune = {"mode": args.tune}
+    if args.tune != "none":
+        # the autotuner picks each mesh's partition size among the owner
+        # candidates (parts of 128 / 256 / 512 points)
+        from cuda_fot_amd import tuner
+        owner = [3, 4, 5]
+        sizes = {3: 128, 4: 256, 5: 512}
+        measured = 0
+        model = None
+        if args.tune == "learned":
+            rows, X, T = tuner.load_corpus(tuner.CORPUS_PATH)
+            model = tuner.CostModel(seed=0)
+            model.fit(X, T)
+        picks = []
+        for p in probs:
+            if args.tune == "exhaustive":
+                t = tuner.measure(p, f, which=owner, warmup=2, steps=5)
+                measured += len(owner)
+            else:
+                pred = model.predict(tuner.features(p, f))[0]
+                pick = owner[int(np.argmin(pred[owner]))]
+                t = tuner.measure(p, f, which=[pick], warmup=2, steps=5)
+                measured += 1
+                if not (t[pick] > 0) or t[pick] > 1.05 * pred[pick]:
+                    t = tuner.measure(p, f, which=owner, warmup=2, steps=5)
+                    measured += len(owner)
+            best = owner[int(np.argmin(np.where(t[owner] > 0, t[owner], np.inf)))]
+            picks.append(sizes[best])
+            p.block_size = sizes[best]
+        tune.update(measured_candidates=measured,
+                    picks={str(k): picks.count(k) for k in sorted(set(picks))})
     specs, keep = [], []
     for p in probs:
         dev = [DeviceArray.from_host(d) for d in p.point_data]
@@ -286,6 +319,7 @@ def bench_volna_batch(args):
                  graph_edges_per_s=edges * args.steps / gwall,
                  graph_ms_per_step=gwall / args.steps * 1e3,
                  per_mesh_launch_edges_per_s=edges * args.steps / seq,
+                 tuning=tune,
                  batch={"grid": st["grid"], "work_items": st["num_work_items"],
                         "threads": st["threads"], "lds_bytes": st["lds_bytes"]})
 

@@ -35,6 +35,9 @@ from .device import DeviceArray, make_desc
 from .mlp import MLP
 
 NAMES = ["atomic", "global", "hier256", "owner128", "owner256", "owner512"]
+# the autotuner's own timings, the cost model's training corpus
+CORPUS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                           "tuner_corpus.jsonl")
 NUM_FEATURES = 24
 
 
@@ -206,7 +209,7 @@ def load_corpus(path):
 
 
 # ---- corpus of autotuner timings ---------------------------------------------
-def corpus_meshes(n: int, seed: int = 0, min_edges: float = 5e4, max_edges: float = 1.2e6):
+def corpus_meshes(n: int, seed: int = 0, min_edges: float = 5e4, max_edges: float = 2e6):
     """(family, ordering, size) specs: 2D quad (mine), airfoil quads
     (res_calc), triangles (volna), 3D hexes (scatter), each in random, RCM
     and point-partitioned orderings, edges log-uniform in [min, max]."""
