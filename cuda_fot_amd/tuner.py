@@ -251,6 +251,11 @@ def build_corpus_problem(spec):
         ny = max(8, int(e / 3 / nx))
         p, func = volna_problem(meshgen.tri_mesh(nx, ny, seed=s)), \
             capi.functor(capi.KERNEL_VOLNA)
+    elif fam == "mini_aero":
+        from .problem import mini_aero_problem
+        n = max(4, int(round((e / 3) ** (1 / 3))))   # ~3 n^3 interior faces
+        p, func = mini_aero_problem(meshgen.hex_faces(n, n, n, seed=s)), \
+            capi.functor(capi.KERNEL_MINI_AERO)
     else:
         m = max(4, int(round(e ** (1 / 3))))
         cells = meshgen.hex_grid(m + 1, m + 1, m + 1)
@@ -265,6 +270,32 @@ def build_corpus_problem(spec):
         p.partition_points(256)
         p.point_partition = None  # the ordering stays; candidates plan their own
     return p, func
+
+
+def heldout_specs():
+    """The held-out evaluation set: mini_aero-style hex face meshes (a kernel
+    the corpus never holds) of 48^3 to 96^3 cells in the three orderings."""
+    out = []
+    for i, n in enumerate([48, 64, 80, 96]):
+        for j, order in enumerate(["random", "rcm", "partitioned"]):
+            out.append({"id": 1000 + 3 * i + j, "family": "mini_aero", "ordering": order,
+                        "edges": 3 * n ** 3, "seed": 7 + i})
+    return out
+
+
+def measure_specs(specs, out):
+    """Measures every candidate on each spec, one JSON line per mesh."""
+    for spec in specs:
+        t0 = time.perf_counter()
+        p, func = build_corpus_problem(spec)
+        x = features(p, func)
+        t = measure(p, func)
+        rec = dict(spec, cells=int(p.num_cells), features=[float(v) for v in x],
+                   ms=[float(v) for v in t], seconds=round(time.perf_counter() - t0, 2))
+        with open(out, "a") as f:
+            f.write(json.dumps(rec) + "\n")
+        print(f"mesh {spec['id']} {spec['family']}/{spec['ordering']} cells {p.num_cells}: "
+              + " ".join(f"{v:.3f}" for v in t) + f" ({rec['seconds']} s)", flush=True)
 
 
 def corpus_main(argv=None):
@@ -302,6 +333,16 @@ def eval_main(argv=None):
     rows, X, T = load_corpus(argv[1])
     held = argv[2] if len(argv) > 2 else "-"
     seed = int(argv[3]) if len(argv) > 3 else 0
+    if os.path.exists(held):
+        # a separate held-out file: train on all of the corpus
+        _, Xh, Th = load_corpus(held)
+        model = CostModel(seed=seed)
+        losses = model.fit(X, T)
+        out = evaluate(model, Xh[np.all(np.isfinite(Th), 1)], Th[np.all(np.isfinite(Th), 1)])
+        out.update(train_rows=len(rows), held_out=os.path.basename(held),
+                   final_train_loss=float(losses[-1]), first_train_loss=float(losses[0]))
+        print(json.dumps(out))
+        return out
     if held == "-":
         test = np.random.default_rng(seed).random(len(rows)) < 0.2
     else:
@@ -321,3 +362,6 @@ if __name__ == "__main__":
         corpus_main()
     elif len(sys.argv) > 1 and sys.argv[1] == "eval":
         eval_main()
+    elif len(sys.argv) > 1 and sys.argv[1] == "heldout":
+        # python -m cuda_fot_amd.tuner heldout OUT.jsonl: measure the held-out set
+        measure_specs(heldout_specs(), sys.argv[2])
