@@ -38,6 +38,9 @@ NAMES = ["atomic", "global", "hier256", "owner128", "owner256", "owner512"]
 # the autotuner's own timings, the cost model's training corpus
 CORPUS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
                            "tuner_corpus.jsonl")
+# held-out evaluation set: mini_aero 48^3 .. 96^3 (heldout_specs), measured
+HELDOUT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                            "tuner_heldout_mini_aero.jsonl")
 NUM_FEATURES = 24
 
 
@@ -113,7 +116,13 @@ class CostModel:
 
     def fit(self, X, T, epochs: int = 3000, lr: float = 3e-3, weight_decay: float = 1e-4):
         X = np.asarray(X, np.float64)
-        T = np.asarray(T, np.float64)
+        T = np.asarray(T, np.float64).copy()
+        # a candidate that cannot be planned (-1) trains as 10x the mesh's
+        # slowest measured time, so the model learns to avoid it
+        for i in range(len(T)):
+            ok = np.isfinite(T[i]) & (T[i] > 0)
+            if ok.any():
+                T[i, np.isfinite(T[i]) & (T[i] < 0)] = 10.0 * T[i, ok].max()
         mask = np.isfinite(T) & (T > 0)
         logT = np.where(mask, np.log(np.where(mask, T, 1.0)), 0.0)
         self.fmu, self.fsd = X.mean(0), X.std(0) + 1e-6
@@ -272,6 +281,20 @@ def build_corpus_problem(spec):
     return p, func
 
 
+def corpus_extra_specs(n: int = 60, seed: int = 1):
+    """mini_aero-style hex face meshes of 16^3 to 44^3 cells in the three
+    orderings, appended to the corpus (ids 500 on); the evaluation set
+    (heldout_specs, 48^3 to 96^3) stays out of it."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        c = int(rng.integers(16, 45))
+        out.append({"id": 500 + i, "family": "mini_aero", "ordering":
+                    ["random", "rcm", "partitioned"][i % 3], "edges": 3 * c ** 3,
+                    "seed": int(rng.integers(1 << 30))})
+    return out
+
+
 def heldout_specs():
     """The held-out evaluation set: mini_aero-style hex face meshes (a kernel
     the corpus never holds) of 48^3 to 96^3 cells in the three orderings."""
@@ -362,6 +385,9 @@ if __name__ == "__main__":
         corpus_main()
     elif len(sys.argv) > 1 and sys.argv[1] == "eval":
         eval_main()
+    elif len(sys.argv) > 1 and sys.argv[1] == "corpus-extra":
+        # python -m cuda_fot_amd.tuner corpus-extra OUT.jsonl
+        measure_specs(corpus_extra_specs(), sys.argv[2])
     elif len(sys.argv) > 1 and sys.argv[1] == "heldout":
         # python -m cuda_fot_amd.tuner heldout OUT.jsonl: measure the held-out set
         measure_specs(heldout_specs(), sys.argv[2])

@@ -131,3 +131,24 @@ def test_learned_tuner_gains_over_the_method_without_it(gpu):
     assert r["final_mean_slowdown"] <= 1.05, r
     assert r["measured_per_mesh"] < r["exhaustive_measured_per_mesh"], r
     assert r["final_mean_slowdown"] <= r["default_mean_slowdown"], r
+
+
+@pytest.mark.gpu
+def test_learned_tuner_on_held_out_mini_aero(gpu):
+    """Trained on the whole corpus (mini_aero up to 44^3 only), evaluated on
+    mini_aero 48^3 .. 96^3 in three orderings: with the fallback the choice
+    is within 5 % of the exhaustive search, with fewer measurements than it,
+    and better than the fixed choice of the method without the model."""
+    import os
+    if not (os.path.exists(CORPUS) and os.path.exists(tuner.HELDOUT_PATH)):
+        pytest.skip("no autotuner corpus committed yet")
+    _, X, T = tuner.load_corpus(CORPUS)
+    _, Xh, Th = tuner.load_corpus(tuner.HELDOUT_PATH)
+    assert all(r["family"] == "mini_aero" and r["cells"] >= 3 * 47 * 48 * 48
+               for r in tuner.load_corpus(tuner.HELDOUT_PATH)[0])
+    model = tuner.CostModel(seed=0)
+    model.fit(X, T)
+    r = tuner.evaluate(model, Xh, Th)
+    assert r["final_mean_slowdown"] <= 1.05, r
+    assert r["measured_per_mesh"] < r["exhaustive_measured_per_mesh"], r
+    assert r["final_mean_slowdown"] < r["default_mean_slowdown"], r
