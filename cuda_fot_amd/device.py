@@ -120,7 +120,9 @@ def make_desc(maps, num_points, point_data, out, cell_data=(), soa=False):
         d.cell_dim[k] = cd.shape[0] if len(cd.shape) > 1 else 1
     d.point_data_out = out.ptr
     d.soa = 1 if soa else 0
-    return d, maps
+    # the device arrays too: the desc holds their raw addresses, and a
+    # DeviceArray frees its memory when the last reference goes
+    return d, (maps, list(point_data), out, tuple(cell_data))
 
 
 class Batch:
@@ -187,6 +189,10 @@ class Loop:
                  block_size=256, partition=None, use_graph=False, flags=0):
         self._maps = [capi.u32(m) for m in maps]
         self._partition = None if partition is None else capi.u32(partition)
+        # the loop launches kernels on these buffers' raw addresses: hold them
+        # for as long as the loop exists, so a caller dropping its own
+        # references cannot free memory the loop still reads and writes
+        self._buffers = (list(point_data), out, tuple(cell_data))
         d = capi.ProblemDesc()
         d.num_cells = self._maps[0].shape[0]
         d.num_mappings = len(self._maps)

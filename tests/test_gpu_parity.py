@@ -248,3 +248,32 @@ def test_res_calc_owner_gather_wide_lists(gpu, copies):
     assert_bitwise(p.point_data[0], ref, f"owner gather, {copies} copies of each edge")
     # the serial reference ran on the same (renumbered) problem
     assert_close_scaled(p.point_data[0], serial, S, what="owner gather vs serial")
+
+
+def test_loop_keeps_its_device_buffers_alive(gpu):
+    """A Loop launches kernels on raw device addresses: it must hold the
+    DeviceArrays it was built on, so a caller that drops its own references
+    (as `loop, _, _ = ...` does) cannot free memory the kernels still read."""
+    import gc
+    import weakref
+    from cuda_fot_amd import DeviceArray, Loop
+    p = airfoil_problem(res0_seed=5)
+    ref, S = _serial(p, O.RES_CALC, iters=1)
+
+    def build():
+        dev = [DeviceArray.from_host(d) for d in p.point_data]
+        out = DeviceArray.from_host(p.point_data[0])
+        loop = Loop(RES_CALC, p.maps, p.num_points, dev, out,
+                    strategy=capi.STRATEGY_ATOMIC, block_size=256)
+        return loop, [weakref.ref(d) for d in dev], weakref.ref(out)
+
+    loop, dev_refs, out_ref = build()
+    gc.collect()
+    assert all(r() is not None for r in dev_refs) and out_ref() is not None
+    loop.run(1)   # out = res; one step; res = out (synchronised)
+    got = dev_refs[0]().download()
+    assert_close_scaled(got, ref, S, what="atomic over loop-held buffers")
+    loop.close()
+    del loop
+    gc.collect()
+    assert all(r() is None for r in dev_refs) and out_ref() is None
