@@ -159,6 +159,25 @@ def launch_durations(loop, steps):
     return float(ms.mean()), float(np.median(ms))
 
 
+def copy_bandwidth(nbytes=256 << 20, reps=10):
+    """Bytes/s a plain device copy (fot_copy, uint4 loads and stores) moves,
+    read + write, over a buffer larger than the 256 MB Infinity Cache: what
+    DRAM delivers to a streaming kernel on this GPU, for context beside the
+    nominal peak."""
+    from cuda_fot_amd import DeviceArray, Timer, capi, synchronize
+    a = DeviceArray((nbytes // 8,), np.float64)
+    b = DeviceArray((nbytes // 8,), np.float64)
+    capi.call("fot_copy", b.ptr, a.ptr, nbytes, None)
+    synchronize()
+    t = Timer()
+    t.start()
+    for _ in range(reps):
+        capi.call("fot_copy", b.ptr, a.ptr, nbytes, None)
+    t.stop()
+    synchronize()
+    return 2.0 * nbytes * reps / (t.elapsed_ms() / 1e3)
+
+
 def load_traffic(args):
     """HBM bytes per launch from the committed rocprofv3 counter summary
     (FETCH_SIZE and WRITE_SIZE, scaled by a copy-kernel calibration) when it
@@ -436,6 +455,8 @@ def main():
                    "avg_thread_colours": round(stats["avg_thread_colours"], 3),
                    "reuse_factor": round(stats["reuse_factor"], 4),
                    "redundancy": round(stats["redundancy"], 4),
+                   "threads_per_block": stats["threads_per_block"],
+                   "stage_per_thread": stats["stage_per_thread"],
                    "plan_seconds": round(plan_s, 2)},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,
@@ -444,6 +465,7 @@ def main():
                      "event_ms_per_step": ev / args.steps * 1e3},
     }
     loop.close()
+    result["roofline"]["copy_gbps"] = copy_bandwidth() / 1e9
     if args.strategy != "atomic" and not args.no_atomic_ref:
         # the plain atomics loop on the same (reordered) mesh, and on the
         # shuffled mesh as generated

@@ -104,7 +104,8 @@ class LoopStats(C.Structure):
                 ("lds_bytes", C.c_uint32), ("max_thread_colours", C.c_uint32),
                 ("plan_device_bytes", C.c_uint64),
                 ("avg_thread_colours", C.c_double), ("reuse_factor", C.c_double),
-                ("redundancy", C.c_double), ("avg_cache_lines", C.c_double)]
+                ("redundancy", C.c_double), ("avg_cache_lines", C.c_double),
+                ("threads_per_block", C.c_uint32), ("stage_per_thread", C.c_uint32)]
 
 
 class BatchStats(C.Structure):

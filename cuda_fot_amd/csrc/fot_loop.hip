@@ -511,22 +511,25 @@ __global__ __launch_bounds__(1024) void res_calc_owner2_kernel(EngineArgs a,
       gx[k] = i < nx_b ? pn : 0u;
     }
   }
-  // every id is a valid point (point 0 past the block's end): unconditional
+  // every id is a valid point (point 0 past the block's end): unconditional.
+  // res is read for owned slots only; the others re-read the first owned
+  // cell (a line the block reads anyway) instead of a halo cell's res
   double2 vq0[KS], vq1[KS], vr0[KS], vr1[KS], vx[KS];
   double vadt[KS];
 #pragma unroll
   for (int k = 0; k < KS; ++k) {
     const uint32_t g = gq[k];
+    const uint32_t gr = (tid + k * bd < nw) ? g : mt.own_start;
     if (SOA) {
       const uint32_t nq = a.num_points[2];
       vq0[k] = make_double2(q[g], q[size_t(nq) + g]);
       vq1[k] = make_double2(q[2 * size_t(nq) + g], q[3 * size_t(nq) + g]);
-      vr0[k] = make_double2(a.out[g], a.out[size_t(np) + g]);
-      vr1[k] = make_double2(a.out[2 * size_t(np) + g], a.out[3 * size_t(np) + g]);
+      vr0[k] = make_double2(a.out[gr], a.out[size_t(np) + gr]);
+      vr1[k] = make_double2(a.out[2 * size_t(np) + gr], a.out[3 * size_t(np) + gr]);
     } else {
       const double2 *q2p = reinterpret_cast<const double2 *>(q);
       vq0[k] = q2p[2 * size_t(g)], vq1[k] = q2p[2 * size_t(g) + 1];
-      const double2 *o = reinterpret_cast<const double2 *>(a.out) + 2 * size_t(g);
+      const double2 *o = reinterpret_cast<const double2 *>(a.out) + 2 * size_t(gr);
       vr0[k] = o[0], vr1[k] = o[1];
     }
     vadt[k] = adt[g];
@@ -635,8 +638,11 @@ __global__ __launch_bounds__(1024) void res_calc_owner2_kernel(EngineArgs a,
 // owner kernels, bit for bit, with three barriers instead of one per
 // colour, no res staging in LDS and a smaller LDS footprint.  plist holds
 // per owned cell up to 2 * LW entries (edge position | slot << 15; LW 32-bit
-// words, 2 or 4), sorted by colour, 0xffff-terminated.
-template <bool SOA, int EPT, int KS, int LW>
+// words, 2 or 4), sorted by colour, 0xffff-terminated.  Owned cells occupy
+// the first KO staging rounds (KO = 1 when every block owns at most
+// blockDim.x cells): later rounds stage halo q / adt only, with no res or
+// list registers.
+template <bool SOA, int EPT, int KS, int LW, int KO>
 __global__ __launch_bounds__(1024) void res_calc_owner3_kernel(EngineArgs a,
                                                                const OwnerMeta *__restrict__ meta,
                                                                const uint32_t *__restrict__ cached,
@@ -667,32 +673,41 @@ __global__ __launch_bounds__(1024) void res_calc_owner3_kernel(EngineArgs a,
     nn[e] = reinterpret_cast<const uint2 *>(a.maps[1])[mt.cell_off + t];
   }
   uint32_t gq[KS];
-  uint32_t lst[KS][4];
+  uint32_t lst[KO][4];
 #pragma unroll
   for (int k = 0; k < KS; ++k) {
     const uint32_t i = tid + k * bd;
     const uint32_t pc = pcached[i];
     gq[k] = (contig && i < nw) ? mt.own_start + i : (i < m ? pc : 0u);
-    const list_t l = lists[mt.pad0 + i];
-    lst[k][0] = l.x, lst[k][1] = l.y;
-    if constexpr (LW == 4) lst[k][2] = l.z, lst[k][3] = l.w;
+    if (k < KO) {
+      const list_t l = lists[mt.pad0 + i];
+      lst[k][0] = l.x, lst[k][1] = l.y;
+      if constexpr (LW == 4) lst[k][2] = l.z, lst[k][3] = l.w;
+    }
   }
-  double2 vq0[KS], vq1[KS], vr0[KS], vr1[KS];
+  // res of owned slots only; the others re-read the first owned cell (a
+  // line the block reads anyway) instead of a halo cell's res
+  double2 vq0[KS], vq1[KS], vr0[KO], vr1[KO];
   double vadt[KS];
 #pragma unroll
   for (int k = 0; k < KS; ++k) {
     const uint32_t g = gq[k];
+    const uint32_t gr = (tid + k * bd < nw) ? g : mt.own_start;
     if (SOA) {
       const uint32_t nq = a.num_points[2];
       vq0[k] = make_double2(q[g], q[size_t(nq) + g]);
       vq1[k] = make_double2(q[2 * size_t(nq) + g], q[3 * size_t(nq) + g]);
-      vr0[k] = make_double2(a.out[g], a.out[size_t(np) + g]);
-      vr1[k] = make_double2(a.out[2 * size_t(np) + g], a.out[3 * size_t(np) + g]);
+      if (k < KO) {
+        vr0[k] = make_double2(a.out[gr], a.out[size_t(np) + gr]);
+        vr1[k] = make_double2(a.out[2 * size_t(np) + gr], a.out[3 * size_t(np) + gr]);
+      }
     } else {
       const double2 *q2p = reinterpret_cast<const double2 *>(q);
       vq0[k] = q2p[2 * size_t(g)], vq1[k] = q2p[2 * size_t(g) + 1];
-      const double2 *o = reinterpret_cast<const double2 *>(a.out) + 2 * size_t(g);
-      vr0[k] = o[0], vr1[k] = o[1];
+      if (k < KO) {
+        const double2 *o = reinterpret_cast<const double2 *>(a.out) + 2 * size_t(gr);
+        vr0[k] = o[0], vr1[k] = o[1];
+      }
     }
     vadt[k] = adt[g];
   }
@@ -748,7 +763,7 @@ __global__ __launch_bounds__(1024) void res_calc_owner3_kernel(EngineArgs a,
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < KS; ++k) {
+  for (int k = 0; k < KO; ++k) {
     const uint32_t i = tid + k * bd;
     if (i < nw) {
       double r[4] = {vr0[k].x, vr0[k].y, vr1[k].x, vr1[k].y};
@@ -822,6 +837,7 @@ struct fot_loop {
   const uint32_t *owner_plist = nullptr;
   uint32_t list_words = 4;       // owner gather: 32-bit words per owned-cell list
   uint32_t stage_per_thread = 1; // owner res_calc: staged points per thread
+  bool owned_first_round = false; // owner res_calc: every block owns <= threads cells
   uint32_t ept = 1;
   DepDev dep{};
   // graph of one iteration
@@ -1258,6 +1274,9 @@ fot_status fot_loop::build_hierarchical() {
     uint32_t most = max_nodes;
     for (size_t b = 0; b + 1 < blk_cache_off.size(); ++b)
       most = std::max(most, blk_cache_off[b + 1] - blk_cache_off[b]);
+    // (widening the block so that one round covers every staged point frees
+    // registers, 8 waves per SIMD instead of 6, but measured slower: 22.0 vs
+    // 21.3 us on the 1200x600 airfoil, 104.7 vs 96.9 us on 2400x1200)
     stage_per_thread = (most + threads - 1) / threads;
     if (stage_per_thread > 4) {
       // widen the block rather than stage more than 4 points per thread
@@ -1268,12 +1287,17 @@ fot_status fot_loop::build_hierarchical() {
                     4u * 1024u);
     }
     stage_per_thread = std::max<uint32_t>(1, stage_per_thread);
+    uint32_t most_owned = 0;
+    for (uint32_t nw : blk_owned) most_owned = std::max(most_owned, nw);
+    owned_first_round = most_owned <= threads;
   }
   stats.num_colours = num_plan_colours;
   stats.num_blocks = nb_total;
   stats.num_launches_per_iter = (dep_order || owner) ? 1u : stats.num_colours;
   stats.max_cached = max_cached;
   stats.lds_bytes = lds_bytes;
+  stats.threads_per_block = threads;
+  stats.stage_per_thread = stage_per_thread;
   stats.max_thread_colours = max_tc;
   stats.avg_thread_colours = nb_total ? tc_sum / nb_total : 0.0;
   // (cached may carry padding at its end: count the plan's entries only)
@@ -1329,10 +1353,15 @@ fot_status fot_loop::enqueue(hipStream_t s) {
   default: FOT_OWNER2(E_, X_, 4); break;                                                \
   }
         if (owner_gather) {
-#define FOT_OWNER3W(E_, K_, W_)                                                         \
-  hipLaunchKernelGGL((res_calc_owner3_kernel<SOA, E_, K_, W_>), dim3(nb), dim3(threads),     \
+#define FOT_OWNER3WO(E_, K_, W_, O_)                                                    \
+  hipLaunchKernelGGL((res_calc_owner3_kernel<SOA, E_, K_, W_, O_>), dim3(nb), dim3(threads), \
                      lds_bytes, s, plan_args, owner_meta, hier.cached, hier.packed,          \
                      owner_plist, nb)
+#define FOT_OWNER3W(E_, K_, W_)                                                         \
+  do {                                                                                  \
+    if (owned_first_round) FOT_OWNER3WO(E_, K_, W_, 1);                                 \
+    else FOT_OWNER3WO(E_, K_, W_, K_);                                                  \
+  } while (0)
 #define FOT_OWNER3(E_, K_)                                                              \
   do {                                                                                  \
     if (list_words == 2) FOT_OWNER3W(E_, K_, 2);                                        \
@@ -1355,6 +1384,7 @@ fot_status fot_loop::enqueue(hipStream_t s) {
           }
 #undef FOT_OWNER3
 #undef FOT_OWNER3W
+#undef FOT_OWNER3WO
         } else if (ept == 2) {
           if (stage_x) FOT_OWNER2_K(2, true)
           else FOT_OWNER2_K(2, false)

@@ -502,6 +502,10 @@ typedef struct {
   /* hierarchical: 128-byte lines of the target data per block, the
    * reference's "average cache_line / block" (problem.hpp:321-334) */
   double avg_cache_lines;
+  /* hierarchical launches: workgroup size, and (owner res_calc) the staging
+   * rounds each thread runs over the block's staged points */
+  uint32_t threads_per_block;
+  uint32_t stage_per_thread;
 } fot_loop_stats;
 fot_status fot_loop_stats_get(const fot_loop *loop, fot_loop_stats *stats);
 

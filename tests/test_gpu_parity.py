@@ -277,3 +277,29 @@ def test_loop_keeps_its_device_buffers_alive(gpu):
     del loop
     gc.collect()
     assert all(r() is None for r in dev_refs) and out_ref() is None
+
+
+@pytest.mark.parametrize("soa,flags", [(False, OWN | TWO | OG), (True, OWN | TWO | OG),
+                                       (False, OWN | TWO)])
+def test_res_calc_owner_blocks_owning_more_cells_than_threads(gpu, soa, flags):
+    # every third edge of the airfoil and blocks of 256 consecutive cells: a
+    # block runs ~170 edges (96 threads at two edges each, rounded to 192)
+    # but owns 256 cells, so owned cells span several staging rounds
+    from cuda_fot_amd import meshgen, res_calc_problem
+    m = meshgen.airfoil(80, 48, seed=7)
+    keep = np.arange(m["pedge"].shape[0]) % 3 == 0
+    sub = dict(m, pedge=m["pedge"][keep], pecell=m["pecell"][keep], num_edges=int(keep.sum()))
+
+    def make():
+        p = res_calc_problem(sub, block_size=128, soa=soa,
+                             res0=meshgen.uniform(m["num_cells"] * 4, 17, -1, 1).reshape(-1, 4))
+        p.point_partition = (np.arange(p.num_points[0]) // 256).astype(np.uint32)
+        return p
+    p = make()
+    ref = _owner_replay(make(), O.RES_CALC, 128, point_partition=p.point_partition)
+    serial, S = _serial(make(), O.RES_CALC)
+    stats = p.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, ITERS, flags=flags)
+    assert stats["stage_per_thread"] > 1
+    assert stats["threads_per_block"] < 256, stats
+    assert_bitwise(p.point_data[0], ref, f"owner, owned cells over several rounds ({flags})")
+    assert_close_scaled(p.point_data[0], serial, S, what="owner vs serial")
