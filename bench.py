@@ -63,6 +63,9 @@ def parse():
     ap.add_argument("--accumulate", default="gather", choices=["gather", "colours"],
                     help="owner compute: owned cells gather their edges' fluxes from LDS "
                          "(three barriers), or add them colour by colour (a barrier each)")
+    ap.add_argument("--persistent", action="store_true",
+                    help="owner compute, gather: persistent workgroups that load the next "
+                         "block's plan entries while computing the current one")
     ap.add_argument("--point-terms", action="store_true",
                     help="owner compute: 1/rho and pressure once per staged cell")
     ap.add_argument("--nodes", default="gather", choices=["gather", "stage"],
@@ -100,7 +103,7 @@ def build_problem(args, partition_kind):
 
 
 def make_loop(p, strategy, block_size, stage, graph, order="launch", ept=1, gather=False,
-              point_terms=False, owner_gather=False):
+              point_terms=False, owner_gather=False, persistent=False):
     from cuda_fot_amd import DeviceArray, Loop, capi
     strat = {"atomic": capi.STRATEGY_ATOMIC, "global": capi.STRATEGY_GLOBAL_COLOUR,
              "hierarchical": capi.STRATEGY_HIERARCHICAL}[strategy]
@@ -116,6 +119,8 @@ def make_loop(p, strategy, block_size, stage, graph, order="launch", ept=1, gath
         flags |= capi.LOOP_POINT_TERMS
     if owner_gather:
         flags |= capi.LOOP_OWNER_GATHER
+    if persistent:
+        flags |= capi.LOOP_PERSISTENT
     part = p.partition_vector if strat == capi.STRATEGY_HIERARCHICAL else None
     if strat == capi.STRATEGY_HIERARCHICAL and order == "owner" and p.point_partition is not None:
         part = p.point_partition
@@ -182,7 +187,7 @@ def load_traffic(args):
     """HBM bytes per launch from the committed rocprofv3 counter summary
     (FETCH_SIZE and WRITE_SIZE, scaled by a copy-kernel calibration) when it
     was taken on this configuration; else None."""
-    path = os.path.join(ROOT, "profiles", "res_calc_owner_airfoil_1200x600_traffic.json")
+    path = os.path.join(ROOT, "profiles", f"res_calc_owner_airfoil_{args.nx}x{args.ny}_traffic.json")
     if not os.path.exists(path):
         return None
     t = json.load(open(path))
@@ -192,7 +197,8 @@ def load_traffic(args):
             args.points_per_block == c["points_per_block"] and
             args.ept == c["cells_per_thread"] and args.nodes == c["nodes"] and
             args.point_terms == c.get("point_terms", False) and
-            args.accumulate == c.get("accumulate", "colours"))
+            args.accumulate == c.get("accumulate", "colours") and
+            args.persistent == c.get("persistent", False))
     return t["traffic_bytes"] if same else None
 
 
@@ -415,7 +421,7 @@ def main():
     mesh, p = build_problem(args, args.partition)
     loop, dev, out = make_loop(p, args.strategy, args.block_size, not args.no_stage,
                                args.graph, args.order, args.ept, args.nodes == "gather", args.point_terms,
-                               args.accumulate == "gather")
+                               args.accumulate == "gather", args.persistent)
     plan_s = time.perf_counter() - t_plan
     stats = loop.stats()
     wall, ev = time_loop(loop, args.steps, args.warmup)
@@ -457,6 +463,7 @@ def main():
                    "redundancy": round(stats["redundancy"], 4),
                    "threads_per_block": stats["threads_per_block"],
                    "stage_per_thread": stats["stage_per_thread"],
+                   "persistent": args.persistent, "launch_grid": stats["launch_grid"],
                    "plan_seconds": round(plan_s, 2)},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,

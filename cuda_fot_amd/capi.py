@@ -51,6 +51,7 @@ LOOP_TWO_PER_THREAD = 16
 LOOP_GATHER_NODES = 32
 LOOP_POINT_TERMS = 64
 LOOP_OWNER_GATHER = 128
+LOOP_PERSISTENT = 256
 
 
 class FotError(RuntimeError):
@@ -105,7 +106,8 @@ class LoopStats(C.Structure):
                 ("plan_device_bytes", C.c_uint64),
                 ("avg_thread_colours", C.c_double), ("reuse_factor", C.c_double),
                 ("redundancy", C.c_double), ("avg_cache_lines", C.c_double),
-                ("threads_per_block", C.c_uint32), ("stage_per_thread", C.c_uint32)]
+                ("threads_per_block", C.c_uint32), ("stage_per_thread", C.c_uint32),
+                ("launch_grid", C.c_uint32)]
 
 
 class BatchStats(C.Structure):

@@ -642,100 +642,126 @@ __global__ __launch_bounds__(1024) void res_calc_owner2_kernel(EngineArgs a,
 // the first KO staging rounds (KO = 1 when every block owns at most
 // blockDim.x cells): later rounds stage halo q / adt only, with no res or
 // list registers.
-template <bool SOA, int EPT, int KS, int LW, int KO>
-__global__ __launch_bounds__(1024) void res_calc_owner3_kernel(EngineArgs a,
-                                                               const OwnerMeta *__restrict__ meta,
-                                                               const uint32_t *__restrict__ cached,
-                                                               const uint32_t *__restrict__ packed,
-                                                               const uint32_t *__restrict__ plist,
-                                                               uint32_t nb) {
-  extern __shared__ double lds[];
-  const uint32_t b = xcd_block(blockIdx.x, nb);
-  const OwnerMeta mt = meta[b];
-  const uint32_t tid = threadIdx.x, bd = blockDim.x;
-  const uint32_t m = mt.ncached, nw = mt.nowned, ne = mt.ncells;
-  const bool contig = mt.contiguous != 0;
-  typedef typename std::conditional<LW == 4, uint4, uint2>::type list_t;
-  const list_t *lists = reinterpret_cast<const list_t *>(plist);
-  double *q_s = lds, *adt_s = q_s + 4 * m;
-  double *f_s = lds; // after the second barrier: 4 doubles per edge
-  const double *x = a.point_data[1], *q = a.point_data[2], *adt = a.point_data[3];
-  const uint32_t np = a.num_points[0];
-  const uint32_t *pcached = cached + mt.cache_off;
-  // plan arrays are padded: branch-free loads
+//
+// A block runs in three steps: its ids (packed edge words, node pairs,
+// staged cell ids, owned-cell lists), the data they name, then the body.
+// res_calc_owner3_kernel runs one block per workgroup; the persistent
+// res_calc_owner4_kernel loads the next block's ids while it runs the body
+// of the current one.
+
+// a block's ids, in registers
+template <int EPT, int KS, int KO> struct Owner3Ids {
   uint32_t pk[EPT];
   uint2 nn[EPT];
+  uint32_t gq[KS];
+  uint32_t lst[KO][4];
+};
+
+// the data a block's ids name, in registers until the body stages them
+template <int EPT, int KS, int KO> struct Owner3Data {
+  double2 vq0[KS], vq1[KS], vr0[KO], vr1[KO];
+  double vadt[KS];
+  double x1[EPT][2], x2[EPT][2];
+};
+
+// plan arrays are padded: branch-free loads
+template <int EPT, int KS, int LW, int KO>
+FOT_DEV void owner3_load_ids(Owner3Ids<EPT, KS, KO> &id, const OwnerMeta &mt, const EngineArgs &a,
+                             const uint32_t *__restrict__ cached,
+                             const uint32_t *__restrict__ packed,
+                             const uint32_t *__restrict__ plist, uint32_t tid, uint32_t bd) {
+  typedef typename std::conditional<LW == 4, uint4, uint2>::type list_t;
+  const list_t *lists = reinterpret_cast<const list_t *>(plist);
+  const uint32_t *pcached = cached + mt.cache_off;
+  const uint32_t m = mt.ncached, nw = mt.nowned, ne = mt.ncells;
+  const bool contig = mt.contiguous != 0;
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     const uint32_t t = tid + e * bd;
     const uint32_t pv = packed[mt.cell_off + t];
-    pk[e] = t < ne ? pv : 0xff000000u;
-    nn[e] = reinterpret_cast<const uint2 *>(a.maps[1])[mt.cell_off + t];
+    id.pk[e] = t < ne ? pv : 0xff000000u;
+    id.nn[e] = reinterpret_cast<const uint2 *>(a.maps[1])[mt.cell_off + t];
   }
-  uint32_t gq[KS];
-  uint32_t lst[KO][4];
 #pragma unroll
   for (int k = 0; k < KS; ++k) {
     const uint32_t i = tid + k * bd;
     const uint32_t pc = pcached[i];
-    gq[k] = (contig && i < nw) ? mt.own_start + i : (i < m ? pc : 0u);
+    id.gq[k] = (contig && i < nw) ? mt.own_start + i : (i < m ? pc : 0u);
     if (k < KO) {
       const list_t l = lists[mt.pad0 + i];
-      lst[k][0] = l.x, lst[k][1] = l.y;
-      if constexpr (LW == 4) lst[k][2] = l.z, lst[k][3] = l.w;
+      id.lst[k][0] = l.x, id.lst[k][1] = l.y;
+      if constexpr (LW == 4) id.lst[k][2] = l.z, id.lst[k][3] = l.w;
     }
   }
-  // res of owned slots only; the others re-read the first owned cell (a
-  // line the block reads anyway) instead of a halo cell's res
-  double2 vq0[KS], vq1[KS], vr0[KO], vr1[KO];
-  double vadt[KS];
+}
+
+// every id is a valid point (point 0 past the block's end): unconditional.
+// res is read for owned slots only; the others re-read the first owned cell
+// (a line the block reads anyway) instead of a halo cell's res
+template <bool SOA, int EPT, int KS, int KO>
+FOT_DEV void owner3_load_data(Owner3Data<EPT, KS, KO> &d, const Owner3Ids<EPT, KS, KO> &id,
+                              const OwnerMeta &mt, const EngineArgs &a, uint32_t tid,
+                              uint32_t bd) {
+  const double *x = a.point_data[1], *q = a.point_data[2], *adt = a.point_data[3];
+  const uint32_t np = a.num_points[0], nw = mt.nowned;
 #pragma unroll
   for (int k = 0; k < KS; ++k) {
-    const uint32_t g = gq[k];
+    const uint32_t g = id.gq[k];
     const uint32_t gr = (tid + k * bd < nw) ? g : mt.own_start;
     if (SOA) {
       const uint32_t nq = a.num_points[2];
-      vq0[k] = make_double2(q[g], q[size_t(nq) + g]);
-      vq1[k] = make_double2(q[2 * size_t(nq) + g], q[3 * size_t(nq) + g]);
+      d.vq0[k] = make_double2(q[g], q[size_t(nq) + g]);
+      d.vq1[k] = make_double2(q[2 * size_t(nq) + g], q[3 * size_t(nq) + g]);
       if (k < KO) {
-        vr0[k] = make_double2(a.out[gr], a.out[size_t(np) + gr]);
-        vr1[k] = make_double2(a.out[2 * size_t(np) + gr], a.out[3 * size_t(np) + gr]);
+        d.vr0[k] = make_double2(a.out[gr], a.out[size_t(np) + gr]);
+        d.vr1[k] = make_double2(a.out[2 * size_t(np) + gr], a.out[3 * size_t(np) + gr]);
       }
     } else {
       const double2 *q2p = reinterpret_cast<const double2 *>(q);
-      vq0[k] = q2p[2 * size_t(g)], vq1[k] = q2p[2 * size_t(g) + 1];
+      d.vq0[k] = q2p[2 * size_t(g)], d.vq1[k] = q2p[2 * size_t(g) + 1];
       if (k < KO) {
         const double2 *o = reinterpret_cast<const double2 *>(a.out) + 2 * size_t(gr);
-        vr0[k] = o[0], vr1[k] = o[1];
+        d.vr0[k] = o[0], d.vr1[k] = o[1];
       }
     }
-    vadt[k] = adt[g];
+    d.vadt[k] = adt[g];
   }
-  double x1[EPT][2], x2[EPT][2];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) { // nn is a valid node pair for every thread (padded map)
     if (SOA) {
       const uint32_t nx = a.num_points[1];
-      x1[e][0] = x[nn[e].x], x1[e][1] = x[nx + nn[e].x];
-      x2[e][0] = x[nn[e].y], x2[e][1] = x[nx + nn[e].y];
+      d.x1[e][0] = x[id.nn[e].x], d.x1[e][1] = x[nx + id.nn[e].x];
+      d.x2[e][0] = x[id.nn[e].y], d.x2[e][1] = x[nx + id.nn[e].y];
     } else {
-      const double2 xa = reinterpret_cast<const double2 *>(x)[nn[e].x];
-      const double2 xb = reinterpret_cast<const double2 *>(x)[nn[e].y];
-      x1[e][0] = xa.x, x1[e][1] = xa.y, x2[e][0] = xb.x, x2[e][1] = xb.y;
+      const double2 xa = reinterpret_cast<const double2 *>(x)[id.nn[e].x];
+      const double2 xb = reinterpret_cast<const double2 *>(x)[id.nn[e].y];
+      d.x1[e][0] = xa.x, d.x1[e][1] = xa.y, d.x2[e][0] = xb.x, d.x2[e][1] = xb.y;
     }
   }
-  // one wait for all of them (see res_calc_owner2_kernel)
+}
+
+// stage q / adt, fluxes, gather, store.  Starts with the LDS free; ends with
+// the LDS still being read by other waves (callers reusing it sync first).
+template <bool SOA, int EPT, int KS, int LW, int KO>
+FOT_DEV void owner3_body(double *lds, Owner3Data<EPT, KS, KO> &d, const Owner3Ids<EPT, KS, KO> &id,
+                         const OwnerMeta &mt, const EngineArgs &a, uint32_t tid, uint32_t bd) {
+  const uint32_t m = mt.ncached, nw = mt.nowned, ne = mt.ncells;
+  const uint32_t np = a.num_points[0];
+  double *q_s = lds, *adt_s = q_s + 4 * m;
+  double *f_s = lds; // after the second barrier: 4 doubles per edge
+  // one wait for all the loads (see res_calc_owner2_kernel)
 #pragma unroll
   for (int k = 0; k < KS; ++k) {
-    asm volatile("" ::"v"(vq0[k].x), "v"(vq0[k].y), "v"(vq1[k].x), "v"(vq1[k].y), "v"(vadt[k]));
+    asm volatile("" ::"v"(d.vq0[k].x), "v"(d.vq0[k].y), "v"(d.vq1[k].x), "v"(d.vq1[k].y),
+                 "v"(d.vadt[k]));
   }
 #pragma unroll
   for (int k = 0; k < KS; ++k) {
     const uint32_t i = tid + k * bd;
     if (i < m) {
-      q_s[0 * m + i] = vq0[k].x, q_s[1 * m + i] = vq0[k].y;
-      q_s[2 * m + i] = vq1[k].x, q_s[3 * m + i] = vq1[k].y;
-      adt_s[i] = vadt[k];
+      q_s[0 * m + i] = d.vq0[k].x, q_s[1 * m + i] = d.vq0[k].y;
+      q_s[2 * m + i] = d.vq1[k].x, q_s[3 * m + i] = d.vq1[k].y;
+      adt_s[i] = d.vadt[k];
     }
   }
   __syncthreads();
@@ -743,12 +769,12 @@ __global__ __launch_bounds__(1024) void res_calc_owner3_kernel(EngineArgs a,
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     if (tid + e * bd < ne) {
-      const uint32_t l0 = pk[e] & 0xfffu, l1 = (pk[e] >> 12) & 0xfffu;
+      const uint32_t l0 = id.pk[e] & 0xfffu, l1 = (id.pk[e] >> 12) & 0xfffu;
       double q1[4], q2[4], r2[4];
 #pragma unroll
-      for (uint32_t d = 0; d < 4; ++d) q1[d] = q_s[d * m + l0], q2[d] = q_s[d * m + l1];
+      for (uint32_t c = 0; c < 4; ++c) q1[c] = q_s[c * m + l0], q2[c] = q_s[c * m + l1];
       const double a1 = adt_s[l0], a2 = adt_s[l1];
-      res_calc_user(x1[e], x2[e], q1, q2, &a1, &a2, fl[e], r2, 1, 1);
+      res_calc_user(d.x1[e], d.x2[e], q1, q2, &a1, &a2, fl[e], r2, 1, 1);
     }
   }
   __syncthreads(); // q_s / adt_s are dead: f_s takes their place
@@ -766,10 +792,10 @@ __global__ __launch_bounds__(1024) void res_calc_owner3_kernel(EngineArgs a,
   for (int k = 0; k < KO; ++k) {
     const uint32_t i = tid + k * bd;
     if (i < nw) {
-      double r[4] = {vr0[k].x, vr0[k].y, vr1[k].x, vr1[k].y};
+      double r[4] = {d.vr0[k].x, d.vr0[k].y, d.vr1[k].x, d.vr1[k].y};
 #pragma unroll
       for (int j = 0; j < 2 * LW; ++j) {
-        const uint32_t ent = (lst[k][j >> 1] >> (16 * (j & 1))) & 0xffffu;
+        const uint32_t ent = (id.lst[k][j >> 1] >> (16 * (j & 1))) & 0xffffu;
         if (ent == 0xffffu) break;
         const double2 *fp = reinterpret_cast<const double2 *>(f_s + 4 * size_t(ent & 0x7fffu));
         const double2 f01 = fp[0], f23 = fp[1];
@@ -779,10 +805,10 @@ __global__ __launch_bounds__(1024) void res_calc_owner3_kernel(EngineArgs a,
           r[0] += f01.x, r[1] += f01.y, r[2] += f23.x, r[3] += f23.y;
         }
       }
-      const uint32_t g = gq[k];
+      const uint32_t g = id.gq[k];
       if (SOA) {
 #pragma unroll
-        for (uint32_t d = 0; d < 4; ++d) a.out[size_t(d) * np + g] = r[d];
+        for (uint32_t c = 0; c < 4; ++c) a.out[size_t(c) * np + g] = r[c];
       } else {
         double2 *o = reinterpret_cast<double2 *>(a.out) + 2 * size_t(g);
         o[0] = make_double2(r[0], r[1]);
@@ -790,6 +816,89 @@ __global__ __launch_bounds__(1024) void res_calc_owner3_kernel(EngineArgs a,
       }
     }
   }
+}
+
+template <bool SOA, int EPT, int KS, int LW, int KO>
+__global__ __launch_bounds__(1024) void res_calc_owner3_kernel(EngineArgs a,
+                                                               const OwnerMeta *__restrict__ meta,
+                                                               const uint32_t *__restrict__ cached,
+                                                               const uint32_t *__restrict__ packed,
+                                                               const uint32_t *__restrict__ plist,
+                                                               uint32_t nb) {
+  extern __shared__ double lds[];
+  const uint32_t b = xcd_block(blockIdx.x, nb);
+  const OwnerMeta mt = meta[b];
+  const uint32_t tid = threadIdx.x, bd = blockDim.x;
+  Owner3Ids<EPT, KS, KO> id;
+  Owner3Data<EPT, KS, KO> d;
+  owner3_load_ids<EPT, KS, LW, KO>(id, mt, a, cached, packed, plist, tid, bd);
+  owner3_load_data<SOA, EPT, KS, KO>(d, id, mt, a, tid, bd);
+  owner3_body<SOA, EPT, KS, LW, KO>(lds, d, id, mt, a, tid, bd);
+}
+
+// Persistent form: gridDim.x workgroups (a multiple of 8, at most the
+// resident count) each run blocks j = blockIdx.x, + gridDim.x, ...; with
+// blocks mapped by xcd_block every workgroup's blocks sit in one XCD's
+// chunk.  The next block's record and ids load while the current block's
+// body runs, so after its first block a workgroup waits for one round trip
+// per block (the data), not two.  Same per-block arithmetic: the same bits.
+template <bool SOA, int EPT, int KS, int LW, int KO>
+__global__ __launch_bounds__(1024) void res_calc_owner4_kernel(EngineArgs a,
+                                                               const OwnerMeta *__restrict__ meta,
+                                                               const uint32_t *__restrict__ cached,
+                                                               const uint32_t *__restrict__ packed,
+                                                               const uint32_t *__restrict__ plist,
+                                                               uint32_t nb) {
+  extern __shared__ double lds[];
+  const uint32_t tid = threadIdx.x, bd = blockDim.x, G = gridDim.x;
+  uint32_t j = blockIdx.x;
+  if (j >= nb) return;
+  OwnerMeta mt = meta[xcd_block(j, nb)];
+  Owner3Ids<EPT, KS, KO> id;
+  owner3_load_ids<EPT, KS, LW, KO>(id, mt, a, cached, packed, plist, tid, bd);
+  for (;;) {
+    Owner3Data<EPT, KS, KO> d;
+    owner3_load_data<SOA, EPT, KS, KO>(d, id, mt, a, tid, bd);
+    const uint32_t jn = j + G;
+    const bool more = jn < nb; // uniform
+    // the last block loads its own entries again: branch-free, in bounds
+    const OwnerMeta mtn = meta[xcd_block(more ? jn : j, nb)];
+    Owner3Ids<EPT, KS, KO> idn;
+    owner3_load_ids<EPT, KS, LW, KO>(idn, mtn, a, cached, packed, plist, tid, bd);
+    owner3_body<SOA, EPT, KS, LW, KO>(lds, d, id, mt, a, tid, bd);
+    if (!more) break;
+    __syncthreads(); // the next block's staging overwrites f_s
+    j = jn;
+    mt = mtn;
+    id = idn;
+  }
+}
+
+// the gather kernel instantiation for a plan: one block per workgroup or
+// persistent, edges per thread, staging rounds, list words, owned rounds
+template <bool SOA, int E, int K, int W, int O> const void *owner3_pick(bool persistent) {
+  return persistent ? reinterpret_cast<const void *>(&res_calc_owner4_kernel<SOA, E, K, W, O>)
+                    : reinterpret_cast<const void *>(&res_calc_owner3_kernel<SOA, E, K, W, O>);
+}
+template <bool SOA, int E, int K>
+const void *owner3_pick_w(bool persistent, uint32_t lw, bool ko1) {
+  if (lw == 2)
+    return ko1 ? owner3_pick<SOA, E, K, 2, 1>(persistent) : owner3_pick<SOA, E, K, 2, K>(persistent);
+  return ko1 ? owner3_pick<SOA, E, K, 4, 1>(persistent) : owner3_pick<SOA, E, K, 4, K>(persistent);
+}
+template <bool SOA, int E>
+const void *owner3_pick_k(bool persistent, uint32_t ks, uint32_t lw, bool ko1) {
+  switch (ks) {
+  case 1: return owner3_pick_w<SOA, E, 1>(persistent, lw, ko1);
+  case 2: return owner3_pick_w<SOA, E, 2>(persistent, lw, ko1);
+  case 3: return owner3_pick_w<SOA, E, 3>(persistent, lw, ko1);
+  default: return owner3_pick_w<SOA, E, 4>(persistent, lw, ko1);
+  }
+}
+template <bool SOA>
+const void *owner3_pick_kernel(bool persistent, uint32_t ept, uint32_t ks, uint32_t lw, bool ko1) {
+  return ept == 2 ? owner3_pick_k<SOA, 2>(persistent, ks, lw, ko1)
+                  : owner3_pick_k<SOA, 1>(persistent, ks, lw, ko1);
 }
 
 // dst[d * n_dst + i] = src[d * n_src + idx[i]] (SOA gather of cell data)
@@ -838,6 +947,8 @@ struct fot_loop {
   uint32_t list_words = 4;       // owner gather: 32-bit words per owned-cell list
   uint32_t stage_per_thread = 1; // owner res_calc: staged points per thread
   bool owned_first_round = false; // owner res_calc: every block owns <= threads cells
+  const void *owner3_fn = nullptr; // owner gather: the kernel instantiation to launch
+  uint32_t launch_grid = 0;        // workgroups of the single-launch strategies
   uint32_t ept = 1;
   DepDev dep{};
   // graph of one iteration
@@ -1291,6 +1402,26 @@ fot_status fot_loop::build_hierarchical() {
     for (uint32_t nw : blk_owned) most_owned = std::max(most_owned, nw);
     owned_first_round = most_owned <= threads;
   }
+  launch_grid = (dep_order || owner) ? nb_total : 0;
+  if (owner && f.kernel == FOT_KERNEL_RES_CALC && owner_gather) {
+    const bool persistent = (cfg.flags & FOT_LOOP_PERSISTENT) != 0;
+    owner3_fn = desc.soa ? owner3_pick_kernel<true>(persistent, ept, stage_per_thread,
+                                                    list_words, owned_first_round)
+                         : owner3_pick_kernel<false>(persistent, ept, stage_per_thread,
+                                                     list_words, owned_first_round);
+    if (persistent && nb_total) {
+      // as many workgroups as are resident at once, a multiple of the 8 XCDs
+      int dev = 0, cus = 0, per_cu = 0;
+      FOT_HIP(hipGetDevice(&dev));
+      FOT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      FOT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, owner3_fn, int(threads),
+                                                           size_t(lds_bytes)));
+      if (per_cu < 1) return fail(FOT_ERR_UNSUPPORTED, "persistent owner kernel does not fit a CU");
+      uint64_t g = std::min<uint64_t>(nb_total, uint64_t(per_cu) * uint64_t(cus));
+      if (g >= kNumXcd) g -= g % kNumXcd;
+      launch_grid = uint32_t(g);
+    }
+  }
   stats.num_colours = num_plan_colours;
   stats.num_blocks = nb_total;
   stats.num_launches_per_iter = (dep_order || owner) ? 1u : stats.num_colours;
@@ -1298,6 +1429,7 @@ fot_status fot_loop::build_hierarchical() {
   stats.lds_bytes = lds_bytes;
   stats.threads_per_block = threads;
   stats.stage_per_thread = stage_per_thread;
+  stats.launch_grid = launch_grid;
   stats.max_thread_colours = max_tc;
   stats.avg_thread_colours = nb_total ? tc_sum / nb_total : 0.0;
   // (cached may carry padding at its end: count the plan's entries only)
@@ -1353,38 +1485,13 @@ fot_status fot_loop::enqueue(hipStream_t s) {
   default: FOT_OWNER2(E_, X_, 4); break;                                                \
   }
         if (owner_gather) {
-#define FOT_OWNER3WO(E_, K_, W_, O_)                                                    \
-  hipLaunchKernelGGL((res_calc_owner3_kernel<SOA, E_, K_, W_, O_>), dim3(nb), dim3(threads), \
-                     lds_bytes, s, plan_args, owner_meta, hier.cached, hier.packed,          \
-                     owner_plist, nb)
-#define FOT_OWNER3W(E_, K_, W_)                                                         \
-  do {                                                                                  \
-    if (owned_first_round) FOT_OWNER3WO(E_, K_, W_, 1);                                 \
-    else FOT_OWNER3WO(E_, K_, W_, K_);                                                  \
-  } while (0)
-#define FOT_OWNER3(E_, K_)                                                              \
-  do {                                                                                  \
-    if (list_words == 2) FOT_OWNER3W(E_, K_, 2);                                        \
-    else FOT_OWNER3W(E_, K_, 4);                                                        \
-  } while (0)
-          if (ept == 2) {
-            switch (stage_per_thread) {
-            case 1: FOT_OWNER3(2, 1); break;
-            case 2: FOT_OWNER3(2, 2); break;
-            case 3: FOT_OWNER3(2, 3); break;
-            default: FOT_OWNER3(2, 4); break;
-            }
-          } else {
-            switch (stage_per_thread) {
-            case 1: FOT_OWNER3(1, 1); break;
-            case 2: FOT_OWNER3(1, 2); break;
-            case 3: FOT_OWNER3(1, 3); break;
-            default: FOT_OWNER3(1, 4); break;
-            }
-          }
-#undef FOT_OWNER3
-#undef FOT_OWNER3W
-#undef FOT_OWNER3WO
+          EngineArgs pa = plan_args;
+          const OwnerMeta *om = owner_meta;
+          const uint32_t *pc = hier.cached, *pp = hier.packed, *pl = owner_plist;
+          uint32_t nbv = nb;
+          void *kargs[] = {&pa, &om, &pc, &pp, &pl, &nbv};
+          FOT_HIP(hipLaunchKernel(owner3_fn, dim3(launch_grid), dim3(threads), kargs,
+                                  size_t(lds_bytes), s));
         } else if (ept == 2) {
           if (stage_x) FOT_OWNER2_K(2, true)
           else FOT_OWNER2_K(2, false)

@@ -478,6 +478,11 @@ typedef struct {
  * the same sums in the same order, the same bits.  Falls back to the
  * colour loop when a cell has more than 8 edges. */
 #define FOT_LOOP_OWNER_GATHER 128u
+/* owner compute, res_calc gather: persistent workgroups (as many as are
+ * resident at once), each running a sequence of blocks and loading the next
+ * block's plan entries while it computes the current one; the same bits.
+ * Ignored by the other kernels. */
+#define FOT_LOOP_PERSISTENT 256u
 
 typedef struct fot_loop fot_loop;
 /* Builds the plan (host), uploads it, permutes cell data. */
@@ -506,6 +511,8 @@ typedef struct {
    * rounds each thread runs over the block's staged points */
   uint32_t threads_per_block;
   uint32_t stage_per_thread;
+  /* single-launch strategies: workgroups per launch */
+  uint32_t launch_grid;
 } fot_loop_stats;
 fot_status fot_loop_stats_get(const fot_loop *loop, fot_loop_stats *stats);
 

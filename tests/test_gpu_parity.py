@@ -63,6 +63,7 @@ TWO = capi.LOOP_TWO_PER_THREAD
 GATHER = capi.LOOP_GATHER_NODES
 TERMS = capi.LOOP_POINT_TERMS
 OG = capi.LOOP_OWNER_GATHER
+PERS = capi.LOOP_PERSISTENT
 
 
 def _owner_replay(p, kernel, block_size, partition=None, iters=ITERS, point_partition=None):
@@ -90,6 +91,10 @@ def _owner_replay(p, kernel, block_size, partition=None, iters=ITERS, point_part
                                                  (False, 128, OWN | OG, "points"),
                                                  (False, 256, OWN | TWO | OG, None),
                                                  (True, 512, OWN | OG, "kway"),
+                                                 (False, 192, OWN | TWO | OG | PERS, "points"),
+                                                 (True, 224, OWN | TWO | OG | PERS, "points"),
+                                                 (False, 128, OWN | OG | PERS, "points"),
+                                                 (True, 512, OWN | OG | PERS, "kway"),
                                                  (True, 128, OWN | GATHER, "points"),
                                                  (False, 256, OWN | TWO, None)])
 def test_res_calc_owner_compute_bitwise(gpu, soa, bs, flags, method):
@@ -280,7 +285,7 @@ def test_loop_keeps_its_device_buffers_alive(gpu):
 
 
 @pytest.mark.parametrize("soa,flags", [(False, OWN | TWO | OG), (True, OWN | TWO | OG),
-                                       (False, OWN | TWO)])
+                                       (False, OWN | TWO | OG | PERS), (False, OWN | TWO)])
 def test_res_calc_owner_blocks_owning_more_cells_than_threads(gpu, soa, flags):
     # every third edge of the airfoil and blocks of 256 consecutive cells: a
     # block runs ~170 edges (96 threads at two edges each, rounded to 192)
@@ -303,3 +308,19 @@ def test_res_calc_owner_blocks_owning_more_cells_than_threads(gpu, soa, flags):
     assert stats["threads_per_block"] < 256, stats
     assert_bitwise(p.point_data[0], ref, f"owner, owned cells over several rounds ({flags})")
     assert_close_scaled(p.point_data[0], serial, S, what="owner vs serial")
+
+
+@pytest.mark.parametrize("soa", [False, True])
+def test_res_calc_persistent_more_blocks_than_resident_workgroups(gpu, soa):
+    # ~2240 blocks of 240 points: more than the resident workgroups (at most
+    # 8 of 256 threads per CU, 2048 on the device), so persistent workgroups
+    # run several blocks, a grid apart
+    p = airfoil_problem(nx=960, ny=560, soa=soa, res0_seed=12, block_size=240)
+    orig = airfoil_problem(nx=960, ny=560, soa=soa, res0_seed=12, block_size=240)
+    p.partition_points(240)
+    ref = _owner_replay(p, O.RES_CALC, 240, point_partition=p.point_partition)
+    serial, S = _serial(orig, O.RES_CALC)
+    stats = p.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, ITERS, flags=OWN | TWO | OG | PERS)
+    assert stats["launch_grid"] % 8 == 0 and stats["launch_grid"] < stats["num_blocks"], stats
+    assert_bitwise(p.point_data[0], ref, "persistent owner gather")
+    assert_close_scaled(p.target_in_original_order(), serial, S, what="persistent vs serial")
