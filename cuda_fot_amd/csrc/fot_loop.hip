@@ -742,12 +742,15 @@ FOT_DEV void owner3_load_data(Owner3Data<EPT, KS, KO> &d, const Owner3Ids<EPT, K
 
 // stage q / adt, fluxes, gather, store.  Starts with the LDS free; ends with
 // the LDS still being read by other waves (callers reusing it sync first).
-template <bool SOA, int EPT, int KS, int LW, int KO>
+// With PRE, 1/rho and the pressure of each staged cell are computed once at
+// staging (LDS) instead of once per edge side: the same operations on the
+// same values, the same bits.
+template <bool SOA, int EPT, int KS, int LW, int KO, bool PRE>
 FOT_DEV void owner3_body(double *lds, Owner3Data<EPT, KS, KO> &d, const Owner3Ids<EPT, KS, KO> &id,
                          const OwnerMeta &mt, const EngineArgs &a, uint32_t tid, uint32_t bd) {
   const uint32_t m = mt.ncached, nw = mt.nowned, ne = mt.ncells;
   const uint32_t np = a.num_points[0];
-  double *q_s = lds, *adt_s = q_s + 4 * m;
+  double *q_s = lds, *adt_s = q_s + 4 * m, *ri_s = adt_s + m, *p_s = ri_s + m;
   double *f_s = lds; // after the second barrier: 4 doubles per edge
   // one wait for all the loads (see res_calc_owner2_kernel)
 #pragma unroll
@@ -762,6 +765,10 @@ FOT_DEV void owner3_body(double *lds, Owner3Data<EPT, KS, KO> &d, const Owner3Id
       q_s[0 * m + i] = d.vq0[k].x, q_s[1 * m + i] = d.vq0[k].y;
       q_s[2 * m + i] = d.vq1[k].x, q_s[3 * m + i] = d.vq1[k].y;
       adt_s[i] = d.vadt[k];
+      if (PRE) {
+        const double qq[4] = {d.vq0[k].x, d.vq0[k].y, d.vq1[k].x, d.vq1[k].y};
+        res_calc_point_terms(qq, 1, &ri_s[i], &p_s[i]);
+      }
     }
   }
   __syncthreads();
@@ -774,7 +781,11 @@ FOT_DEV void owner3_body(double *lds, Owner3Data<EPT, KS, KO> &d, const Owner3Id
 #pragma unroll
       for (uint32_t c = 0; c < 4; ++c) q1[c] = q_s[c * m + l0], q2[c] = q_s[c * m + l1];
       const double a1 = adt_s[l0], a2 = adt_s[l1];
-      res_calc_user(d.x1[e], d.x2[e], q1, q2, &a1, &a2, fl[e], r2, 1, 1);
+      if (PRE)
+        res_calc_user_terms(d.x1[e], d.x2[e], q1, q2, ri_s[l0], ri_s[l1], p_s[l0], p_s[l1], a1,
+                            a2, fl[e], r2);
+      else
+        res_calc_user(d.x1[e], d.x2[e], q1, q2, &a1, &a2, fl[e], r2, 1, 1);
     }
   }
   __syncthreads(); // q_s / adt_s are dead: f_s takes their place
@@ -818,7 +829,7 @@ FOT_DEV void owner3_body(double *lds, Owner3Data<EPT, KS, KO> &d, const Owner3Id
   }
 }
 
-template <bool SOA, int EPT, int KS, int LW, int KO>
+template <bool SOA, int EPT, int KS, int LW, int KO, bool PRE>
 __global__ __launch_bounds__(1024) void res_calc_owner3_kernel(EngineArgs a,
                                                                const OwnerMeta *__restrict__ meta,
                                                                const uint32_t *__restrict__ cached,
@@ -833,7 +844,7 @@ __global__ __launch_bounds__(1024) void res_calc_owner3_kernel(EngineArgs a,
   Owner3Data<EPT, KS, KO> d;
   owner3_load_ids<EPT, KS, LW, KO>(id, mt, a, cached, packed, plist, tid, bd);
   owner3_load_data<SOA, EPT, KS, KO>(d, id, mt, a, tid, bd);
-  owner3_body<SOA, EPT, KS, LW, KO>(lds, d, id, mt, a, tid, bd);
+  owner3_body<SOA, EPT, KS, LW, KO, PRE>(lds, d, id, mt, a, tid, bd);
 }
 
 // Persistent form: gridDim.x workgroups (a multiple of 8, at most the
@@ -842,7 +853,7 @@ __global__ __launch_bounds__(1024) void res_calc_owner3_kernel(EngineArgs a,
 // chunk.  The next block's record and ids load while the current block's
 // body runs, so after its first block a workgroup waits for one round trip
 // per block (the data), not two.  Same per-block arithmetic: the same bits.
-template <bool SOA, int EPT, int KS, int LW, int KO>
+template <bool SOA, int EPT, int KS, int LW, int KO, bool PRE>
 __global__ __launch_bounds__(1024) void res_calc_owner4_kernel(EngineArgs a,
                                                                const OwnerMeta *__restrict__ meta,
                                                                const uint32_t *__restrict__ cached,
@@ -865,7 +876,7 @@ __global__ __launch_bounds__(1024) void res_calc_owner4_kernel(EngineArgs a,
     const OwnerMeta mtn = meta[xcd_block(more ? jn : j, nb)];
     Owner3Ids<EPT, KS, KO> idn;
     owner3_load_ids<EPT, KS, LW, KO>(idn, mtn, a, cached, packed, plist, tid, bd);
-    owner3_body<SOA, EPT, KS, LW, KO>(lds, d, id, mt, a, tid, bd);
+    owner3_body<SOA, EPT, KS, LW, KO, PRE>(lds, d, id, mt, a, tid, bd);
     if (!more) break;
     __syncthreads(); // the next block's staging overwrites f_s
     j = jn;
@@ -876,29 +887,38 @@ __global__ __launch_bounds__(1024) void res_calc_owner4_kernel(EngineArgs a,
 
 // the gather kernel instantiation for a plan: one block per workgroup or
 // persistent, edges per thread, staging rounds, list words, owned rounds
-template <bool SOA, int E, int K, int W, int O> const void *owner3_pick(bool persistent) {
-  return persistent ? reinterpret_cast<const void *>(&res_calc_owner4_kernel<SOA, E, K, W, O>)
-                    : reinterpret_cast<const void *>(&res_calc_owner3_kernel<SOA, E, K, W, O>);
+template <bool SOA, int E, int K, int W, int O>
+const void *owner3_pick(bool persistent, bool pre) {
+  if (pre)
+    return persistent
+               ? reinterpret_cast<const void *>(&res_calc_owner4_kernel<SOA, E, K, W, O, true>)
+               : reinterpret_cast<const void *>(&res_calc_owner3_kernel<SOA, E, K, W, O, true>);
+  return persistent
+             ? reinterpret_cast<const void *>(&res_calc_owner4_kernel<SOA, E, K, W, O, false>)
+             : reinterpret_cast<const void *>(&res_calc_owner3_kernel<SOA, E, K, W, O, false>);
 }
 template <bool SOA, int E, int K>
-const void *owner3_pick_w(bool persistent, uint32_t lw, bool ko1) {
+const void *owner3_pick_w(bool persistent, bool pre, uint32_t lw, bool ko1) {
   if (lw == 2)
-    return ko1 ? owner3_pick<SOA, E, K, 2, 1>(persistent) : owner3_pick<SOA, E, K, 2, K>(persistent);
-  return ko1 ? owner3_pick<SOA, E, K, 4, 1>(persistent) : owner3_pick<SOA, E, K, 4, K>(persistent);
+    return ko1 ? owner3_pick<SOA, E, K, 2, 1>(persistent, pre)
+               : owner3_pick<SOA, E, K, 2, K>(persistent, pre);
+  return ko1 ? owner3_pick<SOA, E, K, 4, 1>(persistent, pre)
+             : owner3_pick<SOA, E, K, 4, K>(persistent, pre);
 }
 template <bool SOA, int E>
-const void *owner3_pick_k(bool persistent, uint32_t ks, uint32_t lw, bool ko1) {
+const void *owner3_pick_k(bool persistent, bool pre, uint32_t ks, uint32_t lw, bool ko1) {
   switch (ks) {
-  case 1: return owner3_pick_w<SOA, E, 1>(persistent, lw, ko1);
-  case 2: return owner3_pick_w<SOA, E, 2>(persistent, lw, ko1);
-  case 3: return owner3_pick_w<SOA, E, 3>(persistent, lw, ko1);
-  default: return owner3_pick_w<SOA, E, 4>(persistent, lw, ko1);
+  case 1: return owner3_pick_w<SOA, E, 1>(persistent, pre, lw, ko1);
+  case 2: return owner3_pick_w<SOA, E, 2>(persistent, pre, lw, ko1);
+  case 3: return owner3_pick_w<SOA, E, 3>(persistent, pre, lw, ko1);
+  default: return owner3_pick_w<SOA, E, 4>(persistent, pre, lw, ko1);
   }
 }
 template <bool SOA>
-const void *owner3_pick_kernel(bool persistent, uint32_t ept, uint32_t ks, uint32_t lw, bool ko1) {
-  return ept == 2 ? owner3_pick_k<SOA, 2>(persistent, ks, lw, ko1)
-                  : owner3_pick_k<SOA, 1>(persistent, ks, lw, ko1);
+const void *owner3_pick_kernel(bool persistent, bool pre, uint32_t ept, uint32_t ks, uint32_t lw,
+                               bool ko1) {
+  return ept == 2 ? owner3_pick_k<SOA, 2>(persistent, pre, ks, lw, ko1)
+                  : owner3_pick_k<SOA, 1>(persistent, pre, ks, lw, ko1);
 }
 
 // dst[d * n_dst + i] = src[d * n_src + idx[i]] (SOA gather of cell data)
@@ -1357,12 +1377,14 @@ fot_status fot_loop::build_hierarchical() {
     }
     if (f.kernel == FOT_KERNEL_RES_CALC && stage_x) lds_bytes += 16u * max_nodes; // staged x
     if (f.kernel == FOT_KERNEL_RES_CALC && owner_gather) {
-      // q / adt of the staged cells, later the edges' fluxes in the same place
+      // q / adt (+ 1/rho and pressure) of the staged cells, later the edges'
+      // fluxes in the same place
+      const uint32_t per_cell = point_terms ? 7u : 5u;
       lds_bytes = 0;
       for (size_t b = 0; b + 1 < blk_cache_off.size(); ++b) {
         const uint32_t m = blk_cache_off[b + 1] - blk_cache_off[b];
         const uint32_t ne = blk_cell_off[b + 1] - blk_cell_off[b];
-        lds_bytes = std::max(lds_bytes, 8u * std::max(5u * m, 4u * ne));
+        lds_bytes = std::max(lds_bytes, 8u * std::max(per_cell * m, 4u * ne));
       }
     }
     if (f.kernel == FOT_KERNEL_RES_CALC && point_terms && !owner_gather) {
@@ -1405,10 +1427,12 @@ fot_status fot_loop::build_hierarchical() {
   launch_grid = (dep_order || owner) ? nb_total : 0;
   if (owner && f.kernel == FOT_KERNEL_RES_CALC && owner_gather) {
     const bool persistent = (cfg.flags & FOT_LOOP_PERSISTENT) != 0;
-    owner3_fn = desc.soa ? owner3_pick_kernel<true>(persistent, ept, stage_per_thread,
-                                                    list_words, owned_first_round)
-                         : owner3_pick_kernel<false>(persistent, ept, stage_per_thread,
-                                                     list_words, owned_first_round);
+    owner3_fn = desc.soa ? owner3_pick_kernel<true>(persistent, point_terms, ept,
+                                                    stage_per_thread, list_words,
+                                                    owned_first_round)
+                         : owner3_pick_kernel<false>(persistent, point_terms, ept,
+                                                     stage_per_thread, list_words,
+                                                     owned_first_round);
     if (persistent && nb_total) {
       // as many workgroups as are resident at once, a multiple of the 8 XCDs
       int dev = 0, cus = 0, per_cu = 0;

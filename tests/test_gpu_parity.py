@@ -95,6 +95,10 @@ def _owner_replay(p, kernel, block_size, partition=None, iters=ITERS, point_part
                                                  (True, 224, OWN | TWO | OG | PERS, "points"),
                                                  (False, 128, OWN | OG | PERS, "points"),
                                                  (True, 512, OWN | OG | PERS, "kway"),
+                                                 (False, 240, OWN | TWO | OG | TERMS, "points"),
+                                                 (True, 192, OWN | TWO | OG | TERMS, "points"),
+                                                 (False, 192, OWN | TWO | OG | TERMS | PERS,
+                                                  "points"),
                                                  (True, 128, OWN | GATHER, "points"),
                                                  (False, 256, OWN | TWO, None)])
 def test_res_calc_owner_compute_bitwise(gpu, soa, bs, flags, method):
