@@ -40,10 +40,10 @@ def parse():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="res_calc",
-                    choices=["res_calc", "volna-batch", "mini_aero", "hex"],
+                    choices=["res_calc", "volna-batch", "mini_aero", "hex", "lulesh"],
                     help="res_calc: the airfoil edge loop (the headline line); the others "
                          "print their own line: 64 batched Volna meshes, mini_aero 96^3, "
-                         "hex scatter 128^3")
+                         "hex scatter 128^3, lulesh element forces 128^3")
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nx", type=int, default=1200)
@@ -350,10 +350,19 @@ def bench_volna_batch(args):
 
 
 def bench_single(args, workload):
-    """mini_aero 96^3 faces or hex scatter 128^3 elements: owner compute on a
-    point-partitioned mesh beside plain atomics."""
-    from cuda_fot_amd import Problem, capi, meshgen, mini_aero_problem
-    if workload == "mini_aero":
+    """mini_aero 96^3 faces, hex scatter or lulesh forces on 128^3 elements:
+    owner compute on a point-partitioned mesh beside plain atomics."""
+    from cuda_fot_amd import Problem, capi, lulesh_problem, meshgen, mini_aero_problem
+    if workload == "lulesh":
+        n = 128
+        p = lulesh_problem(n + 1, n + 1, n + 1, seed=0, block_size=512)
+        f = capi.functor(capi.KERNEL_LULESH)
+        nn, nc = (n + 1) ** 3, n ** 3
+        # xyz 24 B + f 24 B read + 24 B written per node; sigma 24 B, volume
+        # 8 B written and 8 map ids (32 B) per element
+        units, ubytes = nc, 72 * nn + 64 * nc
+        name = f"lulesh element forces {n}^3 elements, shuffled, seed 0"
+    elif workload == "mini_aero":
         n = 96
         m = meshgen.hex_faces(n, n, n, seed=0)
         p = mini_aero_problem(m, block_size=512)
@@ -414,7 +423,7 @@ def main():
     if args.workload == "volna-batch":
         print(json.dumps(bench_volna_batch(args)), flush=True)
         return
-    if args.workload in ("mini_aero", "hex"):
+    if args.workload in ("mini_aero", "hex", "lulesh"):
         print(json.dumps(bench_single(args, args.workload)), flush=True)
         return
     t_plan = time.perf_counter()

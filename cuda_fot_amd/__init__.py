@@ -7,10 +7,11 @@ is a thin Python binding over that library; it has no CPU fallback.
 from . import capi, meshgen, plan  # noqa: F401
 from .capi import FotError, InvalidInputFile, ColourOverflow  # noqa: F401
 from .device import Batch, DeviceArray, Loop, Stream, Timer, synchronize  # noqa: F401
-from .problem import (Problem, getacc_problem, gpu_batch, mini_aero_problem,  # noqa: F401
+from .problem import (Problem, getacc_problem, gpu_batch, lulesh_problem, mini_aero_problem,  # noqa: F401
                       res_calc_problem, volna_problem)
 
 __all__ = ["capi", "meshgen", "plan", "FotError", "InvalidInputFile", "ColourOverflow",
            "Batch", "DeviceArray", "Loop", "Stream", "Timer", "synchronize", "Problem",
-           "getacc_problem", "gpu_batch", "mini_aero_problem", "res_calc_problem",
+           "getacc_problem", "gpu_batch", "lulesh_problem", "mini_aero_problem",
+           "res_calc_problem",
            "volna_problem"]

@@ -37,6 +37,12 @@ KERNEL_HEX_SCATTER = 2
 KERNEL_MINI_AERO = 3
 KERNEL_VOLNA = 4
 KERNEL_GETACC = 5
+KERNEL_LULESH = 6
+
+
+def cell_output_slot(kernel: int) -> int:
+    """The cell data slot a kernel writes (lulesh's element volume), or -1."""
+    return 1 if kernel == KERNEL_LULESH else -1
 
 # fot_strategy
 STRATEGY_ATOMIC = 0
@@ -212,6 +218,7 @@ _PROTOS = {
     "fot_gen_hex_faces": [U32, U32, U32, U64, U32, P, P, P],
     "fot_gen_quad_grid": [U32, U32, P],
     "fot_mini_aero_face_flux": [P, P, U32, P, U32, P],
+    "fot_lulesh_element": [P, P, P, P, U32, P, P],
     "fot_loop_create": [P, P, P, P],
     "fot_loop_destroy": [P],
     "fot_loop_step": [P, P],

@@ -26,6 +26,8 @@ fot_status dispatch(const fot_functor *f, int soa, Body &&body) {
     FOT_SOA(MiniAeroF);
   case FOT_KERNEL_GETACC:
     FOT_SOA(GetaccF);
+  case FOT_KERNEL_LULESH:
+    FOT_SOA(LuleshF);
   case FOT_KERNEL_MINE: {
 #define FOT_MINE_PC(MD_, P_)                                                           \
   if (f->mesh_dim == MD_ && f->point_dim == P_) {                                      \

@@ -393,6 +393,95 @@ struct MiniAeroF {
 // normals (2), right-boundary flag (0.0 / 1.0).  The increments start at 0
 // and are updated with the reference's `-=` / `+=`, so even signed zeros
 // match.  The fourth state component gets no increment.
+// ---- lulesh element forces (kernels/lulesh_func.hpp:13-38 with
+// kernels/lulesh_utils.hpp) ----------------------------------------------------
+// The element volume from the shape-function derivatives at the centre (of
+// which the reference keeps only the volume: its derivative array is then
+// overwritten with the node normals), the node normals (each face adds a
+// quarter of the cross product of its bisectors to its four nodes, faces and
+// nodes in the reference's order) and f = -(sigma * normal).  The
+// reference's association throughout, so host, device and the oracle agree
+// bit for bit.  f[i * 8 + j]: component i of corner j.
+namespace lu {
+
+template <int A, int B, int C, int D>
+FOT_HD void face_normal(const double *x, const double *y, const double *z, double *nx,
+                        double *ny, double *nz) {
+  const double b0x = 0.5 * (x[D] + x[C] - x[B] - x[A]);
+  const double b0y = 0.5 * (y[D] + y[C] - y[B] - y[A]);
+  const double b0z = 0.5 * (z[D] + z[C] - z[B] - z[A]);
+  const double b1x = 0.5 * (x[C] + x[B] - x[D] - x[A]);
+  const double b1y = 0.5 * (y[C] + y[B] - y[D] - y[A]);
+  const double b1z = 0.5 * (z[C] + z[B] - z[D] - z[A]);
+  const double ax = 0.25 * (b0y * b1z - b0z * b1y);
+  const double ay = 0.25 * (b0z * b1x - b0x * b1z);
+  const double az = 0.25 * (b0x * b1y - b0y * b1x);
+  nx[A] += ax, nx[B] += ax, nx[C] += ax, nx[D] += ax;
+  ny[A] += ay, ny[B] += ay, ny[C] += ay, ny[D] += ay;
+  nz[A] += az, nz[B] += az, nz[C] += az, nz[D] += az;
+}
+
+FOT_HD void element(const double *x, const double *y, const double *z, double sxx, double syy,
+                    double szz, double *f, double *determ) {
+  const double fjxxi = .125 * ((x[6] - x[0]) + (x[5] - x[3]) - (x[7] - x[1]) - (x[4] - x[2]));
+  const double fjxet = .125 * ((x[6] - x[0]) - (x[5] - x[3]) + (x[7] - x[1]) - (x[4] - x[2]));
+  const double fjxze = .125 * ((x[6] - x[0]) + (x[5] - x[3]) + (x[7] - x[1]) + (x[4] - x[2]));
+  const double fjyxi = .125 * ((y[6] - y[0]) + (y[5] - y[3]) - (y[7] - y[1]) - (y[4] - y[2]));
+  const double fjyet = .125 * ((y[6] - y[0]) - (y[5] - y[3]) + (y[7] - y[1]) - (y[4] - y[2]));
+  const double fjyze = .125 * ((y[6] - y[0]) + (y[5] - y[3]) + (y[7] - y[1]) + (y[4] - y[2]));
+  const double fjzxi = .125 * ((z[6] - z[0]) + (z[5] - z[3]) - (z[7] - z[1]) - (z[4] - z[2]));
+  const double fjzet = .125 * ((z[6] - z[0]) - (z[5] - z[3]) + (z[7] - z[1]) - (z[4] - z[2]));
+  const double fjzze = .125 * ((z[6] - z[0]) + (z[5] - z[3]) + (z[7] - z[1]) + (z[4] - z[2]));
+  const double cjxet = -(fjyxi * fjzze) + (fjzxi * fjyze);
+  const double cjyet = (fjxxi * fjzze) - (fjzxi * fjxze);
+  const double cjzet = -(fjxxi * fjyze) + (fjyxi * fjxze);
+  *determ = 8. * (fjxet * cjxet + fjyet * cjyet + fjzet * cjzet);
+  double nx[8], ny[8], nz[8];
+  for (int i = 0; i < 8; ++i) nx[i] = ny[i] = nz[i] = 0.0;
+  face_normal<0, 1, 2, 3>(x, y, z, nx, ny, nz);
+  face_normal<0, 4, 5, 1>(x, y, z, nx, ny, nz);
+  face_normal<1, 5, 6, 2>(x, y, z, nx, ny, nz);
+  face_normal<2, 6, 7, 3>(x, y, z, nx, ny, nz);
+  face_normal<3, 7, 4, 0>(x, y, z, nx, ny, nz);
+  face_normal<4, 7, 6, 5>(x, y, z, nx, ny, nz);
+  for (int i = 0; i < 8; ++i) {
+    f[0 * 8 + i] = -(sxx * nx[i]);
+    f[1 * 8 + i] = -(syy * ny[i]);
+    f[2 * 8 + i] = -(szz * nz[i]);
+  }
+}
+
+} // namespace lu
+
+// lulesh::StepSeq / StepGPU* (kernels/lulesh.hpp:23-313): xyz (point data 1)
+// and the force target (point data 0) through mapping 0; cell data 0 the
+// diagonal stresses, cell data 1 the element volume, which the kernel
+// writes (into the loop's copy of the cell data: see cell_output_slot).
+struct LuleshF {
+  static constexpr uint32_t MD = 8, TD = 3;
+  static constexpr bool SLOT_OUTER = true;
+  template <bool SOA, class A>
+  FOT_DEV static void compute(const A &a, uint32_t ind, const uint32_t *tgt, double *inc) {
+    const double *xyz = a.pd(1);
+    const uint32_t xs = SOA ? a.ps(1) : 1, ux = SOA ? 1 : 3;
+    double x[8], y[8], z[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) {
+      const double *c = xyz + size_t(ux) * tgt[k];
+      x[k] = c[0], y[k] = c[xs], z[k] = c[2 * size_t(xs)];
+    }
+    const double *sig = a.cd(0) + ind;
+    const uint32_t cs = a.cs();
+    double f[24], det;
+    lu::element(x, y, z, sig[0], sig[cs], sig[2 * size_t(cs)], f, &det);
+    const_cast<double *>(a.cd(1))[ind] = det;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j)
+#pragma unroll
+      for (uint32_t i = 0; i < 3; ++i) inc[j * 3 + i] = f[i * 8 + j];
+  }
+};
+
 struct VolnaF {
   static constexpr uint32_t MD = 2, TD = 4;
   static constexpr bool SLOT_OUTER = false;

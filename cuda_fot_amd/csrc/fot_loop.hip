@@ -933,6 +933,23 @@ __global__ __launch_bounds__(256) void gather_soa_kernel(double *__restrict__ ds
   for (uint32_t d = 0; d < dim; ++d) dst[size_t(d) * n_dst + i] = src[size_t(d) * n_src + s];
 }
 
+// dst[d * n_dst + ids[i]] = src[d * n_src + pos[i]] (SOA scatter of a cell
+// output back to the caller's cell order)
+__global__ __launch_bounds__(256) void scatter_soa_kernel(double *__restrict__ dst,
+                                                          const double *__restrict__ src,
+                                                          const uint32_t *__restrict__ pos,
+                                                          const uint32_t *__restrict__ ids,
+                                                          uint32_t n, uint32_t n_dst,
+                                                          uint32_t n_src, uint32_t dim) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t p = pos[i], c = ids[i];
+  for (uint32_t d = 0; d < dim; ++d) dst[size_t(d) * n_dst + c] = src[size_t(d) * n_src + p];
+}
+
+// the cell data slot a functor writes (lulesh's element volume), or -1
+int cell_output_slot(const fot_functor &f) { return f.kernel == FOT_KERNEL_LULESH ? 1 : -1; }
+
 // mappings >= 1 each functor reads through compute()
 uint32_t mapping_mask(const fot_functor &f) {
   return f.kernel == FOT_KERNEL_RES_CALC ? 0x2u : 0u;
@@ -1003,6 +1020,16 @@ struct fot_loop {
     return FOT_OK;
   }
   // SOA gather of every cell data array into a new buffer of n_dst cells
+  // a written cell data slot in a plan-order copy: (copy position, caller's
+  // cell) pairs, one per distinct cell, for the scatter back
+  struct CellOut {
+    const double *src;
+    uint32_t src_stride;
+    const uint32_t *pos, *ids;
+    uint32_t count;
+  };
+  std::vector<CellOut> cell_out;
+
   fot_status gather_cell_data(const uint32_t *host_idx, uint32_t n_dst, EngineArgs &args) {
     if (desc.num_cell_data == 0 || n_dst == 0) return FOT_OK;
     uint32_t *d_idx = nullptr;
@@ -1027,6 +1054,38 @@ struct fot_loop {
     (void)hipFree(d_idx);
     if (e != hipSuccess) return fail(FOT_ERR_HIP, "cell data gather: %s", hipGetErrorString(e));
     args.cell_stride = n_dst;
+    const int slot = cell_output_slot(f);
+    if (slot >= 0 && uint32_t(slot) < desc.num_cell_data) {
+      // owner plans repeat halo cells: scatter each cell from its first copy
+      std::vector<uint8_t> seen(desc.num_cells, 0);
+      std::vector<uint32_t> pos, ids;
+      for (uint32_t i = 0; i < n_dst; ++i)
+        if (!seen[host_idx[i]]) {
+          seen[host_idx[i]] = 1;
+          pos.push_back(i);
+          ids.push_back(host_idx[i]);
+        }
+      uint32_t *dp = nullptr, *di = nullptr;
+      fot_status st;
+      if ((st = upload(pos.data(), pos.size(), &dp)) != FOT_OK) return st;
+      if ((st = upload(ids.data(), ids.size(), &di)) != FOT_OK) return st;
+      cell_out.push_back({args.cell_data[slot], n_dst, dp, di, uint32_t(pos.size())});
+    }
+    return FOT_OK;
+  }
+
+  // writes the functor's cell output from the plan-order copies back to the
+  // caller's cell data (no-op when the loop works on the caller's array)
+  fot_status scatter_cell_outputs(hipStream_t s) {
+    const int slot = cell_output_slot(f);
+    for (const CellOut &c : cell_out) {
+      if (c.count == 0) continue;
+      hipLaunchKernelGGL(scatter_soa_kernel, dim3((c.count + 255) / 256), dim3(256), 0, s,
+                         const_cast<double *>(desc.cell_data[slot]), c.src, c.pos, c.ids, c.count,
+                         desc.num_cells, c.src_stride, desc.cell_dim[slot]);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(FOT_ERR_HIP, "cell output scatter: %s", hipGetErrorString(e));
     return FOT_OK;
   }
 
@@ -1592,6 +1651,14 @@ extern "C" fot_status fot_loop_create(const fot_functor *f, const fot_problem_de
   if ((f->kernel == FOT_KERNEL_MINE || f->kernel == FOT_KERNEL_HEX_SCATTER) &&
       (desc->num_cell_data < 1 || desc->cell_dim[0] != f->cell_dim))
     return fail(FOT_ERR_INVALID_ARGUMENT, "functor needs cell data of dim %u", f->cell_dim);
+  if (f->kernel == FOT_KERNEL_LULESH &&
+      (desc->num_mappings < 2 || desc->point_dim[1] != 3 ||
+       desc->num_points[1] != desc->num_points[0] || !desc->point_data[1] ||
+       desc->num_cell_data < 2 || desc->cell_dim[0] != 3 || desc->cell_dim[1] != 1 ||
+       !desc->cell_data[1]))
+    return fail(FOT_ERR_INVALID_ARGUMENT,
+                "lulesh needs node coordinates (point data 1, dim 3, the mapping-0 points), "
+                "sigma(3) and volume(1) cell data");
   if (f->kernel == FOT_KERNEL_GETACC &&
       (desc->num_cell_data < 5 || desc->cell_dim[0] != 4 || desc->cell_dim[1] != 1 ||
        desc->cell_dim[2] != 4 || desc->cell_dim[3] != 4 || desc->cell_dim[4] != 4))
@@ -1683,6 +1750,8 @@ extern "C" fot_status fot_loop_run(fot_loop *loop, uint32_t iters, void *stream)
     if (jacobi || it + 1 == iters)
       if ((st = fot_copy(in, d.point_data_out, bytes, stream)) != FOT_OK) return st;
   }
+  if (iters && (st = loop->scatter_cell_outputs(static_cast<hipStream_t>(stream))) != FOT_OK)
+    return st;
   FOT_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
   return fot_loop_check(loop);
 }
@@ -1864,6 +1933,8 @@ extern "C" fot_status fot_batch_run(fot_batch *b, uint32_t iters, void *stream) 
       if (st == FOT_OK && (b->jacobi || it + 1 == iters)) st = b->launch_copy(b->copy_back, s);
     }
   }
+  for (fot_loop *l : b->loops)
+    if (st == FOT_OK && iters) st = l->scatter_cell_outputs(s);
   if (st != FOT_OK) return st;
   FOT_HIP(hipStreamSynchronize(s));
   return FOT_OK;

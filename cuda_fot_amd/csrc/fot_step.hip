@@ -87,6 +87,7 @@ template <class F, bool SOA> struct HierGatherGlobalIds {
 template <bool SOA> struct HierGather<VolnaF, SOA> : HierGatherGlobalIds<VolnaF, SOA> {};
 template <bool SOA> struct HierGather<MiniAeroF, SOA> : HierGatherGlobalIds<MiniAeroF, SOA> {};
 template <bool SOA> struct HierGather<GetaccF, SOA> : HierGatherGlobalIds<GetaccF, SOA> {};
+template <bool SOA> struct HierGather<LuleshF, SOA> : HierGatherGlobalIds<LuleshF, SOA> {};
 
 __device__ uint32_t g_lds_overflow; // set when a block's cache exceeds cache_size
 
@@ -168,6 +169,9 @@ extern "C" fot_status fot_functor_init(fot_functor *f, fot_kernel kernel, uint32
   case FOT_KERNEL_GETACC: // kernels/getacc_scatter.hpp:22-23 (cell_dim: cnmass)
     f->mesh_dim = 4, f->point_dim = 4, f->cell_dim = 4;
     return FOT_OK;
+  case FOT_KERNEL_LULESH: // kernels/lulesh.hpp:7-10 (cell_dim: sigma)
+    f->mesh_dim = 8, f->point_dim = 3, f->cell_dim = 3;
+    return FOT_OK;
   case FOT_KERNEL_VOLNA: // kernels/volna.hpp:16-19 (cell_dim: the fluxes)
     f->mesh_dim = 2, f->point_dim = 4, f->cell_dim = 3;
     return FOT_OK;
@@ -248,6 +252,17 @@ extern "C" fot_status fot_lds_overflow_flag(uint32_t *flag, int clear) {
     e = hipMemcpyToSymbol(HIP_SYMBOL(g_lds_overflow), &zero, sizeof(uint32_t));
   }
   if (e != hipSuccess) return fail(FOT_ERR_HIP, "%s", hipGetErrorString(e));
+  return FOT_OK;
+}
+
+// The lulesh element function on the host (the code the kernels run): the
+// C++ drop-in's lulesh::StepSeq calls it per element.
+extern "C" fot_status fot_lulesh_element(const double *x, const double *y, const double *z,
+                                         const double *sig, uint32_t sig_stride, double *f,
+                                         double *determ) {
+  if (!x || !y || !z || !sig || !f || !determ)
+    return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
+  fot::lu::element(x, y, z, sig[0], sig[sig_stride], sig[2 * size_t(sig_stride)], f, determ);
   return FOT_OK;
 }
 

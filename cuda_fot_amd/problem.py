@@ -167,6 +167,9 @@ class Problem:
                     partition=part, use_graph=use_graph, flags=flags)
         loop.run(iters)
         self.point_data[0] = dev[0].download()
+        slot = capi.cell_output_slot(func.kernel)
+        if slot >= 0 and iters:
+            self.cell_data[slot] = cdev[slot].download()
         stats = loop.stats()
         loop.close()
         return stats
@@ -229,6 +232,31 @@ def mini_aero_problem(mesh: dict, block_size: int = DEFAULT_BLOCK_SIZE,
     cells = mesh["cells"].T.copy() if soa else mesh["cells"].copy()
     maps = [mesh["face_cells"], mesh["face_cells"]]
     return Problem(maps, [nc, nc], [flux, cells], [mesh["face"]], block_size, soa)
+
+
+def lulesh_problem(N1: int, N2: int, N3: int, seed: int = 0,
+                   block_size: int = DEFAULT_BLOCK_SIZE, soa: bool = False,
+                   shuffle: bool = True) -> Problem:
+    """lulesh as lulesh.cu:10-36 sets it up, on synthetic data in place of
+    its data_* files: fillCellList3D hexes of an N1 x N2 x N3 node lattice
+    (cells shuffled), node (l, r, c) at (r, c, l) + U(-0.2, 0.2) per
+    coordinate; two mappings, both the hexes; point data f (dim 3, N(0,1),
+    the target) and xyz (dim 3); cell data sigma (3, N(0,1)) and the element
+    volume (1, zero; the loop writes it)."""
+    from . import meshgen
+    cells = meshgen.hex_grid(N1, N2, N3)
+    if shuffle:
+        cells = cells[meshgen.permutation(cells.shape[0], seed + 1)]
+    nc, npt = cells.shape[0], N1 * N2 * N3
+    l, r, c = np.meshgrid(np.arange(N3), np.arange(N1), np.arange(N2), indexing="ij")
+    xyz = np.stack([r.ravel(), c.ravel(), l.ravel()], axis=1).astype(np.float64)
+    xyz += meshgen.uniform(3 * npt, seed + 2, -0.2, 0.2).reshape(npt, 3)
+    f = meshgen.normal(3 * npt, seed + 3).reshape(npt, 3)
+    sig = meshgen.normal(3 * nc, seed + 4).reshape(3, nc)
+    determ = np.zeros((1, nc))
+    if soa:
+        f, xyz = f.T.copy(), xyz.T.copy()
+    return Problem([cells, cells.copy()], [npt, npt], [f, xyz], [sig, determ], block_size, soa)
 
 
 def getacc_problem(N: int, M: int, seed: int = 0, block_size: int = DEFAULT_BLOCK_SIZE,

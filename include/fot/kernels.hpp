@@ -223,6 +223,48 @@ using StepGPUAtomic = fot::GPUAtomic<Desc>;
 using StepGPUHierarchical = fot::GPUHierarchical<Desc>;
 }  // namespace hex_scatter
 
+// ---- lulesh (kernels/lulesh.hpp, lulesh_func.hpp) -------------------------------
+// point data 0 the node forces (dim 3, target), point data 1 the node
+// coordinates (dim 3, read through mapping 0 as the reference does); cell
+// data 0 the diagonal stresses (3), cell data 1 the element volume, which
+// the user function writes.  The CPU struct runs libfot's host copy of the
+// element function, the same code as the kernels.
+namespace lulesh {
+constexpr unsigned MESH_DIM = 8, POINT_DIM = 3, CELL_DIM0 = 3, CELL_DIM1 = 1;
+struct Desc {
+  static fot_functor functor() { return fot::make_functor(FOT_KERNEL_LULESH, 8, 3, 3); }
+};
+struct StepSeq : Desc {
+  template <bool SOA>
+  static void call(const void **point_data, void *point_data_out, void **cell_data,
+                   const MY_SIZE **cell_to_node, MY_SIZE ind, const unsigned *point_stride,
+                   unsigned cell_stride) {
+    const double *xyz = static_cast<const double *>(point_data[1]);
+    double *f = static_cast<double *>(point_data_out);
+    const double *sig = static_cast<const double *>(cell_data[0]) + ind;
+    double *determ = static_cast<double *>(cell_data[1]) + ind;
+    const unsigned xs = SOA ? point_stride[1] : 1, fs = SOA ? point_stride[0] : 1;
+    double x[MESH_DIM], y[MESH_DIM], z[MESH_DIM], inc[POINT_DIM * MESH_DIM];
+    for (unsigned k = 0; k < MESH_DIM; ++k) {
+      const MY_SIZE p = cell_to_node[0][MESH_DIM * ind + k];
+      const double *c = xyz + (SOA ? p : POINT_DIM * p);
+      x[k] = c[0], y[k] = c[xs], z[k] = c[2 * std::size_t(xs)];
+    }
+    fot::check(fot_lulesh_element(x, y, z, sig, cell_stride, inc, determ),
+               "fot_lulesh_element");
+    for (unsigned j = 0; j < MESH_DIM; ++j) {
+      const MY_SIZE p = cell_to_node[0][MESH_DIM * ind + j];
+      for (unsigned i = 0; i < POINT_DIM; ++i)
+        f[std::size_t(SOA ? p : POINT_DIM * p) + i * fs] += inc[i * MESH_DIM + j];
+    }
+  }
+};
+using StepOMP = StepSeq;
+using StepGPUGlobal = fot::GPUGlobal<Desc>;
+using StepGPUAtomic = fot::GPUAtomic<Desc>;
+using StepGPUHierarchical = fot::GPUHierarchical<Desc>;
+}  // namespace lulesh
+
 // ---- volna (kernels/volna.hpp, volna_func.hpp) in FP64 ---------------------
 // point data 0 the state (dim 4, target), point data 1 the cell volume;
 // cell data fluxes (3), bathymetry source (2), normals (2) and the

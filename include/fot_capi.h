@@ -113,7 +113,12 @@ typedef enum {
   /* corner scatter of a 2D quad mesh, kernels/getacc_scatter.hpp +
    * getacc_scatter_func.hpp: 4 points of dim 4 (ndmass, ndarea, ndub, ndvb);
    * cell data cnmass (4), rho (1), cnwt (4), cnfx (4), cnfy (4) */
-  FOT_KERNEL_GETACC = 5
+  FOT_KERNEL_GETACC = 5,
+  /* lulesh element forces, kernels/lulesh.hpp + lulesh_func.hpp:13-38:
+   * 8 points of dim 3 (the forces, target); point data 1 the node
+   * coordinates (dim 3, same point set, read through mapping 0); cell data 0
+   * the diagonal stresses (3), cell data 1 the element volume (1), written */
+  FOT_KERNEL_LULESH = 6
 } fot_kernel;
 
 typedef struct {
@@ -187,6 +192,15 @@ fot_status fot_step_gpu_hierarchical(
 fot_status fot_mini_aero_face_flux(const double *left, const double *right,
                                    uint32_t cell_stride, const double *face,
                                    uint32_t face_stride, double *flux);
+
+/* lulesh's element function (kernels/lulesh_func.hpp:13-38, the reference's
+ * user_func_host) on the host, the same code the kernels run: corner
+ * coordinates x, y, z [8], stresses sig[0], sig[sig_stride],
+ * sig[2 * sig_stride] -> forces f[i * 8 + j] (component i, corner j) and the
+ * element volume. */
+fot_status fot_lulesh_element(const double *x, const double *y, const double *z,
+                              const double *sig, uint32_t sig_stride, double *f,
+                              double *determ);
 
 /* Reads (and with clear != 0 resets) the flag a hierarchical launch sets
  * when a block's points do not fit cache_size; such blocks are skipped.  No

@@ -320,6 +320,68 @@ static void ma_face(const double *L, const double *R, uint32_t s, const double *
   for (int c = 0; c < 5; ++c) flux[c] -= vf[c];
 }
 
+/* lulesh (kernels/lulesh_func.hpp:13-38 with kernels/lulesh_utils.hpp).
+ * The volume comes from the shape-function derivatives at the element
+ * centre (CalcElemShapeFunctionDerivatives, lulesh_utils.hpp:4-93), whose
+ * derivative values the reference then overwrites with the node normals:
+ * only the three cofactors of the eta column enter the volume.  Node normals
+ * (CalcElemNodeNormals, :95-183): each of the six faces, in the reference's
+ * order and node order, adds a quarter of the cross product of its two
+ * bisectors to each of its four nodes.  Forces (SumElemStressesToNodeForces,
+ * :185-197): f = -(sigma * normal), component by component.  Expressions
+ * keep the reference's association so the results are bit-identical. */
+void oracle_lulesh_func(const double *x, const double *y, const double *z, double sxx,
+                        double syy, double szz, double *f, double *determ) {
+  const double fjxxi = .125 * ((x[6] - x[0]) + (x[5] - x[3]) - (x[7] - x[1]) - (x[4] - x[2]));
+  const double fjxet = .125 * ((x[6] - x[0]) - (x[5] - x[3]) + (x[7] - x[1]) - (x[4] - x[2]));
+  const double fjxze = .125 * ((x[6] - x[0]) + (x[5] - x[3]) + (x[7] - x[1]) + (x[4] - x[2]));
+  const double fjyxi = .125 * ((y[6] - y[0]) + (y[5] - y[3]) - (y[7] - y[1]) - (y[4] - y[2]));
+  const double fjyet = .125 * ((y[6] - y[0]) - (y[5] - y[3]) + (y[7] - y[1]) - (y[4] - y[2]));
+  const double fjyze = .125 * ((y[6] - y[0]) + (y[5] - y[3]) + (y[7] - y[1]) + (y[4] - y[2]));
+  const double fjzxi = .125 * ((z[6] - z[0]) + (z[5] - z[3]) - (z[7] - z[1]) - (z[4] - z[2]));
+  const double fjzet = .125 * ((z[6] - z[0]) - (z[5] - z[3]) + (z[7] - z[1]) - (z[4] - z[2]));
+  const double fjzze = .125 * ((z[6] - z[0]) + (z[5] - z[3]) + (z[7] - z[1]) + (z[4] - z[2]));
+  const double cjxet = -(fjyxi * fjzze) + (fjzxi * fjyze);
+  const double cjyet = (fjxxi * fjzze) - (fjzxi * fjxze);
+  const double cjzet = -(fjxxi * fjyze) + (fjyxi * fjxze);
+  *determ = 8. * (fjxet * cjxet + fjyet * cjyet + fjzet * cjzet);
+
+  static const int face[6][4] = {{0, 1, 2, 3}, {0, 4, 5, 1}, {1, 5, 6, 2},
+                                 {2, 6, 7, 3}, {3, 7, 4, 0}, {4, 7, 6, 5}};
+  double nx[8], ny[8], nz[8];
+  for (int i = 0; i < 8; ++i) nx[i] = ny[i] = nz[i] = 0.0;
+  for (int q = 0; q < 6; ++q) {
+    const int a = face[q][0], b = face[q][1], c = face[q][2], d = face[q][3];
+    const double b0x = 0.5 * (x[d] + x[c] - x[b] - x[a]);
+    const double b0y = 0.5 * (y[d] + y[c] - y[b] - y[a]);
+    const double b0z = 0.5 * (z[d] + z[c] - z[b] - z[a]);
+    const double b1x = 0.5 * (x[c] + x[b] - x[d] - x[a]);
+    const double b1y = 0.5 * (y[c] + y[b] - y[d] - y[a]);
+    const double b1z = 0.5 * (z[c] + z[b] - z[d] - z[a]);
+    const double ax = 0.25 * (b0y * b1z - b0z * b1y);
+    const double ay = 0.25 * (b0z * b1x - b0x * b1z);
+    const double az = 0.25 * (b0x * b1y - b0y * b1x);
+    const int nodes[4] = {a, b, c, d};
+    for (int k = 0; k < 4; ++k) {
+      nx[nodes[k]] += ax;
+      ny[nodes[k]] += ay;
+      nz[nodes[k]] += az;
+    }
+  }
+  for (int i = 0; i < 8; ++i) {
+    f[0 * 8 + i] = -(sxx * nx[i]);
+    f[1 * 8 + i] = -(syy * ny[i]);
+    f[2 * 8 + i] = -(szz * nz[i]);
+  }
+}
+
+/* Kernels whose increments the reference adds slot by slot (corner-major):
+ * lulesh's StepSeq (kernels/lulesh.hpp:52-56) and the scatter-only hex
+ * functor behind the same API. */
+static inline int slot_outer(const oracle_problem *p) {
+  return p->kernel == ORACLE_HEX_SCATTER || p->kernel == ORACLE_LULESH;
+}
+
 /* Index of element (point i, component d) of a point set (data_t.hpp:104-115). */
 static inline size_t pidx(int soa, uint32_t n, uint32_t i, uint32_t dim,
                           uint32_t d) {
@@ -387,6 +449,28 @@ static void compute_inc(const oracle_problem *p, uint32_t ind, double *inc) {
     for (uint32_t j = 0; j < 8; ++j)
       for (uint32_t i = 0; i < 3; ++i)
         inc[j * 3 + i] = F[(size_t)(i * 8 + j) * cs];
+    break;
+  }
+  case ORACLE_LULESH: {
+    /* lulesh::StepSeq (kernels/lulesh.hpp:23-58): xyz (point data 1) and the
+     * force target through mapping 0; cell data 0 = sigma (xx, yy, zz), cell
+     * data 1 = the element volume, written by the user function. */
+    const uint32_t *m0 = p->cell_to_node[0];
+    const double *xyz = p->point_data[1];
+    const uint32_t xs = soa ? p->num_points[1] : 1, ux = soa ? 1 : 3;
+    const uint32_t cs = p->num_cells;
+    double xv[8], yv[8], zv[8], fc[24], determ;
+    for (int k = 0; k < 8; ++k) {
+      const double *c = xyz + (size_t)ux * m0[8 * ind + k];
+      xv[k] = c[0];
+      yv[k] = c[xs];
+      zv[k] = c[2 * (size_t)xs];
+    }
+    const double *sig = p->cell_data[0] + ind;
+    oracle_lulesh_func(xv, yv, zv, sig[0], sig[cs], sig[2 * (size_t)cs], fc, &determ);
+    ((double *)p->cell_data[1])[ind] = determ; /* the kernel's cell output */
+    for (int j = 0; j < 8; ++j)
+      for (int i = 0; i < 3; ++i) inc[j * 3 + i] = fc[i * 8 + j];
     break;
   }
   case ORACLE_GETACC: {
@@ -464,7 +548,7 @@ static void add_inc(const oracle_problem *p, uint32_t ind, const double *inc,
   const uint32_t md = p->mesh_dim[0], P = p->point_dim[0];
   const uint32_t n = p->num_points[0];
   const uint32_t *m0 = p->cell_to_node[0];
-  if (p->kernel == ORACLE_HEX_SCATTER) { /* slot outer (lulesh.hpp:52-56) */
+  if (slot_outer(p)) { /* slot outer (lulesh.hpp:52-56) */
     for (uint32_t k = 0; k < md; ++k)
       for (uint32_t d = 0; d < P; ++d) {
         const size_t g = pidx(p->soa, n, m0[md * ind + k], P, d);
@@ -605,7 +689,7 @@ int oracle_loop_hier_replay(const oracle_problem *p, const oracle_hplan *h,
             if (h->cell_colours[pos] != tc) continue;
             const uint32_t cell = h->cell_inv_permutation[pos];
             compute_inc(p, cell, inc);
-            if (p->kernel == ORACLE_HEX_SCATTER) {
+            if (slot_outer(p)) {
               for (uint32_t k = 0; k < md; ++k)
                 for (uint32_t d = 0; d < P; ++d) {
                   const uint32_t l = h->cell_list[0][(size_t)md * c0 +
@@ -669,7 +753,7 @@ int oracle_loop_owner_replay(const oracle_problem *p, const oracle_oplan *o,
           if (o->colour[pos] != tc) continue;
           compute_inc(p, o->cells[pos], inc);
           const uint32_t *l = o->loc + (size_t)pos * md;
-          if (p->kernel == ORACLE_HEX_SCATTER) {
+          if (slot_outer(p)) {
             for (uint32_t k = 0; k < md; ++k)
               for (uint32_t d = 0; d < P; ++d)
                 if (l[k] < no) cache[(size_t)d * no + l[k]] += inc[k * P + d];

@@ -36,7 +36,8 @@ enum {
   ORACLE_HEX_SCATTER = 4, /* lulesh struct API, scatter-only functor  */
   ORACLE_MINI_AERO = 5, /* kernels/mini_aero_func.hpp:12-85           */
   ORACLE_VOLNA = 6,     /* kernels/volna_func.hpp:16-36 in FP64       */
-  ORACLE_GETACC = 7     /* kernels/getacc_scatter_func.hpp:17-94      */
+  ORACLE_GETACC = 7,    /* kernels/getacc_scatter_func.hpp:17-94      */
+  ORACLE_LULESH = 8     /* kernels/lulesh_func.hpp:13-38              */
 };
 
 /*
@@ -66,6 +67,12 @@ void oracle_res_calc_func(const double *x1, const double *x2, const double *q1,
                           const double *q2, const double *adt1,
                           const double *adt2, double *res1, double *res2,
                           unsigned x_stride, unsigned q_stride);
+
+/* lulesh's element function (kernels/lulesh_func.hpp:13-38): corner
+ * coordinates x, y, z [8] and the diagonal stresses give the corner forces
+ * f[i * 8 + j] (component i, corner j) and the element volume *determ. */
+void oracle_lulesh_func(const double *x, const double *y, const double *z, double sxx,
+                        double syy, double szz, double *f, double *determ);
 
 /* ---- loops ------------------------------------------------------------- */
 /* Serial loop, cells in index order, Jacobi double buffer

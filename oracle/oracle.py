@@ -14,7 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "liboracle.so")
 
-RES_CALC, MINE2, MINE4, MINE8, HEX_SCATTER, MINI_AERO, VOLNA, GETACC = range(8)
+RES_CALC, MINE2, MINE4, MINE8, HEX_SCATTER, MINI_AERO, VOLNA, GETACC, LULESH = range(9)
 MAXM, MAXCD = 8, 8
 
 
@@ -58,6 +58,7 @@ def _load():
     V, U = C.c_void_p, C.c_uint32
     sig = {
         "oracle_res_calc_func": [V] * 8 + [C.c_uint, C.c_uint],
+        "oracle_lulesh_func": [V, V, V, C.c_double, C.c_double, C.c_double, V, V],
         "oracle_loop_serial": [V, U, V, V],
         "oracle_loop_colour": [V, U, C.c_int, V],
         "oracle_loop_hier_replay": [V, V, U],
@@ -83,6 +84,7 @@ def _load():
     lib.oracle_cell_to_cell_graph.restype = C.c_uint64
     lib.oracle_hplan_free.restype = None
     lib.oracle_res_calc_func.restype = None
+    lib.oracle_lulesh_func.restype = None
     return lib
 
 
@@ -109,7 +111,8 @@ class OracleProblem:
         self.maps = [_u32(m) for m in maps]
         self.point_data = [None if d is None else np.array(d, dtype=np.float64, order="C")
                            for d in point_data]
-        self.cell_data = [np.ascontiguousarray(c, dtype=np.float64) for c in cell_data]
+        # copies: lulesh writes its element volume into cell data 1
+        self.cell_data = [np.array(c, dtype=np.float64, order="C") for c in cell_data]
         s = Problem()
         s.kernel, s.soa = kernel, int(soa)
         s.num_cells = self.maps[0].shape[0]
@@ -243,6 +246,18 @@ def res_calc_func(x1, x2, q1, q2, adt1, adt2):
     r2 = np.zeros(4)
     lib.oracle_res_calc_func(*[_p(v) for v in ins], _p(a1), _p(a2), _p(r1), _p(r2), 1, 1)
     return r1, r2
+
+
+def lulesh_func(xyz, sig):
+    """lulesh's element function on one hex: xyz (8, 3) corner coordinates,
+    sig (3,) diagonal stresses -> (forces (8, 3), volume)."""
+    c = np.asarray(xyz, dtype=np.float64)
+    x, y, z = (np.ascontiguousarray(c[:, d]) for d in range(3))
+    f = np.zeros(24)
+    det = np.zeros(1)
+    lib.oracle_lulesh_func(_p(x), _p(y), _p(z), float(sig[0]), float(sig[1]), float(sig[2]),
+                           _p(f), _p(det))
+    return f.reshape(3, 8).T.copy(), float(det[0])
 
 
 def colour_cells(maps0, num_points):

@@ -105,6 +105,38 @@ template <bool SOA> Problem<SOA> hex_grid() {
   return p;
 }
 
+// lulesh on a 5 x 4 x 3 node lattice: node (l, r, c) at (r, c, l) +
+// U(-0.2, 0.2) (seed 21), forces N(0,1) (seed 22), sigma N(0,1) (seed 23),
+// volumes zero; both mappings the hexes (lulesh.cu:10-24)
+template <bool SOA> Problem<SOA> lulesh_grid() {
+  const std::uint32_t n1 = 5, n2 = 4, n3 = 3, np = n1 * n2 * n3,
+                      nc = (n1 - 1) * (n2 - 1) * (n3 - 1);
+  std::vector<std::uint32_t> c(8 * nc);
+  fot::check(fot_gen_hex_grid(n1, n2, n3, c.data()), "fot_gen_hex_grid");
+  Problem<SOA> p(Mesh({np, np}, nc, {c.data(), c.data()}, {8, 8}),
+                 {{3, sizeof(double)}, {3, sizeof(double)}},
+                 {{3, sizeof(double)}, {1, sizeof(double)}}, 128);
+  std::vector<double> jit(3 * np), f(3 * np), sig(3 * nc);
+  fot::check(fot_gen_uniform(jit.size(), 21, -0.2, 0.2, jit.data()), "fot_gen_uniform");
+  fot::check(fot_gen_normal(f.size(), 22, 0.0, 1.0, f.data()), "fot_gen_normal");
+  fot::check(fot_gen_normal(sig.size(), 23, 0.0, 1.0, sig.data()), "fot_gen_normal");
+  for (std::uint32_t l = 0; l < n3; ++l)
+    for (std::uint32_t r = 0; r < n1; ++r)
+      for (std::uint32_t cc = 0; cc < n2; ++cc) {
+        const MY_SIZE i = l * n1 * n2 + r * n2 + cc;
+        const double base[3] = {double(r), double(cc), double(l)};
+        for (MY_SIZE d = 0; d < 3; ++d) {
+          p.point_weights[0].template operator[]<double>(index<SOA>(np, i, 3, d)) = f[3 * i + d];
+          p.point_weights[1].template operator[]<double>(index<SOA>(np, i, 3, d)) =
+              base[d] + jit[3 * i + d];
+        }
+      }
+  std::copy(sig.begin(), sig.end(), p.cell_weights[0].template begin<double>());
+  std::fill(p.cell_weights[1].template begin<double>(),
+            p.cell_weights[1].template begin<double>() + nc, 0.0);
+  return p;
+}
+
 // volna on a 14 x 9 triangle mesh (seed 4), mini_aero on 5 x 4 x 3 hexes (seed 2)
 template <bool SOA> Problem<SOA> volna_mesh() {
   const std::uint32_t nx = 14, ny = 9, nc = 2 * nx * ny;
@@ -246,6 +278,16 @@ template <bool SOA> void cpu_cases(const char *tag) {
     p.template loopCPUCellCentred<mini_aero::StepSeq>(2);
     dump(p, std::string("ma_serial_") + tag);
   }
+  {
+    auto p = lulesh_grid<SOA>();
+    p.template loopCPUCellCentred<lulesh::StepSeq>(2);
+    dump(p, std::string("lulesh_serial_") + tag);
+  }
+  {
+    auto p = lulesh_grid<SOA>();
+    p.template loopCPUCellCentredOMP<lulesh::StepOMP>(2);
+    dump(p, std::string("lulesh_omp_") + tag);
+  }
 }
 
 template <bool SOA> void gpu_cases(const char *tag) {
@@ -298,6 +340,21 @@ template <bool SOA> void gpu_cases(const char *tag) {
     auto p = mini_aero_mesh<SOA>();
     p.template loopGPUHierarchical<mini_aero::StepGPUHierarchical>(2);
     dump(p, "ma_owner_" + t);
+  }
+  {
+    auto p = lulesh_grid<SOA>();
+    p.template loopGPUCellCentred<lulesh::StepGPUGlobal>(2);
+    dump(p, "lulesh_global_" + t);
+  }
+  {
+    auto p = lulesh_grid<SOA>();
+    p.template loopGPUEdgeCentred<lulesh::StepGPUAtomic>(2);
+    dump(p, "lulesh_atomic_" + t);
+  }
+  {
+    auto p = lulesh_grid<SOA>();
+    p.template loopGPUHierarchical<lulesh::StepGPUHierarchical>(2);
+    dump(p, "lulesh_owner_" + t);
   }
 }
 

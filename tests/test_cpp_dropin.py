@@ -64,6 +64,21 @@ def _mini_aero_oracle():
     return lambda: O.OracleProblem(O.MINI_AERO, p.maps, p.num_points, p.point_data, p.cell_data)
 
 
+def _lulesh_oracle():
+    # the driver's lulesh: 5 x 4 x 3 node lattice, jitter seed 21, forces
+    # seed 22, sigma seed 23, volumes zero
+    n1, n2, n3 = 5, 4, 3
+    c = meshgen.hex_grid(n1, n2, n3)
+    npt, nc = n1 * n2 * n3, c.shape[0]
+    l, r, cc = np.meshgrid(np.arange(n3), np.arange(n1), np.arange(n2), indexing="ij")
+    xyz = np.stack([r.ravel(), cc.ravel(), l.ravel()], axis=1).astype(np.float64)
+    xyz += meshgen.uniform(3 * npt, 21, -0.2, 0.2).reshape(npt, 3)
+    f = meshgen.normal(3 * npt, 22).reshape(npt, 3)
+    sig = meshgen.normal(3 * nc, 23).reshape(3, nc)
+    return lambda: O.OracleProblem(O.LULESH, [c, c], [npt, npt], [f, xyz],
+                                   [sig, np.zeros((1, nc))])
+
+
 def _serial(make, iters):
     o = make()
     S = o.loop_serial(iters, with_scale=True)
@@ -103,7 +118,11 @@ def test_cpp_dropin_cpu_loops(built, tmp_path):
     hs, _ = _serial(hexp, ITERS_HEX)
     vs, _ = _serial(_volna_oracle(), 2)
     mas, _ = _serial(_mini_aero_oracle(), 2)
+    ls, _ = _serial(_lulesh_oracle(), 2)
+    lc = _colour(_lulesh_oracle(), 2)
     for tag in ("aos", "soa"):
+        assert_bitwise(_load(tmp_path, "lulesh_serial_" + tag, 3), ls, "C++ lulesh " + tag)
+        assert_bitwise(_load(tmp_path, "lulesh_omp_" + tag, 3), lc, "C++ lulesh OMP " + tag)
         assert_bitwise(_load(tmp_path, "mine_serial_" + tag, 4), ms, "C++ mine " + tag)
         assert_bitwise(_load(tmp_path, "hex_serial_" + tag, 3), hs, "C++ hex " + tag)
         assert_bitwise(_load(tmp_path, "volna_serial_" + tag, 4), vs, "C++ volna " + tag)
@@ -122,7 +141,12 @@ def test_cpp_dropin_gpu_loops(gpu, tmp_path):
     hc = _colour(hexp, ITERS_HEX)
     vs, vS = _serial(_volna_oracle(), 2)
     mas, maS = _serial(_mini_aero_oracle(), 2)
+    ls, lS = _serial(_lulesh_oracle(), 2)
+    lc = _colour(_lulesh_oracle(), 2)
     for tag in ("aos", "soa"):
+        assert_bitwise(_load(tmp_path, "lulesh_global_" + tag, 3), lc, "lulesh global " + tag)
+        for name in ("lulesh_atomic_", "lulesh_owner_"):
+            assert_close_scaled(_load(tmp_path, name + tag, 3), ls, lS, what=name + tag)
         assert_bitwise(_load(tmp_path, "res_global_" + tag, 4), colour, "global " + tag)
         for name in ("res_atomic_", "res_owner_", "res_hier_"):
             assert_close_scaled(_load(tmp_path, name + tag, 4), serial, S, what=name + tag)
