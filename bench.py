@@ -55,8 +55,10 @@ def parse():
                     choices=["none", "kway", "ours", "gps", "points"],
                     help="points: owner partition of mapping-0 points (owner compute)")
     ap.add_argument("--points-per-block", type=int, default=240)
-    ap.add_argument("--tune", default="none", choices=["none", "exhaustive", "learned"],
-                    help="volna-batch: per-mesh partition size from the autotuner")
+    ap.add_argument("--tune", default=None, choices=["none", "exhaustive", "learned"],
+                    help="res_calc (default learned): the configuration the autotuner "
+                         "picks for the airfoil as generated; volna-batch (default none): "
+                         "each mesh's partition size")
     ap.add_argument("--ept", type=int, default=2, choices=[1, 2],
                     help="owner compute: cells per thread")
     ap.add_argument("--no-stage", action="store_true")
@@ -262,7 +264,7 @@ def bench_volna_batch(args):
         # candidates (parts of 128 / 256 / 512 points)
         from cuda_fot_amd import tuner
         owner = [3, 4, 5]
-        sizes = {3: 128, 4: 256, 5: 512}
+        sizes = dict(zip(owner, tuner.owner_points(f)))
         measured = 0
         model = None
         if args.tune == "learned":
@@ -349,6 +351,48 @@ def bench_volna_batch(args):
                         "threads": st["threads"], "lds_bytes": st["lds_bytes"]})
 
 
+def tune_res_calc(args) -> dict:
+    """The autotuner's choice for the airfoil as generated (shuffled, not
+    reordered): the learned cost model's pick, measured, with the fallback
+    to the measured search when it misses its prediction by more than 5 %
+    (or every candidate measured, with --tune exhaustive).  Sets args to the
+    chosen configuration; an owner-compute pick runs on the mesh partitioned
+    and renumbered for its part size.  Returns the tuning record."""
+    from cuda_fot_amd import capi, tuner
+    _, p = build_problem(args, "none")
+    f = capi.functor(capi.KERNEL_RES_CALC)
+    rec = {"mode": args.tune}
+    if args.tune == "learned":
+        rows, X, T = tuner.load_corpus(tuner.CORPUS_PATH)
+        model = tuner.CostModel(seed=0)
+        model.fit(X, T)
+        r = tuner.Tuner(model).learned(p, f, warmup=2, steps=5)
+        pick = r["pick"]
+        rec.update(corpus_rows=len(rows), fell_back=bool(r["fell_back"]),
+                   measured_candidates=int(r["measured"]),
+                   predicted_ms={tuner.candidate_name(f, i): round(float(v), 4)
+                                 for i, v in enumerate(r["predicted"])},
+                   measured_pick_ms=round(float(r["measured_pick"]), 4))
+    else:
+        pick, t = tuner.Tuner.exhaustive(p, f, warmup=2, steps=5)
+        rec.update(measured_candidates=len(t),
+                   measured_ms={tuner.candidate_name(f, i): round(float(v), 4)
+                                for i, v in enumerate(t)})
+    rec["pick"] = tuner.candidate_name(f, pick)
+    if pick == 0:
+        args.strategy, args.partition = "atomic", "none"
+    elif pick == 1:
+        args.strategy, args.partition, args.block_size = "global", "none", 256
+    elif pick == 2:
+        args.strategy, args.order, args.partition, args.block_size = \
+            "hierarchical", "launch", "none", 256
+    else:
+        args.strategy, args.order, args.partition = "hierarchical", "owner", "points"
+        args.points_per_block = tuner.owner_points(f)[pick - 3]
+        args.ept, args.accumulate, args.nodes = 2, "gather", "gather"
+    return rec
+
+
 def bench_single(args, workload):
     """mini_aero 96^3 faces, hex scatter or lulesh forces on 128^3 elements:
     owner compute on a point-partitioned mesh beside plain atomics."""
@@ -421,11 +465,16 @@ def main():
         print("no HIP device", file=sys.stderr)
         sys.exit(1)
     if args.workload == "volna-batch":
+        args.tune = args.tune or "none"
         print(json.dumps(bench_volna_batch(args)), flush=True)
         return
     if args.workload in ("mini_aero", "hex", "lulesh"):
         print(json.dumps(bench_single(args, args.workload)), flush=True)
         return
+    tune = None
+    if (args.tune or "learned") != "none":
+        args.tune = args.tune or "learned"
+        tune = tune_res_calc(args)
     t_plan = time.perf_counter()
     mesh, p = build_problem(args, args.partition)
     loop, dev, out = make_loop(p, args.strategy, args.block_size, not args.no_stage,
@@ -473,7 +522,7 @@ def main():
                    "threads_per_block": stats["threads_per_block"],
                    "stage_per_thread": stats["stage_per_thread"],
                    "persistent": args.persistent, "launch_grid": stats["launch_grid"],
-                   "plan_seconds": round(plan_s, 2)},
+                   "plan_seconds": round(plan_s, 2), "tune": tune},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,
                      "algorithmic_bytes_per_launch": ubytes,

@@ -5,9 +5,16 @@ Six candidate strategies per loop (the choices the engine offers):
   0 atomic       FP64 global atomics, 256 threads
   1 global       global colouring, 256 threads
   2 hier256      hierarchical, per-block-colour launches, 256 cells per block
-  3 owner128     owner-compute hierarchical, parts of 128 points
-  4 owner256     owner-compute hierarchical, parts of 256 points
-  5 owner512     owner-compute hierarchical, parts of 512 points
+  3 owner small  owner-compute hierarchical, parts of 128 points
+  4 owner medium owner-compute hierarchical, parts of 256 points
+  5 owner large  owner-compute hierarchical, parts of 512 points
+For res_calc the owner candidates run the gather kernel with two edges per
+thread on parts of 120 / 240 / 480 points: about 2.13 edges per point, so
+those parts fill 128 / 256 / 512 threads exactly (256 points would round up
+to 320 threads).  They are measured as they are deployed: on a copy of the
+mesh partitioned into parts of that size and renumbered so that each part's
+points are contiguous (Problem.partition_points), which that kernel needs to
+address its owned cells directly.
 
 `measure` is the exhaustive search: every candidate planned and timed on the
 GPU (fot_autotune_measure).  `CostModel` is an FP32 MLP 24 -> 64 -> 64 -> 6
@@ -44,14 +51,32 @@ HELDOUT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
 NUM_FEATURES = 24
 
 
+def owner_points(func: capi.Functor) -> tuple[int, int, int]:
+    """Points per part of the three owner-compute candidates."""
+    return (120, 240, 480) if func.kernel == capi.KERNEL_RES_CALC else (128, 256, 512)
+
+
+def owner_flags(func: capi.Functor) -> int:
+    """Loop flags of the owner-compute candidates."""
+    if func.kernel == capi.KERNEL_RES_CALC:
+        return capi.LOOP_OWNER_COMPUTE | capi.LOOP_TWO_PER_THREAD | capi.LOOP_OWNER_GATHER
+    return capi.LOOP_OWNER_COMPUTE
+
+
+def candidate_name(func: capi.Functor, i: int) -> str:
+    if i < 3:
+        return NAMES[i]
+    return f"owner{owner_points(func)[i - 3]}"
+
+
 def candidate_configs(func: capi.Functor):
     """The six fot_loop_config candidates for a functor."""
     res_calc = func.kernel == capi.KERNEL_RES_CALC
-    own = capi.LOOP_OWNER_COMPUTE | (capi.LOOP_TWO_PER_THREAD if res_calc else 0)
+    own, pts = owner_flags(func), owner_points(func)
     spec = [(capi.STRATEGY_ATOMIC, 256, 0), (capi.STRATEGY_GLOBAL_COLOUR, 256, 0),
             (capi.STRATEGY_HIERARCHICAL, 256, capi.LOOP_STAGE_INPUTS if res_calc else 0),
-            (capi.STRATEGY_HIERARCHICAL, 128, own), (capi.STRATEGY_HIERARCHICAL, 256, own),
-            (capi.STRATEGY_HIERARCHICAL, 512, own)]
+            (capi.STRATEGY_HIERARCHICAL, pts[0], own), (capi.STRATEGY_HIERARCHICAL, pts[1], own),
+            (capi.STRATEGY_HIERARCHICAL, pts[2], own)]
     cfgs = (capi.LoopConfig * len(spec))()
     for i, (s, bs, fl) in enumerate(spec):
         cfgs[i].strategy, cfgs[i].block_size, cfgs[i].flags = s, bs, fl
@@ -91,19 +116,43 @@ class _Uploaded:
                                           self.cdev, problem.soa)
 
 
+def renumbered_owner(func: capi.Functor) -> bool:
+    """Whether the owner candidates run on the mesh renumbered for their
+    partition (res_calc's gather kernel)."""
+    return func.kernel == capi.KERNEL_RES_CALC
+
+
+def _measure_cfgs(problem, func, cfgs_list, warmup, steps, partition=None):
+    up = _Uploaded(problem)
+    cfgs = (capi.LoopConfig * len(cfgs_list))(*cfgs_list)
+    keep = None
+    if partition is not None:
+        keep = capi.u32(partition)
+        for c in cfgs:
+            c.partition = ptr(keep)
+            c.flags |= capi.LOOP_POINT_PARTITION
+    t = np.empty(len(cfgs_list), np.float32)
+    call("fot_autotune_measure", C.byref(func), C.byref(up.desc), cfgs, len(cfgs_list), warmup,
+         steps, ptr(t))
+    return t
+
+
 def measure(problem, func: capi.Functor, which=None, warmup: int = 3,
             steps: int = 10) -> np.ndarray:
     """ms per step of the candidates `which` (default all); -1 where a
     candidate cannot be planned, NaN where it was not measured."""
-    up = _Uploaded(problem)
+    import copy
     all_cfgs = candidate_configs(func)
     idx = list(range(len(NAMES))) if which is None else list(which)
-    cfgs = (capi.LoopConfig * len(idx))(*[all_cfgs[i] for i in idx])
-    t = np.empty(len(idx), np.float32)
-    call("fot_autotune_measure", C.byref(func), C.byref(up.desc), cfgs, len(idx), warmup,
-         steps, ptr(t))
     out = np.full(len(NAMES), np.nan, np.float32)
-    out[idx] = t
+    own = [i for i in idx if i >= 3] if renumbered_owner(func) else []
+    plain = [i for i in idx if i not in own]
+    if plain:
+        out[plain] = _measure_cfgs(problem, func, [all_cfgs[i] for i in plain], warmup, steps)
+    for i in own:
+        q = copy.deepcopy(problem)
+        q.partition_points(owner_points(func)[i - 3])
+        out[i] = _measure_cfgs(q, func, [all_cfgs[i]], warmup, steps, q.point_partition)[0]
     return out
 
 
@@ -174,7 +223,7 @@ def fallback_pick(pred_ms, measured_ms, tolerance: float = 0.05):
 
 
 # ---- evaluation ---------------------------------------------------------------
-DEFAULT_CHOICE = 4  # the method without the learned part: owner256 everywhere
+DEFAULT_CHOICE = 4  # the method without the learned part: the medium owner parts everywhere
 
 
 def evaluate(model: CostModel, X, T, tolerance: float = 0.05) -> dict:
@@ -388,6 +437,11 @@ if __name__ == "__main__":
     elif len(sys.argv) > 1 and sys.argv[1] == "corpus-extra":
         # python -m cuda_fot_amd.tuner corpus-extra OUT.jsonl
         measure_specs(corpus_extra_specs(), sys.argv[2])
+    elif len(sys.argv) > 1 and sys.argv[1] == "remeasure":
+        # python -m cuda_fot_amd.tuner remeasure FAMILY OUT.jsonl: measure the
+        # seed-0 corpus meshes of one family again (after a candidate change)
+        measure_specs([sp for sp in corpus_meshes(500, 0) if sp["family"] == sys.argv[2]],
+                      sys.argv[3])
     elif len(sys.argv) > 1 and sys.argv[1] == "heldout":
         # python -m cuda_fot_amd.tuner heldout OUT.jsonl: measure the held-out set
         measure_specs(heldout_specs(), sys.argv[2])
