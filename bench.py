@@ -189,7 +189,8 @@ def load_traffic(args):
     """HBM bytes per launch from the committed rocprofv3 counter summary
     (FETCH_SIZE and WRITE_SIZE, scaled by a copy-kernel calibration) when it
     was taken on this configuration; else None."""
-    path = os.path.join(ROOT, "profiles", f"res_calc_owner_airfoil_{args.nx}x{args.ny}_traffic.json")
+    path = os.path.join(ROOT, "profiles", f"res_calc_owner_airfoil_{args.nx}x{args.ny}_"
+                        f"ppb{args.points_per_block}_traffic.json")
     if not os.path.exists(path):
         return None
     t = json.load(open(path))
