@@ -40,10 +40,11 @@ def parse():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="res_calc",
-                    choices=["res_calc", "volna-batch", "mini_aero", "hex", "lulesh"],
+                    choices=["res_calc", "volna-batch", "mini_aero", "hex", "lulesh", "mine"],
                     help="res_calc: the airfoil edge loop (the headline line); the others "
                          "print their own line: 64 batched Volna meshes, mini_aero 96^3, "
-                         "hex scatter 128^3, lulesh element forces 128^3")
+                         "hex scatter 128^3, lulesh element forces 128^3, mine on the "
+                         "1024^2 grid with its serial and OpenMP CPU loops")
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nx", type=int, default=1200)
@@ -234,6 +235,33 @@ def cpu_baseline(mesh, seconds):
             "serial_edges_per_s": mesh["num_edges"] / serial_per}
 
 
+def cpu_baseline_mine(p, f, seconds):
+    """mine's CPU loops (loopCPUCellCentred / loopCPUCellCentredOMP,
+    problem.hpp:342-437, restated by the oracle) on the same grid: a bounded
+    number of iterations each, iteration-loop time only (colouring and the
+    per-iteration copy excluded, as the reference times them)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    cores = min(len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+                else (os.cpu_count() or 1), 16)
+    ne = p.maps[0].shape[0]
+    o = O.OracleProblem(O.MINE2, p.maps, p.num_points, p.point_data, p.cell_data)
+    per = o.loop_colour(1, cores)
+    iters = int(max(1, min(100, seconds / 2 / max(per, 1e-6))))
+    secs = o.loop_colour(iters, cores)
+    s = O.OracleProblem(O.MINE2, p.maps, p.num_points, p.point_data, p.cell_data)
+    s.loop_serial(1)
+    per_s = s.last_loop_seconds
+    it_s = int(max(1, min(100, seconds / 2 / max(per_s, 1e-6))))
+    s.loop_serial(it_s)
+    secs_s = s.last_loop_seconds
+    return {"value": ne * iters / secs, "unit": "edges/s", "cores": cores, "kind": "port",
+            "sample": f"{iters} iterations ({secs:.1f} s) of the colour-parallel OpenMP oracle "
+                      f"loop and {it_s} ({secs_s:.1f} s) of the serial one on the same grid, "
+                      f"iteration loop only",
+            "serial_edges_per_s": ne * it_s / secs_s}
+
+
 def _line(metric, value, unit, steps, warmup, ms, workload, **extra):
     d = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
          "warmup": warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -398,7 +426,21 @@ def bench_single(args, workload):
     """mini_aero 96^3 faces, hex scatter or lulesh forces on 128^3 elements:
     owner compute on a point-partitioned mesh beside plain atomics."""
     from cuda_fot_amd import Problem, capi, lulesh_problem, meshgen, mini_aero_problem
-    if workload == "lulesh":
+    if workload == "mine":
+        # the fillEdgeList grid as generate_grid lays it out (not shuffled),
+        # point data and edge weights dim 4 U[0,1), seed 0
+        n = 1024
+        e = meshgen.grid_edges(n, n)
+        pd = meshgen.uniform(n * n * 4, 0).reshape(n * n, 4)
+        w = meshgen.uniform(e.shape[0] * 4, 1).reshape(4, -1)
+        p = Problem([e], [n * n], [pd], [w], 512)
+        f = capi.functor(capi.KERNEL_MINE, 2, 4, 4)
+        # unique bytes: point data read (32 B per point), weights (32 B) and
+        # the map (8 B) per edge, out read and written (64 B per point):
+        # 184 467 456 B, 88 B per edge, at 1024^2
+        units, ubytes = e.shape[0], 32 * n * n + 32 * e.shape[0] + 8 * e.shape[0] + 64 * n * n
+        name = f"mine {n}x{n} grid, {e.shape[0]} edges, dim 4 point and edge data, seed 0"
+    elif workload == "lulesh":
         n = 128
         p = lulesh_problem(n + 1, n + 1, n + 1, seed=0, block_size=512)
         f = capi.functor(capi.KERNEL_LULESH)
@@ -444,7 +486,10 @@ def bench_single(args, workload):
     p.partition_points(args.points_per_block)
     wall, ev = run(p, capi.STRATEGY_HIERARCHICAL, capi.LOOP_OWNER_COMPUTE,
                    args.points_per_block)
-    unit = "faces/s" if workload == "mini_aero" else "elements/s"
+    unit = {"mini_aero": "faces/s", "mine": "edges/s"}.get(workload, "elements/s")
+    extra = {}
+    if workload == "mine" and not args.no_cpu:
+        extra["cpu_baseline"] = cpu_baseline_mine(p, f, args.cpu_seconds)
     return _line(f"{workload} {unit}", units * args.steps / wall, unit, args.steps, args.warmup,
                  wall / args.steps * 1e3, name,
                  roofline={"bound": "hbm", "achieved": ubytes * args.steps / ev / 1e9,
@@ -452,7 +497,7 @@ def bench_single(args, workload):
                            "frac": ubytes * args.steps / ev / PEAK_HBM, "traffic": None,
                            "algorithmic_bytes_per_step": ubytes},
                  atomic_per_s=units * args.steps / wa,
-                 speedup_vs_atomic=wa / wall)
+                 speedup_vs_atomic=wa / wall, **extra)
 
 
 def main():
@@ -469,7 +514,7 @@ def main():
         args.tune = args.tune or "none"
         print(json.dumps(bench_volna_batch(args)), flush=True)
         return
-    if args.workload in ("mini_aero", "hex", "lulesh"):
+    if args.workload in ("mini_aero", "hex", "lulesh", "mine"):
         print(json.dumps(bench_single(args, args.workload)), flush=True)
         return
     tune = None
