@@ -9,9 +9,11 @@ A step is one res_calc iteration over all edges: the increments of every edge
 added into res on the device (the reference's timed region, problem.hpp:
 269-304; its out->in copy is timed separately there and is a no-op for
 res_calc, whose user function never reads res).  Inputs are resident in HBM.
+The configuration is the learned autotuner's pick for this mesh (--tune
+learned, the default; see tune_res_calc), recorded in config.tune.
 
-Prints ONE JSON line (rank 0).  --gpus above 1 is refused: this build runs
-on one GPU.
+Prints ONE JSON line (rank 0).  --gpus above 1 is refused: the loops run on
+one GPU.
 """
 from __future__ import annotations
 
@@ -503,8 +505,9 @@ def bench_single(args, workload):
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: the engine has no multi-GPU path; an "
+              "owner-compute plan sharded by point partition would need no result "
+              "exchange, only a halo refresh of the read-only inputs", file=sys.stderr)
         sys.exit(2)
     from cuda_fot_amd import capi
     if capi.device_count() < 1:
