@@ -20,10 +20,12 @@ address its owned cells directly.
 GPU (fot_autotune_measure).  `CostModel` is an FP32 MLP 24 -> 64 -> 64 -> 6
 (libfot's MFMA kernels) trained on the autotuner's own timings to predict
 log(ms per step) of every candidate from 24 cheap mesh features
-(fot_loop_features).  `Tuner.learned` takes the predicted-best candidate and
-measures only it; when that measurement exceeds the model's prediction for it
-by more than the tolerance (5 %), or the pick cannot be planned, it falls back
-to the measured search.
+(fot_loop_features).  `Tuner.learned` measures only the candidates the model
+predicts within the tolerance (5 %) of its best prediction (the shortlist:
+one candidate unless the model sees a near tie it cannot resolve) and takes
+the fastest of them; when that one's measurement exceeds the model's
+prediction for it by more than the tolerance, or nothing on the shortlist can
+be planned, it falls back to the measured search.
 """
 from __future__ import annotations
 
@@ -199,14 +201,36 @@ class Tuner:
 
     def learned(self, problem, func, **kw) -> dict:
         pred = self.model.predict(features(problem, func))[0]
-        pick = int(np.argmin(pred))
-        t = measure(problem, func, which=[pick], **kw)
-        res = {"pick": pick, "predicted": pred, "measured_pick": float(t[pick]),
-               "fell_back": False, "measured": 1}
+        short = shortlist(pred, self.tolerance)
+        t = measure(problem, func, which=short, **kw)
+        ok = np.where(t[short] > 0, t[short], np.inf)
+        pick = short[int(np.argmin(ok))]
+        res = {"pick": pick, "predicted": pred, "shortlist": short,
+               "measured_pick": float(t[pick]), "fell_back": False, "measured": len(short)}
         if not (t[pick] > 0) or t[pick] > (1.0 + self.tolerance) * pred[pick]:
             best, times = self.exhaustive(problem, func, **kw)
-            res.update(pick=best, fell_back=True, measured=1 + len(NAMES), times=times)
+            res.update(pick=best, fell_back=True, measured=len(short) + len(NAMES), times=times)
         return res
+
+
+def shortlist(pred_ms, tolerance: float = 0.05) -> list[int]:
+    """Candidates predicted within `tolerance` of the best prediction."""
+    pred_ms = np.asarray(pred_ms, np.float64)
+    return [int(i) for i in np.nonzero(pred_ms <= (1.0 + tolerance) * pred_ms.min())[0]]
+
+
+def shortlist_pick(pred_ms, measured_ms, tolerance: float = 0.05):
+    """The learned choice on known measurements: the fastest measured of the
+    shortlist; if that one exceeds the model's prediction for it by more
+    than `tolerance` (or nothing on the shortlist can be planned), the
+    measured best.  Returns (choice, fell_back, candidates measured)."""
+    short = shortlist(pred_ms, tolerance)
+    m = np.asarray(measured_ms, np.float64)
+    ok = np.where(m > 0, m, np.inf)
+    pick = short[int(np.argmin(ok[short]))]
+    if not (m[pick] > 0) or m[pick] > (1.0 + tolerance) * pred_ms[pick]:
+        return int(np.argmin(ok)), True, len(short) + len(NAMES)
+    return pick, False, len(short)
 
 
 def fallback_pick(pred_ms, measured_ms, tolerance: float = 0.05):
@@ -237,6 +261,7 @@ def evaluate(model: CostModel, X, T, tolerance: float = 0.05) -> dict:
     pred = model.predict(X)
     best = np.where(T > 0, T, np.inf).min(1)
     pick_t, final_t, measured, within, fell = [], [], 0, 0, 0
+    sl_t, sl_measured, sl_fell, sl_within = [], 0, 0, 0
     for i in range(len(X)):
         pick = int(np.argmin(pred[i]))
         choice, fb = fallback_pick(pred[i], T[i], tolerance)
@@ -246,6 +271,11 @@ def evaluate(model: CostModel, X, T, tolerance: float = 0.05) -> dict:
         within += t_pick <= (1 + tolerance) * best[i]
         fell += fb
         measured += 1 + (len(NAMES) if fb else 0)
+        c, f2, k = shortlist_pick(pred[i], T[i], tolerance)
+        sl_t.append(T[i, c] / best[i])
+        sl_within += T[i, c] <= (1 + tolerance) * best[i]
+        sl_fell += f2
+        sl_measured += k
     d = T[:, DEFAULT_CHOICE]
     default = np.where(d > 0, d, np.inf) / best
     n = len(X)
@@ -255,6 +285,11 @@ def evaluate(model: CostModel, X, T, tolerance: float = 0.05) -> dict:
             "final_max_slowdown": float(np.max(final_t)),
             "fallbacks": fell / n,
             "measured_per_mesh": measured / n, "exhaustive_measured_per_mesh": len(NAMES),
+            "shortlist_within_tolerance": sl_within / n,
+            "shortlist_mean_slowdown": float(np.mean(sl_t)),
+            "shortlist_max_slowdown": float(np.max(sl_t)),
+            "shortlist_fallbacks": sl_fell / n,
+            "shortlist_measured_per_mesh": sl_measured / n,
             "default_mean_slowdown": float(np.mean(default)),
             "default_within_tolerance": float(np.mean(default <= 1 + tolerance))}
 

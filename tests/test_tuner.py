@@ -104,8 +104,12 @@ def test_measure_and_learned_tuner(gpu):
     pred = model.predict(x)[0]
     np.testing.assert_allclose(pred, t, rtol=0.05)
     res = tuner.Tuner(model, tolerance=0.5).learned(p, f, warmup=2, steps=5)
-    assert res["pick"] == int(np.argmin(pred))
-    assert res["measured"] in (1, 7)
+    # the shortlist holds the predicted best and every candidate predicted
+    # within the tolerance of it; the pick is its fastest measured one
+    assert res["shortlist"] == tuner.shortlist(pred, 0.5)
+    assert int(np.argmin(pred)) in res["shortlist"]
+    assert res["fell_back"] or res["pick"] in res["shortlist"]
+    assert res["measured"] == len(res["shortlist"]) + (6 if res["fell_back"] else 0)
 
 
 CORPUS = tuner.CORPUS_PATH
@@ -152,3 +156,18 @@ def test_learned_tuner_on_held_out_mini_aero(gpu):
     assert r["final_mean_slowdown"] <= 1.05, r
     assert r["measured_per_mesh"] < r["exhaustive_measured_per_mesh"], r
     assert r["final_mean_slowdown"] < r["default_mean_slowdown"], r
+
+
+def test_shortlist_rule_closed_form():
+    meas = np.array([5.0, 4.0, 3.0, 2.0, 2.1, 2.5])
+    # a near tie (within 5 %): both measured, the faster kept
+    assert tuner.shortlist([9, 9, 9, 2.05, 2.0, 3]) == [3, 4]
+    assert tuner.shortlist_pick(np.array([9, 9, 9, 2.05, 2.0, 3]), meas) == (3, False, 2)
+    # a clear winner: one measurement, as the single-pick rule
+    assert tuner.shortlist_pick(np.array([9, 9, 9, 2.0, 3, 3]), meas) == (3, False, 1)
+    # the shortlist's best misses its prediction by more than 5 %: measured search
+    assert tuner.shortlist_pick(np.array([9, 9, 9, 1.8, 1.85, 9]), meas) == (3, True, 8)
+    # nothing on the shortlist can be planned
+    m2 = meas.copy()
+    m2[0] = -1
+    assert tuner.shortlist_pick(np.array([1.0, 9, 9, 9, 9, 9]), m2) == (3, True, 7)
