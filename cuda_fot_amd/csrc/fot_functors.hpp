@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "fot_capi.h"
 
@@ -190,6 +191,9 @@ struct MineF {
 struct HexScatterF {
   static constexpr uint32_t MD = 8, TD = 3;
   static constexpr bool SLOT_OUTER = true;
+  // sequential cache adds: batching them costs this light kernel occupancy
+  // (69 -> 80..110 VGPRs, 7 -> 6..4 waves per SIMD), measured slower
+  static constexpr uint32_t ADD_CHUNK = 1;
   FOT_DEV static void user(const double *F, uint32_t cs, double *inc) {
 #pragma unroll
     for (uint32_t j = 0; j < 8; ++j)
@@ -569,10 +573,62 @@ FOT_DEV void add_increments(double *out, uint32_t n, const uint32_t *tgt, const 
   }
 }
 
+// Whether a cell's block-local ids are pairwise distinct (every cell of
+// the meshes here; a degenerate one takes the sequential adds).
+template <class F> FOT_DEV bool distinct_slots(const uint32_t *loc) {
+  bool ok = true;
+#pragma unroll
+  for (uint32_t k = 0; k < F::MD; ++k)
+#pragma unroll
+    for (uint32_t l = k + 1; l < F::MD; ++l) ok = ok && loc[k] != loc[l];
+  return ok;
+}
+
+// With pairwise distinct slots a cell's adds can be batched: read up to
+// kAddChunk cache values, then write their sums, one LDS round trip per
+// chunk instead of one per value (the compiler cannot prove the slots
+// distinct, so it serialises plain +=).  The same additions, the same bits.
+// Slots at or above `owned` read slot 0 and write nothing.
+constexpr uint32_t kAddChunk = 8;
+// A functor's chunk: kAddChunk unless it sets ADD_CHUNK (1: sequential adds).
+template <class F, class = void> struct AddChunk {
+  static constexpr uint32_t value = kAddChunk;
+};
+template <class F> struct AddChunk<F, std::void_t<decltype(F::ADD_CHUNK)>> {
+  static constexpr uint32_t value = F::ADD_CHUNK;
+};
+
+template <class F>
+FOT_DEV void add_cell_batched(double *cache, uint32_t m, const uint32_t *loc, const double *inc,
+                              uint32_t owned) {
+  constexpr uint32_t S = F::MD * F::TD, CH = AddChunk<F>::value;
+#pragma unroll
+  for (uint32_t c0 = 0; c0 < S; c0 += CH) {
+    double v[CH];
+#pragma unroll
+    for (uint32_t j = 0; j < CH; ++j)
+      if (c0 + j < S) {
+        const uint32_t k = (c0 + j) / F::TD, d = (c0 + j) % F::TD;
+        v[j] = cache[d * m + (loc[k] < owned ? loc[k] : 0u)];
+      }
+#pragma unroll
+    for (uint32_t j = 0; j < CH; ++j)
+      if (c0 + j < S) {
+        const uint32_t k = (c0 + j) / F::TD, d = (c0 + j) % F::TD;
+        if (loc[k] < owned) cache[d * m + loc[k]] = v[j] + inc[c0 + j];
+      }
+  }
+}
+
 // Adds the increments of one cell to an LDS cache (SOA, stride m) at
-// block-local ids `loc`, in the functor's order.
+// block-local ids `loc`, in the functor's order (batched when the ids are
+// distinct).
 template <class F>
 FOT_DEV void add_to_cache(double *cache, uint32_t m, const uint32_t *loc, const double *inc) {
+  if (AddChunk<F>::value > 1 && distinct_slots<F>(loc)) {
+    add_cell_batched<F>(cache, m, loc, inc, 0xffffffffu);
+    return;
+  }
   if (F::SLOT_OUTER) {
 #pragma unroll
     for (uint32_t k = 0; k < F::MD; ++k)
@@ -592,6 +648,11 @@ FOT_DEV void add_to_cache(double *cache, uint32_t m, const uint32_t *loc, const 
 template <class F>
 FOT_DEV void add_to_cache_owned(double *cache, uint32_t m, const uint32_t *loc, const double *inc,
                                 uint32_t owned) {
+  if (owned == 0) return;
+  if (AddChunk<F>::value > 1 && distinct_slots<F>(loc)) {
+    add_cell_batched<F>(cache, m, loc, inc, owned);
+    return;
+  }
   if (F::SLOT_OUTER) {
 #pragma unroll
     for (uint32_t k = 0; k < F::MD; ++k)
