@@ -84,13 +84,15 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-atomic-ref", action="store_true")
+    ap.add_argument("--soa", action="store_true",
+                    help="res_calc: point data in SOA layout (d * n + i) instead of AOS")
     return ap.parse_args()
 
 
 def build_problem(args, partition_kind):
     from cuda_fot_amd import meshgen, res_calc_problem
     mesh = meshgen.airfoil(args.nx, args.ny, seed=0, shuffle=7)
-    p = res_calc_problem(mesh, block_size=args.block_size)
+    p = res_calc_problem(mesh, block_size=args.block_size, soa=getattr(args, "soa", False))
     if partition_kind == "gps":
         p.reorder("gps")
     elif partition_kind == "kway":
@@ -131,7 +133,7 @@ def make_loop(p, strategy, block_size, stage, graph, order="launch", ept=1, gath
         part = p.point_partition
         flags |= capi.LOOP_POINT_PARTITION
     loop = Loop(capi.functor(capi.KERNEL_RES_CALC), p.maps, p.num_points, dev, out,
-                strategy=strat, block_size=block_size, partition=part,
+                soa=p.soa, strategy=strat, block_size=block_size, partition=part,
                 use_graph=graph, flags=flags)
     return loop, dev, out
 
@@ -204,7 +206,8 @@ def load_traffic(args):
             args.ept == c["cells_per_thread"] and args.nodes == c["nodes"] and
             args.point_terms == c.get("point_terms", False) and
             args.accumulate == c.get("accumulate", "colours") and
-            args.persistent == c.get("persistent", False))
+            args.persistent == c.get("persistent", False) and
+            args.soa == c.get("soa", False))
     return t["traffic_bytes"] if same else None
 
 
@@ -564,6 +567,7 @@ def main():
                    "partition": args.partition,
                    "points_per_block": args.points_per_block if args.partition == "points" else None,
                    "stage_inputs": not args.no_stage, "graph": args.graph,
+                   "layout": "SOA" if args.soa else "AOS",
                    "block_order": args.order, "cells_per_thread": args.ept,
                    "nodes": args.nodes, "point_terms": args.point_terms,
                    "accumulate": args.accumulate,
