@@ -1532,6 +1532,7 @@ fot_status fot_loop::build_hierarchical() {
 }
 
 fot_status fot_loop::enqueue(hipStream_t s) {
+  if (desc.num_cells == 0) return FOT_OK; // nothing to launch
   const fot_status st = dispatch(&f, desc.soa, [&]<class F, bool SOA>() -> fot_status {
     const uint32_t bs = cfg.block_size;
     if (cfg.strategy == FOT_STRATEGY_ATOMIC) {
@@ -1696,6 +1697,16 @@ extern "C" fot_status fot_loop_create(const fot_functor *f, const fot_problem_de
   L->stats.strategy = cfg->strategy;
   L->stats.block_size = cfg->block_size;
   fot_status st;
+  if (cfg->strategy > FOT_STRATEGY_HIERARCHICAL)
+    return fail(FOT_ERR_INVALID_ARGUMENT, "unknown strategy %u", cfg->strategy);
+  if (desc->num_cells == 0) {
+    // a loop over no cells is a no-op: no plan, no launches, no graph
+    // (fot_loop_run still copies point data 0 through, as the reference does)
+    L->colour_blk_off.assign(1, 0);
+    L->threads = 64;
+    *out = L.release();
+    return FOT_OK;
+  }
   L->cfg.partition = cfg->partition;
   switch (cfg->strategy) {
   case FOT_STRATEGY_ATOMIC: st = L->build_atomic(); break;

@@ -215,8 +215,11 @@ def gpu_batch(problems, func: capi.Functor, iters: int, use_graph: bool = False)
                           point_partition=p.point_partition))
     b = Batch(func, specs, use_graph=use_graph)
     b.run(iters)
-    for p, (dev, _, _) in zip(problems, keep):
+    slot = capi.cell_output_slot(func.kernel)
+    for p, (dev, _, cdev) in zip(problems, keep):
         p.point_data[0] = dev[0].download()
+        if slot >= 0 and iters:
+            p.cell_data[slot] = cdev[slot].download()
     stats = b.stats()
     b.close()
     return stats

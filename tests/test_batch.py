@@ -81,3 +81,25 @@ def test_res_calc_batch_matches_unbatched(gpu):
         s.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, iters,
                    flags=capi.LOOP_OWNER_COMPUTE | capi.LOOP_TWO_PER_THREAD)
         assert_bitwise(b.point_data[0], s.point_data[0], f"res_calc batch vs single, mesh {i}")
+
+
+def test_batch_with_an_empty_mesh(gpu):
+    # an empty mesh in a batch contributes no work items; the others are
+    # unaffected, and a batch of empty meshes runs (and copies) nothing
+    iters = 2
+    empty = airfoil_problem(nx=6, ny=4, res0_seed=21)
+    empty.maps = [m[:0].copy() for m in empty.maps]
+    full = airfoil_problem(nx=30, ny=20, res0_seed=22, block_size=128)
+    single = copy.deepcopy(full)
+    before = empty.point_data[0].copy()
+    gpu_batch([empty, full], RES_CALC, iters)
+    single.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, iters,
+                    flags=capi.LOOP_OWNER_COMPUTE | capi.LOOP_TWO_PER_THREAD)
+    assert_bitwise(empty.point_data[0], before, "empty mesh in a batch")
+    assert_bitwise(full.point_data[0], single.point_data[0], "batch with an empty mesh")
+    only = airfoil_problem(nx=6, ny=4, res0_seed=23)
+    only.maps = [m[:0].copy() for m in only.maps]
+    before = only.point_data[0].copy()
+    stats = gpu_batch([only], RES_CALC, iters)
+    assert stats["num_work_items"] == 0
+    assert_bitwise(only.point_data[0], before, "batch of one empty mesh")

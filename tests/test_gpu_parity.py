@@ -328,3 +328,18 @@ def test_res_calc_persistent_more_blocks_than_resident_workgroups(gpu, soa):
     assert stats["launch_grid"] % 8 == 0 and stats["launch_grid"] < stats["num_blocks"], stats
     assert_bitwise(p.point_data[0], ref, "persistent owner gather")
     assert_close_scaled(p.target_in_original_order(), serial, S, what="persistent vs serial")
+
+
+def test_empty_mesh_is_a_no_op(gpu):
+    # no cells: every strategy plans nothing, launches nothing and leaves the
+    # point data as it was
+    base = airfoil_problem(nx=8, ny=6, res0_seed=14)
+    for strategy, flags in [(capi.STRATEGY_ATOMIC, 0), (capi.STRATEGY_GLOBAL_COLOUR, 0),
+                            (capi.STRATEGY_HIERARCHICAL, 0), (capi.STRATEGY_HIERARCHICAL, DEP),
+                            (capi.STRATEGY_HIERARCHICAL, OWN | TWO | OG)]:
+        for graph in (False, True):
+            p = airfoil_problem(nx=8, ny=6, res0_seed=14)
+            p.maps = [m[:0].copy() for m in p.maps]
+            p.gpu_loop(RES_CALC, strategy, ITERS, flags=flags, use_graph=graph, partition=False)
+            assert_bitwise(p.point_data[0], base.point_data[0],
+                           f"empty mesh, strategy {strategy}/{flags}, graph {graph}")
