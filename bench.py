@@ -58,6 +58,9 @@ def parse():
                     choices=["none", "kway", "ours", "gps", "points"],
                     help="points: owner partition of mapping-0 points (owner compute)")
     ap.add_argument("--points-per-block", type=int, default=240)
+    ap.add_argument("--numbering", default="cm", choices=["id", "cm"],
+                    help="owner partition: points inside a part in their previous order "
+                         "(id) or in Cuthill-McKee order (cm)")
     ap.add_argument("--tune", default=None, choices=["none", "exhaustive", "learned"],
                     help="res_calc (default learned): the configuration the autotuner "
                          "picks for the airfoil as generated; volna-batch (default none): "
@@ -105,7 +108,7 @@ def build_problem(args, partition_kind):
         p.reorder_to_partition()
         p.renumber_points()
     elif partition_kind == "points":
-        p.partition_points(args.points_per_block)
+        p.partition_points(args.points_per_block, args.numbering)
     return mesh, p
 
 
@@ -203,6 +206,7 @@ def load_traffic(args):
     same = (args.strategy == "hierarchical" and args.order == "owner" and
             args.partition == c["partition"] and args.nx == c["nx"] and args.ny == c["ny"] and
             args.points_per_block == c["points_per_block"] and
+            args.numbering == c.get("numbering", "id") and
             args.ept == c["cells_per_thread"] and args.nodes == c["nodes"] and
             args.point_terms == c.get("point_terms", False) and
             args.accumulate == c.get("accumulate", "colours") and
@@ -492,7 +496,7 @@ def bench_single(args, workload):
         return wall, ev
 
     wa, _ = run(p, capi.STRATEGY_ATOMIC, 0, 256)
-    p.partition_points(args.points_per_block)
+    p.partition_points(args.points_per_block, args.numbering)
     wall, ev = run(p, capi.STRATEGY_HIERARCHICAL, capi.LOOP_OWNER_COMPUTE,
                    args.points_per_block)
     unit = {"mini_aero": "faces/s", "mine": "edges/s"}.get(workload, "elements/s")
@@ -566,6 +570,7 @@ def main():
                    "strategy": args.strategy, "block_size": args.block_size,
                    "partition": args.partition,
                    "points_per_block": args.points_per_block if args.partition == "points" else None,
+                   "numbering": args.numbering if args.partition == "points" else None,
                    "stage_inputs": not args.no_stage, "graph": args.graph,
                    "layout": "SOA" if args.soa else "AOS",
                    "block_order": args.order, "cells_per_thread": args.ept,

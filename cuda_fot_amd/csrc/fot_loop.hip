@@ -750,8 +750,12 @@ FOT_DEV void owner3_body(double *lds, Owner3Data<EPT, KS, KO> &d, const Owner3Id
                          const OwnerMeta &mt, const EngineArgs &a, uint32_t tid, uint32_t bd) {
   const uint32_t m = mt.ncached, nw = mt.nowned, ne = mt.ncells;
   const uint32_t np = a.num_points[0];
+  // LDS holds double2 planes, so lanes on consecutive slots touch
+  // consecutive 16-byte words: q as (q0, q1)[m] then (q2, q3)[m], and after
+  // the second barrier the fluxes as (f0, f1)[ne] then (f2, f3)[ne]
   double *q_s = lds, *adt_s = q_s + 4 * m, *ri_s = adt_s + m, *p_s = ri_s + m;
-  double *f_s = lds; // after the second barrier: 4 doubles per edge
+  double2 *q2_s = reinterpret_cast<double2 *>(q_s);
+  double2 *f2_s = reinterpret_cast<double2 *>(lds);
   // one wait for all the loads (see res_calc_owner2_kernel)
 #pragma unroll
   for (int k = 0; k < KS; ++k) {
@@ -762,8 +766,8 @@ FOT_DEV void owner3_body(double *lds, Owner3Data<EPT, KS, KO> &d, const Owner3Id
   for (int k = 0; k < KS; ++k) {
     const uint32_t i = tid + k * bd;
     if (i < m) {
-      q_s[0 * m + i] = d.vq0[k].x, q_s[1 * m + i] = d.vq0[k].y;
-      q_s[2 * m + i] = d.vq1[k].x, q_s[3 * m + i] = d.vq1[k].y;
+      q2_s[i] = d.vq0[k];
+      q2_s[m + i] = d.vq1[k];
       adt_s[i] = d.vadt[k];
       if (PRE) {
         const double qq[4] = {d.vq0[k].x, d.vq0[k].y, d.vq1[k].x, d.vq1[k].y};
@@ -777,25 +781,24 @@ FOT_DEV void owner3_body(double *lds, Owner3Data<EPT, KS, KO> &d, const Owner3Id
   for (int e = 0; e < EPT; ++e) {
     if (tid + e * bd < ne) {
       const uint32_t l0 = id.pk[e] & 0xfffu, l1 = (id.pk[e] >> 12) & 0xfffu;
-      double q1[4], q2[4], r2[4];
-#pragma unroll
-      for (uint32_t c = 0; c < 4; ++c) q1[c] = q_s[c * m + l0], q2[c] = q_s[c * m + l1];
-      const double a1 = adt_s[l0], a2 = adt_s[l1];
+      const double2 a0 = q2_s[l0], a1 = q2_s[m + l0], b0 = q2_s[l1], b1 = q2_s[m + l1];
+      const double q1[4] = {a0.x, a0.y, a1.x, a1.y}, q2[4] = {b0.x, b0.y, b1.x, b1.y};
+      double r2[4];
+      const double adt1 = adt_s[l0], adt2 = adt_s[l1];
       if (PRE)
-        res_calc_user_terms(d.x1[e], d.x2[e], q1, q2, ri_s[l0], ri_s[l1], p_s[l0], p_s[l1], a1,
-                            a2, fl[e], r2);
+        res_calc_user_terms(d.x1[e], d.x2[e], q1, q2, ri_s[l0], ri_s[l1], p_s[l0], p_s[l1], adt1,
+                            adt2, fl[e], r2);
       else
-        res_calc_user(d.x1[e], d.x2[e], q1, q2, &a1, &a2, fl[e], r2, 1, 1);
+        res_calc_user(d.x1[e], d.x2[e], q1, q2, &adt1, &adt2, fl[e], r2, 1, 1);
     }
   }
-  __syncthreads(); // q_s / adt_s are dead: f_s takes their place
+  __syncthreads(); // q and adt are dead: the fluxes take their place
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     const uint32_t t = tid + e * bd;
     if (t < ne) {
-      double2 *fp = reinterpret_cast<double2 *>(f_s + 4 * size_t(t));
-      fp[0] = make_double2(fl[e][0], fl[e][1]);
-      fp[1] = make_double2(fl[e][2], fl[e][3]);
+      f2_s[t] = make_double2(fl[e][0], fl[e][1]);
+      f2_s[ne + t] = make_double2(fl[e][2], fl[e][3]);
     }
   }
   __syncthreads();
@@ -808,8 +811,8 @@ FOT_DEV void owner3_body(double *lds, Owner3Data<EPT, KS, KO> &d, const Owner3Id
       for (int j = 0; j < 2 * LW; ++j) {
         const uint32_t ent = (id.lst[k][j >> 1] >> (16 * (j & 1))) & 0xffffu;
         if (ent == 0xffffu) break;
-        const double2 *fp = reinterpret_cast<const double2 *>(f_s + 4 * size_t(ent & 0x7fffu));
-        const double2 f01 = fp[0], f23 = fp[1];
+        const uint32_t pos = ent & 0x7fffu;
+        const double2 f01 = f2_s[pos], f23 = f2_s[ne + pos];
         if (ent & 0x8000u) { // the edge's second cell: res2 = -f, and r + (-f) == r - f
           r[0] -= f01.x, r[1] -= f01.y, r[2] -= f23.x, r[3] -= f23.y;
         } else {
@@ -1141,6 +1144,41 @@ fot_status fot_loop::build_global_colour() {
   return FOT_OK;
 }
 
+// The gather kernel adds each owned cell's fluxes in thread-colour order
+// through per-cell position lists, so the positions of a block can come in
+// any order without changing a bit.  Ascending (first, second) block-local
+// ids put neighbouring lanes of the flux step on neighbouring LDS words.
+static void sort_positions_by_local_ids(const std::vector<uint32_t> &blk_cell_off,
+                                        std::vector<uint32_t> &cells,
+                                        std::vector<uint8_t> &colour,
+                                        std::vector<uint16_t> &loc) {
+  std::vector<uint32_t> idx, c2;
+  std::vector<uint8_t> k2;
+  std::vector<uint16_t> l2;
+  for (size_t b = 0; b + 1 < blk_cell_off.size(); ++b) {
+    const uint32_t p0 = blk_cell_off[b], n = blk_cell_off[b + 1] - p0;
+    idx.resize(n);
+    std::iota(idx.begin(), idx.end(), p0);
+    auto key = [&](uint32_t i) {
+      return (uint32_t(loc[2 * size_t(i)]) << 16) | loc[2 * size_t(i) + 1];
+    };
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](uint32_t a, uint32_t c) { return key(a) < key(c); });
+    c2.resize(n);
+    k2.resize(n);
+    l2.resize(2 * size_t(n));
+    for (uint32_t i = 0; i < n; ++i) {
+      c2[i] = cells[idx[i]];
+      k2[i] = colour[idx[i]];
+      l2[2 * size_t(i)] = loc[2 * size_t(idx[i])];
+      l2[2 * size_t(i) + 1] = loc[2 * size_t(idx[i]) + 1];
+    }
+    std::copy(c2.begin(), c2.end(), cells.begin() + p0);
+    std::copy(k2.begin(), k2.end(), colour.begin() + p0);
+    std::copy(l2.begin(), l2.end(), loc.begin() + 2 * size_t(p0));
+  }
+}
+
 fot_status fot_loop::build_hierarchical() {
   fot_status st;
   const uint32_t md = desc.mesh_dim[0];
@@ -1187,6 +1225,8 @@ fot_status fot_loop::build_hierarchical() {
     colour = op.colour;
     blk_ncol = op.blk_ncol;
     loc.assign(op.loc.begin(), op.loc.end());
+    if (f.kernel == FOT_KERNEL_RES_CALC && (cfg.flags & FOT_LOOP_OWNER_GATHER) && md == 2)
+      sort_positions_by_local_ids(blk_cell_off, plan_cell, colour, loc);
     for (uint32_t b = 0; b < nb; ++b) {
       max_cached = std::max(max_cached, blk_cache_off[b + 1] - blk_cache_off[b]);
       max_tc = std::max<uint32_t>(max_tc, blk_ncol[b]);

@@ -88,14 +88,20 @@ class Problem:
         self.partition_vector = plan.partition_ours(self.maps, self.num_points, self.block_size,
                                                     num_blocks_in_coarse, tolerance)
 
-    def partition_points(self, points_per_block: int) -> None:
+    def partition_points(self, points_per_block: int, numbering: str = "cm") -> None:
         """Owner partition for the owner-compute loop: parts of at most
         points_per_block mapping-0 points, then the mesh renumbered so every
         part's points are contiguous (mappings identical to mapping 0 follow),
         cells sorted by the part of their first point, and the other mappings
-        renumbered by first touch."""
-        part = plan.partition_points(self.maps[0], self.num_points[0], points_per_block)
-        order = np.lexsort((np.arange(part.size), part)).astype(np.uint32)
+        renumbered by first touch.  Inside a part, points keep their relative
+        order (numbering "id") or follow the Cuthill-McKee order of the point
+        graph ("cm"), which gives neighbouring points nearby block-local ids."""
+        if numbering not in ("id", "cm"):
+            raise ValueError(f"numbering {numbering!r}: 'id' or 'cm'")
+        ind, end = plan.graph_csr(self.maps[0], self.num_points[0])
+        part = plan.kway_partition(ind, end, points_per_block)[0]
+        within = plan.gps_order(ind, end) if numbering == "cm" else np.arange(part.size)
+        order = np.lexsort((within, part)).astype(np.uint32)
         perm = np.empty_like(order)
         perm[order] = np.arange(order.size, dtype=np.uint32)
         same = [k for k in range(1, len(self.maps)) if np.array_equal(self.maps[k], self.maps[0])]
