@@ -282,8 +282,8 @@ def _line(metric, value, unit, steps, warmup, ms, workload, **extra):
 
 def bench_volna_batch(args):
     """64 shallow-water triangle meshes, cells log-uniform in [2e4, 4e5], one
-    persistent owner-compute launch per step; timed eagerly and as hipGraph
-    replays (increments + copy back), beside one owner loop per mesh."""
+    persistent owner-compute launch per step; fot_batch_run timed with eager
+    launches and with hipGraph replays, beside one owner loop per mesh."""
     from cuda_fot_amd import Batch, DeviceArray, Timer, capi, meshgen, synchronize, volna_problem
     from cuda_fot_amd.device import Loop
     rng = np.random.default_rng(0)
@@ -349,11 +349,8 @@ def bench_volna_batch(args):
         t = Timer()
         t0 = time.perf_counter()
         t.start()
-        if graph:
-            b.run(args.steps)          # steps graph replays (increments + copy back)
-        else:
-            for _ in range(args.steps):
-                b.step()
+        # copy in, the steps (eager launches or graph replays), copy back
+        b.run(args.steps)
         t.stop()
         synchronize()
         res[graph] = (time.perf_counter() - t0, t.elapsed_ms() / 1e3, b.stats())

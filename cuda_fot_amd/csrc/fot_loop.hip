@@ -1945,12 +1945,14 @@ extern "C" fot_status fot_batch_create(const fot_functor *f, uint32_t num_proble
   FOT_HIP(hipDeviceSynchronize());
   if (use_graph) {
     // one timestep: the batched increments, then in <- out for every problem
+    // when the loop reads its own iterate (else the copy runs once, after the
+    // last step, as in the eager run)
     hipStream_t cs = nullptr;
     FOT_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
       st = B->launch_step(cs);
-      if (st == FOT_OK) st = B->launch_copy(B->copy_back, cs);
+      if (st == FOT_OK && B->jacobi) st = B->launch_copy(B->copy_back, cs);
       e = hipStreamEndCapture(cs, &B->graph);
     }
     if (e == hipSuccess && st == FOT_OK) e = hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0);
@@ -1978,12 +1980,13 @@ extern "C" fot_status fot_batch_run(fot_batch *b, uint32_t iters, void *stream) 
   fot_status st = b->launch_copy(b->copy_in, s);
   for (uint32_t it = 0; it < iters && st == FOT_OK; ++it) {
     if (b->exec) {
-      FOT_HIP(hipGraphLaunch(b->exec, s));
+      FOT_HIP(hipGraphLaunch(b->exec, s)); // copies back itself when jacobi
     } else {
       st = b->launch_step(s);
-      if (st == FOT_OK && (b->jacobi || it + 1 == iters)) st = b->launch_copy(b->copy_back, s);
+      if (st == FOT_OK && b->jacobi) st = b->launch_copy(b->copy_back, s);
     }
   }
+  if (st == FOT_OK && iters && !b->jacobi) st = b->launch_copy(b->copy_back, s);
   for (fot_loop *l : b->loops)
     if (st == FOT_OK && iters) st = l->scatter_cell_outputs(s);
   if (st != FOT_OK) return st;

@@ -540,8 +540,10 @@ typedef struct fot_batch fot_batch;
 /* descs/cfgs: one per problem (same functor and layout).  Each cfg's
  * block_size / partition (+ FOT_LOOP_POINT_PARTITION) shape that problem's
  * plan; the strategy is always owner-compute hierarchical.  use_graph != 0
- * captures one timestep (the batched increments, then point_data[0] = out
- * for every problem) in a hipGraph that fot_batch_run replays. */
+ * captures one timestep in a hipGraph that fot_batch_run replays: the
+ * batched increments, then point_data[0] = out for every problem when the
+ * functor reads its own iterate (mine); otherwise that copy runs once,
+ * after the last step, as in the eager run. */
 fot_status fot_batch_create(const fot_functor *f, uint32_t num_problems,
                             const fot_problem_desc *descs, const fot_loop_config *cfgs,
                             uint32_t use_graph, fot_batch **batch);

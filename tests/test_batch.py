@@ -41,6 +41,28 @@ def test_volna_batch_matches_unbatched(gpu, use_graph):
         assert_bitwise(b.point_data[0], o.target, f"batch vs owner replay, mesh {i}")
 
 
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_mine_batch_reads_its_iterate(gpu, use_graph):
+    # mine reads the point data it increments: every step copies the result
+    # back before the next one, in the eager run and inside the graph
+    from helpers import mine_problem
+    iters = 3
+    f = capi.functor(capi.KERNEL_MINE, 2, 4, 4)
+    def make():
+        return [mine_problem(N=24, M=20, seed=1, block_size=128),
+                mine_problem(N=40, M=18, seed=2, block_size=96)]
+    batch, singles = make(), make()
+    gpu_batch(batch, f, iters, use_graph=use_graph)
+    for i, (b, s) in enumerate(zip(batch, singles)):
+        ref = copy.deepcopy(s)
+        s.gpu_loop(f, capi.STRATEGY_HIERARCHICAL, iters, flags=capi.LOOP_OWNER_COMPUTE)
+        assert_bitwise(b.point_data[0], s.point_data[0], f"mine batch vs single, mesh {i}")
+        o = oracle_of(ref, O.MINE2)
+        O.loop_owner_replay(o, plan.owner_plan(ref.maps[0], ref.num_points[0],
+                                               ref.block_size), iters)
+        assert_bitwise(b.point_data[0], o.target, f"mine batch vs owner replay, mesh {i}")
+
+
 def test_volna_batch_graph_replays_repeat(gpu):
     # two runs of the same graph-backed batch: the work queue resets itself
     from cuda_fot_amd import Batch, DeviceArray
