@@ -215,6 +215,18 @@ def load_traffic(args):
     return t["traffic_bytes"] if same else None
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), as lscpu reports it."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(mesh, seconds):
     """The oracle's colour-parallel loop (problem.hpp:405-437 restated in C,
     OpenMP over all host cores) timed on a bounded number of iterations of
@@ -238,7 +250,7 @@ def cpu_baseline(mesh, seconds):
     serial.loop_serial(2)
     serial_per = serial.last_loop_seconds / 2
     return {"value": mesh["num_edges"] * iters / secs, "unit": "edges/s", "cores": cores,
-            "kind": "port",
+            "kind": "port", "cpu_model": cpu_model(),
             "sample": f"{iters} iterations ({secs:.1f} s) of the colour-parallel OpenMP oracle "
                       f"loop on the same {mesh['nx']}x{mesh['ny']} mesh, iteration loop only",
             "serial_edges_per_s": mesh["num_edges"] / serial_per}
@@ -265,6 +277,7 @@ def cpu_baseline_mine(p, f, seconds):
     s.loop_serial(it_s)
     secs_s = s.last_loop_seconds
     return {"value": ne * iters / secs, "unit": "edges/s", "cores": cores, "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": f"{iters} iterations ({secs:.1f} s) of the colour-parallel OpenMP oracle "
                       f"loop and {it_s} ({secs_s:.1f} s) of the serial one on the same grid, "
                       f"iteration loop only",

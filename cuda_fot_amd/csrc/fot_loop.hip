@@ -1943,10 +1943,11 @@ extern "C" fot_status fot_batch_create(const fot_functor *f, uint32_t num_proble
   if (st != FOT_OK) return st;
   B->grid = std::max<uint32_t>(1, std::min<uint32_t>(B->nwork, uint32_t(std::max(1, per_cu) * cus)));
   FOT_HIP(hipDeviceSynchronize());
-  if (use_graph) {
+  if (use_graph && (B->nwork > 0 || B->jacobi)) {
     // one timestep: the batched increments, then in <- out for every problem
     // when the loop reads its own iterate (else the copy runs once, after the
-    // last step, as in the eager run)
+    // last step, as in the eager run).  A step with nothing to launch is not
+    // captured: the eager run does nothing for it either.
     hipStream_t cs = nullptr;
     FOT_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);

@@ -543,7 +543,8 @@ typedef struct fot_batch fot_batch;
  * captures one timestep in a hipGraph that fot_batch_run replays: the
  * batched increments, then point_data[0] = out for every problem when the
  * functor reads its own iterate (mine); otherwise that copy runs once,
- * after the last step, as in the eager run. */
+ * after the last step, as in the eager run.  A batch with no work items
+ * (every mesh empty) and no per-step copy captures no graph. */
 fot_status fot_batch_create(const fot_functor *f, uint32_t num_problems,
                             const fot_problem_desc *descs, const fot_loop_config *cfgs,
                             uint32_t use_graph, fot_batch **batch);

@@ -122,6 +122,7 @@ def test_batch_with_an_empty_mesh(gpu):
     only = airfoil_problem(nx=6, ny=4, res0_seed=23)
     only.maps = [m[:0].copy() for m in only.maps]
     before = only.point_data[0].copy()
-    stats = gpu_batch([only], RES_CALC, iters)
-    assert stats["num_work_items"] == 0
-    assert_bitwise(only.point_data[0], before, "batch of one empty mesh")
+    for graph in (False, True):
+        stats = gpu_batch([only], RES_CALC, iters, use_graph=graph)
+        assert stats["num_work_items"] == 0 and stats["uses_graph"] == 0
+        assert_bitwise(only.point_data[0], before, f"batch of one empty mesh, graph {graph}")
