@@ -87,6 +87,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-atomic-ref", action="store_true")
+    ap.add_argument("--sweep-block-sizes", action="store_true",
+                    help="mini_aero / hex / lulesh / mine: also time plain atomics and the "
+                         "reference-shaped hierarchical loop (GPS-ordered mesh, one launch per "
+                         "block colour) at block sizes 128-1024")
     ap.add_argument("--soa", action="store_true",
                     help="res_calc: point data in SOA layout (d * n + i) instead of AOS")
     return ap.parse_args()
@@ -492,7 +496,7 @@ def bench_single(args, workload):
     from cuda_fot_amd import DeviceArray
     from cuda_fot_amd.device import Loop
 
-    def run(prob, strategy, flags, bs):
+    def run(prob, strategy, flags, bs, with_stats=False):
         dev = [DeviceArray.from_host(d) for d in prob.point_data]
         out = DeviceArray.from_host(prob.point_data[0])
         cdev = [DeviceArray.from_host(c) for c in prob.cell_data]
@@ -502,10 +506,32 @@ def bench_single(args, workload):
         l = Loop(f, prob.maps, prob.num_points, dev, out, cdev, soa=prob.soa, strategy=strategy,
                  block_size=bs, partition=part, flags=flags)
         wall, ev = time_loop(l, args.steps, args.warmup)
+        st = l.stats()
         l.close()
-        return wall, ev
+        return (wall, ev, st) if with_stats else (wall, ev)
 
     wa, _ = run(p, capi.STRATEGY_ATOMIC, 0, 256)
+    sweep = None
+    if args.sweep_block_sizes:
+        # atomics on the mesh as generated; the hierarchical loop (LDS-cached
+        # point data, thread colours, one launch per block colour) on the
+        # GPS-ordered mesh with consecutive blocks
+        import copy
+        g = copy.deepcopy(p)
+        g.reorder("gps")
+        sweep = {}
+        for bs in (128, 256, 512, 1024):
+            _, ea = run(p, capi.STRATEGY_ATOMIC, 0, bs)
+            row = {"atomic_ms": ea / args.steps * 1e3}
+            try:
+                _, eh, st = run(g, capi.STRATEGY_HIERARCHICAL, 0, bs, with_stats=True)
+                row.update(hierarchical_gps_ms=eh / args.steps * 1e3,
+                           hierarchical_launches=st["num_launches_per_iter"])
+            except capi.ColourOverflow as e:
+                row["hierarchical_gps_ms"] = None
+                row["hierarchical_error"] = str(e)
+            sweep[str(bs)] = row
+        del g
     p.partition_points(args.points_per_block, args.numbering)
     wall, ev = run(p, capi.STRATEGY_HIERARCHICAL, capi.LOOP_OWNER_COMPUTE,
                    args.points_per_block)
@@ -513,7 +539,7 @@ def bench_single(args, workload):
     extra = {}
     if workload == "mine" and not args.no_cpu:
         extra["cpu_baseline"] = cpu_baseline_mine(p, f, args.cpu_seconds)
-    return _line(f"{workload} {unit}", units * args.steps / wall, unit, args.steps, args.warmup,
+    line = _line(f"{workload} {unit}", units * args.steps / wall, unit, args.steps, args.warmup,
                  wall / args.steps * 1e3, name,
                  roofline={"bound": "hbm", "achieved": ubytes * args.steps / ev / 1e9,
                            "peak": PEAK_HBM / 1e9, "unit": "GB/s",
@@ -521,6 +547,9 @@ def bench_single(args, workload):
                            "algorithmic_bytes_per_step": ubytes},
                  atomic_per_s=units * args.steps / wa,
                  speedup_vs_atomic=wa / wall, **extra)
+    if sweep is not None:
+        line["block_size_sweep"] = sweep
+    return line
 
 
 def main():
