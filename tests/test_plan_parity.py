@@ -250,6 +250,35 @@ def test_owner_plan_invariants():
         assert runs.min() >= 1 and runs.max() <= md
 
 
+def test_partition_points_numbering():
+    # parts contiguous after renumbering; inside a part, "id" keeps the points'
+    # previous relative order and "cm" follows the Cuthill-McKee order of the
+    # point graph; the mesh is only renumbered (same cells, same point values)
+    import pytest
+    base = airfoil_problem(nx=30, ny=20, seed=3, block_size=64, res0_seed=5)
+    ind, end = plan.graph_csr(base.maps[0], base.num_points[0])
+    part = plan.kway_partition(ind, end, 48)[0]
+    cm_rank = plan.gps_order(ind, end)
+    for numbering, key in (("id", np.arange(part.size)), ("cm", cm_rank)):
+        p = airfoil_problem(nx=30, ny=20, seed=3, block_size=64, res0_seed=5)
+        p.partition_points(48, numbering)
+        pp = p.point_partition
+        assert np.all(np.diff(pp.astype(np.int64)) >= 0)  # contiguous parts
+        # new id of each old point: the stable order of (part, key)
+        order = np.lexsort((key, part))
+        perm = np.empty_like(order)
+        perm[order] = np.arange(order.size)
+        assert np.array_equal(pp[perm], part)
+        # the res values moved with their points
+        assert np.array_equal(p.point_data[0][perm], base.point_data[0])
+        # every cell kept: the same point pairs up to the renumbering
+        cells_new = {tuple(r) for r in p.maps[0].tolist()}
+        cells_old = {tuple(perm[r]) for r in base.maps[0].tolist()}
+        assert cells_new == cells_old
+    with pytest.raises(ValueError):
+        airfoil_problem(nx=8, ny=6).partition_points(16, "rcm")
+
+
 def test_generators():
     e = meshgen.grid_edges(3, 3)
     # Grid::fillEdgeList (grid.hpp:77-104) on a 3x3 grid
