@@ -230,6 +230,21 @@ _PROTOS = {
     "fot_batch_step": [P, P],
     "fot_batch_run": [P, U32, P],
     "fot_batch_stats_get": [P, P],
+    "fot_ctx_create": [P, C.c_int],
+    "fot_ctx_destroy": [P],
+    "fot_ctx_stream": [P, P],
+    "fot_ctx_synchronize": [P],
+    "fot_buf_alloc": [P, C.c_size_t, P],
+    "fot_buf_upload": [P, P, P, C.c_size_t],
+    "fot_buf_download": [P, P, P, C.c_size_t],
+    "fot_buf_free": [P, P],
+    "fot_plan_upload": [P, P, P, P, P],
+    "fot_plan_free": [P],
+    "fot_loop_launch": [P, P, P, P, U32],
+    "fot_graph_capture": [P, P, P],
+    "fot_graph_launch": [P, P],
+    "fot_graph_free": [P],
+    "fot_batch_launch": [P, P, U32],
     "fot_loop_features": [P, P, P],
     "fot_autotune_measure": [P, P, P, U32, U32, U32, P],
     "fot_mlp_create": [U32, U32, U32, U64, P],

@@ -2007,6 +2007,98 @@ extern "C" fot_status fot_batch_stats_get(const fot_batch *b, fot_batch_stats *s
 }
 
 // ---------------------------------------------------------------------
+// Context layer: plans, one-shot loops, graphs and batches on a context's
+// stream (the context and its buffers are in fot_runtime.hip).
+// ---------------------------------------------------------------------
+namespace {
+fot_status ctx_stream(fot_ctx *ctx, hipStream_t *s) {
+  void *p = nullptr;
+  const fot_status st = fot_ctx_stream(ctx, &p);
+  *s = static_cast<hipStream_t>(p);
+  return st;
+}
+} // namespace
+
+extern "C" fot_status fot_plan_upload(fot_ctx *ctx, const fot_functor *f,
+                                      const fot_problem_desc *desc, const fot_loop_config *cfg,
+                                      fot_plan **plan) {
+  if (!ctx || !plan) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
+  *plan = nullptr;
+  return fot_loop_create(f, desc, cfg, plan);
+}
+
+extern "C" fot_status fot_plan_free(fot_plan *plan) { return fot_loop_destroy(plan); }
+
+extern "C" fot_status fot_loop_launch(fot_ctx *ctx, const fot_functor *f,
+                                      const fot_problem_desc *desc, const fot_loop_config *cfg,
+                                      uint32_t iters) {
+  hipStream_t s = nullptr;
+  fot_status st = ctx ? ctx_stream(ctx, &s) : fail(FOT_ERR_INVALID_ARGUMENT, "null context");
+  if (st != FOT_OK) return st;
+  fot_loop *loop = nullptr;
+  if ((st = fot_loop_create(f, desc, cfg, &loop)) != FOT_OK) return st;
+  st = fot_loop_run(loop, iters, s); // waits for the stream
+  fot_loop_destroy(loop);
+  return st;
+}
+
+struct fot_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~fot_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+extern "C" fot_status fot_graph_capture(fot_ctx *ctx, fot_plan *plan, fot_graph **out) {
+  if (!ctx || !plan || !out) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  std::unique_ptr<fot_graph> G(new fot_graph);
+  if (plan->desc.num_cells > 0) {
+    // the plan's own enqueue (not a replay of a graph it may hold), captured
+    // on a private stream as fot_loop_create does for use_graph
+    hipStream_t cs = nullptr;
+    FOT_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    fot_status st = FOT_OK;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      st = plan->enqueue(cs);
+      e = hipStreamEndCapture(cs, &G->graph);
+    }
+    if (e == hipSuccess && st == FOT_OK) e = hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0);
+    (void)hipStreamDestroy(cs);
+    if (st != FOT_OK) return st;
+    if (e != hipSuccess) return fail(FOT_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+  }
+  *out = G.release();
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_graph_launch(fot_ctx *ctx, fot_graph *graph) {
+  if (!ctx || !graph) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
+  if (!graph->exec) return FOT_OK; // a plan over no cells
+  hipStream_t s = nullptr;
+  const fot_status st = ctx_stream(ctx, &s);
+  if (st != FOT_OK) return st;
+  FOT_HIP(hipGraphLaunch(graph->exec, s));
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_graph_free(fot_graph *graph) {
+  delete graph;
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_batch_launch(fot_ctx *ctx, fot_batch *batch, uint32_t iters) {
+  if (!ctx || !batch) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
+  hipStream_t s = nullptr;
+  const fot_status st = ctx_stream(ctx, &s);
+  if (st != FOT_OK) return st;
+  return fot_batch_run(batch, iters, s);
+}
+
+// ---------------------------------------------------------------------
 // Autotuner measurement: each candidate config planned and timed on a
 // private stream with HIP events (no reference counterpart).
 // ---------------------------------------------------------------------

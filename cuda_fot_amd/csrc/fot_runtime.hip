@@ -4,7 +4,9 @@
 
 #include <algorithm>
 #include <cstring>
+#include <new>
 #include <string>
+#include <vector>
 
 #include "fot_internal.hpp"
 #include "fot_hip.hpp"
@@ -194,5 +196,105 @@ extern "C" fot_status fot_timer_elapsed_ms(fot_timer *t, float *ms) {
   if (!t || !ms) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
   FOT_HIP(hipEventSynchronize(t->stop));
   FOT_HIP(hipEventElapsedTime(ms, t->start, t->stop));
+  return FOT_OK;
+}
+
+// ---------------------------------------------------------------------
+// Context: a device, one stream, and the buffers allocated through it.
+struct fot_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<void *> bufs;
+};
+
+extern "C" fot_status fot_ctx_create(fot_ctx **ctx, int device) {
+  if (!ctx) return fail(FOT_ERR_INVALID_ARGUMENT, "null context");
+  *ctx = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(FOT_ERR_NO_DEVICE, "no HIP device");
+  if (device < 0 || device >= n)
+    return fail(FOT_ERR_INVALID_ARGUMENT, "device %d of %d", device, n);
+  FOT_HIP(hipSetDevice(device));
+  fot_ctx *c = new (std::nothrow) fot_ctx;
+  if (!c) return fail(FOT_ERR_OUT_OF_MEMORY, "fot_ctx_create");
+  c->device = device;
+  // default priority: streams of other priorities are not used anywhere
+  const hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(FOT_ERR_HIP, "hipStreamCreateWithFlags: %s", hipGetErrorString(e));
+  }
+  *ctx = c;
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_ctx_destroy(fot_ctx *ctx) {
+  if (!ctx) return FOT_OK;
+  (void)hipSetDevice(ctx->device);
+  const hipError_t e = hipStreamSynchronize(ctx->stream);
+  for (void *p : ctx->bufs) (void)hipFree(p);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  if (e != hipSuccess) return fail(FOT_ERR_HIP, "context stream: %s", hipGetErrorString(e));
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_ctx_stream(const fot_ctx *ctx, void **stream) {
+  if (!ctx || !stream) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
+  *stream = ctx->stream;
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_ctx_synchronize(fot_ctx *ctx) {
+  if (!ctx) return fail(FOT_ERR_INVALID_ARGUMENT, "null context");
+  FOT_HIP(hipStreamSynchronize(ctx->stream));
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_buf_alloc(fot_ctx *ctx, size_t bytes, void **device_ptr) {
+  if (!ctx || !device_ptr) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
+  *device_ptr = nullptr;
+  FOT_HIP(hipSetDevice(ctx->device));
+  void *p = nullptr;
+  if (bytes) {
+    const hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return fail(FOT_ERR_OUT_OF_MEMORY, "hipMalloc(%zu): %s", bytes,
+                                     hipGetErrorString(e));
+    ctx->bufs.push_back(p);
+  }
+  *device_ptr = p;
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_buf_upload(fot_ctx *ctx, void *device_dst, const void *host_src,
+                                     size_t bytes) {
+  if (!ctx || (bytes && (!device_dst || !host_src)))
+    return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
+  if (!bytes) return FOT_OK;
+  FOT_HIP(hipMemcpyAsync(device_dst, host_src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  FOT_HIP(hipStreamSynchronize(ctx->stream));
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_buf_download(fot_ctx *ctx, void *host_dst, const void *device_src,
+                                       size_t bytes) {
+  if (!ctx || (bytes && (!host_dst || !device_src)))
+    return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
+  if (!bytes) return FOT_OK;
+  FOT_HIP(hipMemcpyAsync(host_dst, device_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  FOT_HIP(hipStreamSynchronize(ctx->stream));
+  return FOT_OK;
+}
+
+extern "C" fot_status fot_buf_free(fot_ctx *ctx, void *device_ptr) {
+  if (!ctx) return fail(FOT_ERR_INVALID_ARGUMENT, "null context");
+  if (!device_ptr) return FOT_OK;
+  auto it = std::find(ctx->bufs.begin(), ctx->bufs.end(), device_ptr);
+  if (it == ctx->bufs.end())
+    return fail(FOT_ERR_INVALID_ARGUMENT, "buffer %p is not this context's", device_ptr);
+  ctx->bufs.erase(it);
+  // the stream may still use the buffer
+  FOT_HIP(hipStreamSynchronize(ctx->stream));
+  FOT_HIP(hipFree(device_ptr));
   return FOT_OK;
 }

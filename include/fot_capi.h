@@ -559,6 +559,65 @@ typedef struct {
 fot_status fot_batch_stats_get(const fot_batch *batch, fot_batch_stats *stats);
 
 /* ------------------------------------------------------------------ */
+/* Context layer: a device and one stream, the buffers allocated through */
+/* it, plans, one-shot loops, graphs and batches run on that stream.     */
+/* The reference runs on one host thread and the default stream          */
+/* (problem.hpp:177, 302), with Problem owning its device mirrors         */
+/* (data_t.cu:3-14, 50-56).  A context is not thread-safe; use one per    */
+/* host thread.  Every call below runs on the context's stream and       */
+/* returns once its work is done, unless it says otherwise.               */
+/* ------------------------------------------------------------------ */
+typedef struct fot_ctx fot_ctx;
+/* Makes `device` current and creates the context's stream (default
+ * priority). */
+fot_status fot_ctx_create(fot_ctx **ctx, int device);
+/* Waits for the stream, frees every buffer the context still owns, destroys
+ * the stream.  Plans, graphs and batches are freed by their own calls. */
+fot_status fot_ctx_destroy(fot_ctx *ctx);
+/* The context's hipStream_t, for the entry points that take a stream. */
+fot_status fot_ctx_stream(const fot_ctx *ctx, void **stream);
+fot_status fot_ctx_synchronize(fot_ctx *ctx);
+
+/* device_data_t (data_t.hpp:10-74): a device buffer owned by the context;
+ * upload / download are initDeviceMemory / flushToHost (data_t.cu:3-56),
+ * complete on return. */
+fot_status fot_buf_alloc(fot_ctx *ctx, size_t bytes, void **device_ptr);
+fot_status fot_buf_upload(fot_ctx *ctx, void *device_dst, const void *host_src, size_t bytes);
+fot_status fot_buf_download(fot_ctx *ctx, void *host_dst, const void *device_src,
+                            size_t bytes);
+/* FOT_ERR_INVALID_ARGUMENT for a pointer this context did not allocate. */
+fot_status fot_buf_free(fot_ctx *ctx, void *device_ptr);
+
+/* A plan on the device for one problem and config: the colourings and the
+ * block plan the reference builds and uploads at the start of every loop
+ * call (problem.hpp:106-140, 203, 258-265), built once and kept.  The same
+ * object as fot_loop_create's; fot_loop_step / fot_loop_run take it. */
+typedef struct fot_loop fot_plan;
+fot_status fot_plan_upload(fot_ctx *ctx, const fot_functor *f, const fot_problem_desc *desc,
+                           const fot_loop_config *cfg, fot_plan **plan);
+fot_status fot_plan_free(fot_plan *plan);
+
+/* Problem::loopGPUCellCentred / loopGPUHierarchical (problem.hpp:90-339) and
+ * loopGPUEdgeCentred (atomics), as the reference calls them: plans, runs
+ * `iters` iterations with fot_loop_run's semantics on the context's stream,
+ * waits, and frees the plan. */
+fot_status fot_loop_launch(fot_ctx *ctx, const fot_functor *f, const fot_problem_desc *desc,
+                           const fot_loop_config *cfg, uint32_t iters);
+
+/* One iteration's launches of a plan (fot_loop_step: the increments into
+ * point_data_out, no copy back) captured in a hipGraph (no reference
+ * counterpart).  A plan over no cells captures nothing and its graph's launch
+ * does nothing. */
+typedef struct fot_graph fot_graph;
+fot_status fot_graph_capture(fot_ctx *ctx, fot_plan *plan, fot_graph **graph);
+/* Replays the graph on the context's stream; returns without waiting. */
+fot_status fot_graph_launch(fot_ctx *ctx, fot_graph *graph);
+fot_status fot_graph_free(fot_graph *graph);
+
+/* fot_batch_run on the context's stream (no reference counterpart). */
+fot_status fot_batch_launch(fot_ctx *ctx, fot_batch *batch, uint32_t iters);
+
+/* ------------------------------------------------------------------ */
 /* Autotuner support (no reference counterpart)                          */
 /* ------------------------------------------------------------------ */
 #define FOT_NUM_FEATURES 24
