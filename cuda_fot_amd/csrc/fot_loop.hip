@@ -1677,6 +1677,20 @@ extern "C" fot_status fot_loop_create(const fot_functor *f, const fot_problem_de
   for (uint32_t m = 0; m < desc->num_mappings; ++m)
     if (!desc->cell_to_node[m] && desc->num_cells)
       return fail(FOT_ERR_INVALID_ARGUMENT, "mapping %u has no host array", m);
+  {
+    // the point data every functor reads (the kernels do not check pointers)
+    const uint32_t reads = f->kernel == FOT_KERNEL_RES_CALC ? 4u
+                           : (f->kernel == FOT_KERNEL_LULESH || f->kernel == FOT_KERNEL_MINI_AERO ||
+                              f->kernel == FOT_KERNEL_VOLNA)
+                               ? 2u
+                               : 1u;
+    for (uint32_t m = 0; m < reads && m < desc->num_mappings; ++m)
+      if (!desc->point_data[m] && desc->num_cells)
+        return fail(FOT_ERR_INVALID_ARGUMENT, "point data %u is missing", m);
+    for (uint32_t k = 0; k < desc->num_cell_data; ++k)
+      if (!desc->cell_data[k] && desc->num_cells)
+        return fail(FOT_ERR_INVALID_ARGUMENT, "cell data %u is missing", k);
+  }
   // every mapping entry must name a point of its set: the kernels do not
   // bounds-check their gathers
   for (uint32_t m = 0; m < desc->num_mappings; ++m)

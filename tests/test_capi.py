@@ -43,6 +43,28 @@ def test_invalid_arguments_return_status():
     assert capi.lib.fot_functor_init(C.byref(f), 99, 0, 0, 0) == capi.ERR_INVALID_ARGUMENT
 
 
+def test_missing_point_or_cell_data_is_refused():
+    # a null array the functor reads would fault on the device: refused on
+    # the host before any plan or launch
+    d = capi.ProblemDesc()
+    edges = np.array([[0, 1], [1, 2]], dtype=np.uint32)
+    d.num_cells, d.num_mappings = 2, 1
+    d.cell_to_node[0] = edges.ctypes.data
+    d.mesh_dim[0], d.num_points[0], d.point_dim[0] = 2, 3, 4
+    d.num_cell_data, d.cell_dim[0] = 1, 4
+    d.point_data_out = 1  # never dereferenced: refused before
+    f = capi.functor(capi.KERNEL_MINE, 2, 4, 4)
+    cfg = capi.LoopConfig(capi.STRATEGY_ATOMIC, 64, None, 0, 0)
+    loop = C.c_void_p()
+    assert capi.lib.fot_loop_create(C.byref(f), C.byref(d), C.byref(cfg), C.byref(loop)) == \
+        capi.ERR_INVALID_ARGUMENT
+    assert b"point data 0" in capi.lib.fot_last_error_message()
+    d.point_data[0] = 1
+    assert capi.lib.fot_loop_create(C.byref(f), C.byref(d), C.byref(cfg), C.byref(loop)) == \
+        capi.ERR_INVALID_ARGUMENT
+    assert b"cell data 0" in capi.lib.fot_last_error_message()
+
+
 def test_out_of_range_mesh_is_invalid_input():
     # the reference throws InvalidInputFile on a mapping beyond its point set
     bad = np.array([[0, 1], [1, 7]], dtype=np.uint32)
