@@ -1,8 +1,9 @@
 // fot_mlp.hip -- the learned cost model: a small FP32 MLP
 // (in -> hidden -> hidden -> out, ReLU) trained and evaluated on the GPU by
 // hand-written kernels.  Every matrix product runs on the matrix cores
-// (v_mfma_f32_16x16x4_f32, one 16x16 tile per wave); bias, ReLU, the masked
-// MSE gradient and Adam are elementwise kernels.  No reference counterpart.
+// (v_mfma_f32_16x16x4_f32, one 16x16 tile per workgroup, K split across its
+// waves); bias, ReLU, the masked MSE gradient and Adam are elementwise
+// kernels.  No reference counterpart.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,29 +22,37 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 // C[m][n] = sum_k A(m,k) B(k,n) (+ bias[n]) (ReLU), with A(m,k) =
 // A[m*sam + k*sak] and B(k,n) = B[k*sbk + n*sbn]: any transpose through the
-// strides.  One wave per 16x16 tile of C, four waves per workgroup along N.
+// strides.  One 16x16 tile of C per workgroup of kGemmWaves waves; wave w
+// takes the K steps w, w + kGemmWaves, ... (four k each), so a long K (the
+// weight gradients reduce over the whole batch) is split across the waves,
+// and the partial tiles are summed in wave order through LDS.
 // MFMA 16x16x4 f32 operand layout: lane l supplies A(i = l%16, k = l/16) and
 // B(k = l/16, j = l%16); its four results are C(4*(l/16) + v, l%16).
-__global__ __launch_bounds__(256) void gemm_mfma_kernel(const float *__restrict__ A,
-                                                        const float *__restrict__ B,
-                                                        float *__restrict__ C,
-                                                        const float *__restrict__ bias,
-                                                        uint32_t M, uint32_t N, uint32_t K,
-                                                        uint32_t sam, uint32_t sak, uint32_t sbk,
-                                                        uint32_t sbn, uint32_t ldc, int relu,
-                                                        int accumulate) {
+constexpr uint32_t kGemmWaves = 8;
+
+__global__ __launch_bounds__(64 * kGemmWaves) void gemm_mfma_kernel(
+    const float *__restrict__ A, const float *__restrict__ B, float *__restrict__ C,
+    const float *__restrict__ bias, uint32_t M, uint32_t N, uint32_t K, uint32_t sam,
+    uint32_t sak, uint32_t sbk, uint32_t sbn, uint32_t ldc, int relu, int accumulate) {
+  __shared__ v4f part[kGemmWaves][64];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  const uint32_t m0 = blockIdx.x * 16u, n0 = (blockIdx.y * 4u + wave) * 16u;
-  if (n0 >= N) return; // whole wave out of range: no MFMA issued by it
+  const uint32_t m0 = blockIdx.x * 16u, n0 = blockIdx.y * 16u;
   const uint32_t r = lane & 15u, q = lane >> 4;
   const uint32_t am = m0 + r, bn = n0 + r;
   v4f acc = {0.f, 0.f, 0.f, 0.f};
-  for (uint32_t k0 = 0; k0 < K; k0 += 4) {
+#pragma unroll 4
+  for (uint32_t k0 = 4u * wave; k0 < K; k0 += 4u * kGemmWaves) {
     const uint32_t k = k0 + q;
     const float a = (am < M && k < K) ? A[size_t(am) * sam + size_t(k) * sak] : 0.f;
     const float b = (bn < N && k < K) ? B[size_t(k) * sbk + size_t(bn) * sbn] : 0.f;
     acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
   }
+  part[wave][lane] = acc;
+  __syncthreads();
+  if (wave != 0) return;
+  v4f sum = part[0][lane];
+#pragma unroll
+  for (uint32_t w = 1; w < kGemmWaves; ++w) sum += part[w][lane];
   const uint32_t col = n0 + r;
   if (col >= N) return;
   const float bb = bias ? bias[col] : 0.f;
@@ -51,7 +60,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(const float *__restrict_
   for (uint32_t v = 0; v < 4; ++v) {
     const uint32_t row = m0 + 4u * q + v;
     if (row >= M) continue;
-    float x = acc[v] + bb;
+    float x = sum[v] + bb;
     if (relu) x = x > 0.f ? x : 0.f;
     float *c = C + size_t(row) * ldc + col;
     *c = accumulate ? *c + x : x;
@@ -129,7 +138,8 @@ fot_status gemm(const float *A, const float *B, float *C, const float *bias, uin
                 uint32_t N, uint32_t K, uint32_t sam, uint32_t sak, uint32_t sbk, uint32_t sbn,
                 uint32_t ldc, bool relu, bool accumulate, hipStream_t s) {
   if (M == 0 || N == 0) return FOT_OK;
-  hipLaunchKernelGGL(gemm_mfma_kernel, dim3(cdiv(M, 16), cdiv(N, 64)), dim3(256), 0, s, A, B, C,
+  hipLaunchKernelGGL(gemm_mfma_kernel, dim3(cdiv(M, 16), cdiv(N, 16)), dim3(64 * kGemmWaves), 0, s,
+                     A, B, C,
                      bias, M, N, K, sam, sak, sbk, sbn, ldc, relu ? 1 : 0, accumulate ? 1 : 0);
   FOT_HIP(hipGetLastError());
   return FOT_OK;
