@@ -41,3 +41,68 @@ def test_res_calc_at_benchmark_size(gpu, nx, ny):
     # res started at zero: the total of all increments cancels edge by edge
     total = got.sum(axis=0)
     assert np.all(np.abs(total) <= 1e-12 * S.sum(axis=0)), total
+
+
+def _mine_1024():
+    from cuda_fot_amd import Problem
+    n = 1024
+    e = meshgen.grid_edges(n, n)
+    pd = meshgen.uniform(n * n * 4, 0).reshape(n * n, 4)
+    w = meshgen.uniform(e.shape[0] * 4, 1).reshape(4, -1)
+    return Problem([e], [n * n], [pd], [w], 512), capi.functor(capi.KERNEL_MINE, 2, 4, 4), O.MINE2
+
+
+def _hex_128():
+    from cuda_fot_amd import Problem
+    n = 128
+    cells = meshgen.hex_grid(n + 1, n + 1, n + 1)
+    cells = cells[meshgen.permutation(cells.shape[0], 5)]
+    nn = (n + 1) ** 3
+    forces = meshgen.normal(cells.shape[0] * 24, 6).reshape(24, -1)
+    return (Problem([cells], [nn], [np.zeros((nn, 3))], [forces], 512),
+            capi.functor(capi.KERNEL_HEX_SCATTER), O.HEX_SCATTER)
+
+
+def _mini_aero_96():
+    from cuda_fot_amd import mini_aero_problem
+    return (mini_aero_problem(meshgen.hex_faces(96, 96, 96, seed=0), block_size=512),
+            capi.functor(capi.KERNEL_MINI_AERO), O.MINI_AERO)
+
+
+def _lulesh_128():
+    from cuda_fot_amd import lulesh_problem
+    return (lulesh_problem(129, 129, 129, seed=0, block_size=512),
+            capi.functor(capi.KERNEL_LULESH), O.LULESH)
+
+
+@pytest.mark.parametrize("make", [_mine_1024, _hex_128, _mini_aero_96, _lulesh_128],
+                         ids=["mine-1024^2", "hex-128^3", "mini_aero-96^3", "lulesh-128^3"])
+def test_other_workloads_at_benchmark_size(gpu, make):
+    # one step of owner compute (240-point parts, as bench.py runs it) and of
+    # the hierarchical loop on the GPS-ordered mesh (B = 256, the autotuner's
+    # pick for the 3D meshes): each bit for bit against the oracle's replay of
+    # its plan and, mapped back to the original numbering, within 1e-12 * S of
+    # the serial oracle loop
+    p, f, kernel = make()
+    print(f"{make.__name__}: {p.num_cells} cells", flush=True)
+    s = oracle_of(p, kernel)
+    S = s.loop_serial(1, with_scale=True)
+    q = copy.deepcopy(p)
+    q.partition_points(240)
+    q0 = copy.deepcopy(q)
+    q.gpu_loop(f, capi.STRATEGY_HIERARCHICAL, 1, block_size=240, flags=capi.LOOP_OWNER_COMPUTE)
+    o = oracle_of(q0, kernel)
+    O.loop_owner_replay(o, plan.owner_plan(q0.maps[0], q0.num_points[0], 240, None,
+                                           q0.point_partition), 1)
+    assert_bitwise(q.point_data[0], o.target, f"{make.__name__} owner compute vs replay")
+    assert_close_scaled(q.target_in_original_order(), s.target, S,
+                        what=f"{make.__name__} owner compute vs serial")
+    g = copy.deepcopy(p)
+    g.reorder("gps")
+    g0 = copy.deepcopy(g)
+    g.gpu_loop(f, capi.STRATEGY_HIERARCHICAL, 1, block_size=256, partition=False)
+    h = oracle_of(g0, kernel)
+    h.loop_hier_replay(O.OraclePlan(g0.maps, g0.num_points[0], 256, None), 1)
+    assert_bitwise(g.point_data[0], h.target, f"{make.__name__} GPS hierarchical vs replay")
+    assert_close_scaled(g.target_in_original_order(), s.target, S,
+                        what=f"{make.__name__} GPS hierarchical vs serial")
