@@ -106,3 +106,40 @@ def test_other_workloads_at_benchmark_size(gpu, make):
     assert_bitwise(g.point_data[0], h.target, f"{make.__name__} GPS hierarchical vs replay")
     assert_close_scaled(g.target_in_original_order(), s.target, S,
                         what=f"{make.__name__} GPS hierarchical vs serial")
+
+
+def test_volna_batch_at_benchmark_size(gpu):
+    # the 64 uneven triangle meshes of the batched benchmark (cells
+    # log-uniform in [2e4, 4e5]): one persistent launch per step over all of
+    # them, each mesh bit for bit equal to its own owner-compute loop; the
+    # smallest and the largest also against the oracle's replay of the plan
+    from cuda_fot_amd import gpu_batch, volna_problem
+    rng = np.random.default_rng(0)
+    shapes = []
+    for i in range(64):
+        cells = float(np.exp(rng.uniform(np.log(2e4), np.log(4e5))))
+        aspect = float(rng.uniform(0.5, 2.0))
+        nx = max(4, int(round(np.sqrt(cells / 2 * aspect))))
+        ny = max(4, int(round(cells / 2 / nx)))
+        shapes.append((nx, ny, 100 + i))
+
+    def make():
+        return [volna_problem(meshgen.tri_mesh(nx, ny, seed=s), block_size=240)
+                for nx, ny, s in shapes]
+    iters = 2
+    f = capi.functor(capi.KERNEL_VOLNA)
+    batch = make()
+    print(f"volna batch: {sum(p.num_cells for p in batch)} edges", flush=True)
+    refs = [copy.deepcopy(p) for p in batch]
+    gpu_batch(batch, f, iters)
+    sizes = [p.num_cells for p in batch]
+    check = {int(np.argmin(sizes)), int(np.argmax(sizes))}
+    for i, (b, r) in enumerate(zip(batch, refs)):
+        s = copy.deepcopy(r)
+        s.gpu_loop(f, capi.STRATEGY_HIERARCHICAL, iters, flags=capi.LOOP_OWNER_COMPUTE)
+        assert_bitwise(b.point_data[0], s.point_data[0], f"batched vs single, mesh {i}")
+        if i in check:
+            o = oracle_of(r, O.VOLNA)
+            O.loop_owner_replay(o, plan.owner_plan(r.maps[0], r.num_points[0], r.block_size),
+                                iters)
+            assert_bitwise(b.point_data[0], o.target, f"batched vs owner replay, mesh {i}")
