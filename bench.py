@@ -42,10 +42,12 @@ def parse():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="res_calc",
-                    choices=["res_calc", "volna-batch", "mini_aero", "hex", "lulesh", "mine"],
+                    choices=["res_calc", "volna-batch", "mini_aero", "hex", "lulesh", "mine",
+                             "getacc"],
                     help="res_calc: the airfoil edge loop (the headline line); the others "
                          "print their own line: 64 batched Volna meshes, mini_aero 96^3, "
-                         "hex scatter 128^3, lulesh element forces 128^3, mine on the "
+                         "hex scatter 128^3, lulesh element forces 128^3, getacc corner "
+                         "scatter on 1024^2 quads, mine on the "
                          "1024^2 grid with its serial and OpenMP CPU loops")
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
@@ -450,10 +452,20 @@ def tune_res_calc(args) -> dict:
 
 
 def bench_single(args, workload):
-    """mini_aero 96^3 faces, hex scatter or lulesh forces on 128^3 elements:
-    owner compute on a point-partitioned mesh beside plain atomics."""
-    from cuda_fot_amd import Problem, capi, lulesh_problem, meshgen, mini_aero_problem
-    if workload == "mine":
+    """mini_aero 96^3 faces, hex scatter or lulesh forces on 128^3 elements,
+    getacc on 1024^2 quads, mine on the 1024^2 grid: owner compute on a
+    point-partitioned mesh beside plain atomics."""
+    from cuda_fot_amd import Problem, capi, getacc_problem, lulesh_problem, meshgen, mini_aero_problem
+    if workload == "getacc":
+        n = 1024
+        p = getacc_problem(n + 1, n + 1, seed=0, block_size=512)
+        f = capi.functor(capi.KERNEL_GETACC)
+        nn, nc = (n + 1) ** 2, n * n
+        # ndmass/ndarea/ndub/ndvb read and written (64 B per node); cnmass,
+        # rho, cnwt, cnfx, cnfy (17 doubles) and 4 map ids (152 B per quad)
+        units, ubytes = nc, 64 * nn + 152 * nc
+        name = f"getacc corner scatter {n}x{n} quads, shuffled, seed 0"
+    elif workload == "mine":
         # the fillEdgeList grid as generate_grid lays it out (not shuffled),
         # point data and edge weights dim 4 U[0,1), seed 0
         n = 1024
@@ -567,7 +579,7 @@ def main():
         args.tune = args.tune or "none"
         print(json.dumps(bench_volna_batch(args)), flush=True)
         return
-    if args.workload in ("mini_aero", "hex", "lulesh", "mine"):
+    if args.workload in ("mini_aero", "hex", "lulesh", "mine", "getacc"):
         print(json.dumps(bench_single(args, args.workload)), flush=True)
         return
     tune = None
