@@ -10,7 +10,7 @@
 
 using fot::fail;
 
-extern "C" fot_status fot_loop_features(const fot_functor *f, const fot_problem_desc *d,
+static fot_status fot_loop_features_impl(const fot_functor *f, const fot_problem_desc *d,
                                         float *out) {
   if (!f || !d || !out) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
   if (d->num_mappings == 0 || d->num_cells == 0 || !d->cell_to_node[0])
@@ -89,4 +89,8 @@ extern "C" fot_status fot_loop_features(const fot_functor *f, const fot_problem_
   };
   std::copy(v, v + 24, out);
   return FOT_OK;
+}
+extern "C" fot_status fot_loop_features(const fot_functor *f, const fot_problem_desc *d,
+                                        float *out) {
+  return fot::guarded("fot_loop_features", [&] { return fot_loop_features_impl(f, d, out); });
 }

@@ -5,6 +5,8 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -15,6 +17,22 @@ namespace fot {
 // Records the message of the last failing call on this thread and returns
 // the status, so call sites read `return fail(FOT_ERR_..., "...")`.
 fot_status fail(fot_status status, const char *fmt, ...);
+
+// Every exported entry point runs its body through this, so no C++ exception
+// crosses the C ABI (the host planner allocates through the standard
+// library): out of host memory becomes FOT_ERR_OUT_OF_MEMORY, anything else
+// FOT_ERR_INVALID_ARGUMENT, with the message kept as for any failure.
+template <class Fn> fot_status guarded(const char *name, Fn &&fn) noexcept {
+  try {
+    return fn();
+  } catch (const std::bad_alloc &) {
+    return fail(FOT_ERR_OUT_OF_MEMORY, "%s: out of host memory", name);
+  } catch (const std::exception &e) {
+    return fail(FOT_ERR_INVALID_ARGUMENT, "%s: %s", name, e.what());
+  } catch (...) {
+    return fail(FOT_ERR_INVALID_ARGUMENT, "%s: unexpected exception", name);
+  }
+}
 
 // 128-bit colour set, the width of the reference's Mesh::colourset_t
 // (mesh.hpp:26).

@@ -1665,7 +1665,7 @@ fot_status fot_loop::enqueue(hipStream_t s) {
   return st;
 }
 
-extern "C" fot_status fot_loop_create(const fot_functor *f, const fot_problem_desc *desc,
+static fot_status fot_loop_create_impl(const fot_functor *f, const fot_problem_desc *desc,
                                       const fot_loop_config *cfg, fot_loop **out) {
   if (!f || !desc || !cfg || !out) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
   *out = nullptr;
@@ -1786,13 +1786,20 @@ extern "C" fot_status fot_loop_create(const fot_functor *f, const fot_problem_de
   *out = L.release();
   return FOT_OK;
 }
+extern "C" fot_status fot_loop_create(const fot_functor *f, const fot_problem_desc *desc,
+                                      const fot_loop_config *cfg, fot_loop **out) {
+  return fot::guarded("fot_loop_create", [&] { return fot_loop_create_impl(f, desc, cfg, out); });
+}
 
-extern "C" fot_status fot_loop_destroy(fot_loop *loop) {
+static fot_status fot_loop_destroy_impl(fot_loop *loop) {
   delete loop;
   return FOT_OK;
 }
+extern "C" fot_status fot_loop_destroy(fot_loop *loop) {
+  return fot::guarded("fot_loop_destroy", [&] { return fot_loop_destroy_impl(loop); });
+}
 
-extern "C" fot_status fot_loop_step(fot_loop *loop, void *stream) {
+static fot_status fot_loop_step_impl(fot_loop *loop, void *stream) {
   if (!loop) return fail(FOT_ERR_INVALID_ARGUMENT, "null loop");
   const hipStream_t s = static_cast<hipStream_t>(stream);
   if (loop->exec) {
@@ -1801,8 +1808,11 @@ extern "C" fot_status fot_loop_step(fot_loop *loop, void *stream) {
   }
   return loop->enqueue(s);
 }
+extern "C" fot_status fot_loop_step(fot_loop *loop, void *stream) {
+  return fot::guarded("fot_loop_step", [&] { return fot_loop_step_impl(loop, stream); });
+}
 
-extern "C" fot_status fot_loop_run(fot_loop *loop, uint32_t iters, void *stream) {
+static fot_status fot_loop_run_impl(fot_loop *loop, uint32_t iters, void *stream) {
   if (!loop) return fail(FOT_ERR_INVALID_ARGUMENT, "null loop");
   const fot_problem_desc &d = loop->desc;
   double *in = const_cast<double *>(d.point_data[0]);
@@ -1820,8 +1830,11 @@ extern "C" fot_status fot_loop_run(fot_loop *loop, uint32_t iters, void *stream)
   FOT_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
   return fot_loop_check(loop);
 }
+extern "C" fot_status fot_loop_run(fot_loop *loop, uint32_t iters, void *stream) {
+  return fot::guarded("fot_loop_run", [&] { return fot_loop_run_impl(loop, iters, stream); });
+}
 
-extern "C" fot_status fot_loop_check(const fot_loop *loop) {
+static fot_status fot_loop_check_impl(const fot_loop *loop) {
   if (!loop) return fail(FOT_ERR_INVALID_ARGUMENT, "null loop");
   if (!loop->dep_order) return FOT_OK;
   DepSync host{};
@@ -1830,11 +1843,17 @@ extern "C" fot_status fot_loop_check(const fot_loop *loop) {
     return fail(FOT_ERR_TIMEOUT, "a block waited too long for its predecessors");
   return FOT_OK;
 }
+extern "C" fot_status fot_loop_check(const fot_loop *loop) {
+  return fot::guarded("fot_loop_check", [&] { return fot_loop_check_impl(loop); });
+}
 
-extern "C" fot_status fot_loop_stats_get(const fot_loop *loop, fot_loop_stats *stats) {
+static fot_status fot_loop_stats_get_impl(const fot_loop *loop, fot_loop_stats *stats) {
   if (!loop || !stats) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
   *stats = loop->stats;
   return FOT_OK;
+}
+extern "C" fot_status fot_loop_stats_get(const fot_loop *loop, fot_loop_stats *stats) {
+  return fot::guarded("fot_loop_stats_get", [&] { return fot_loop_stats_get_impl(loop, stats); });
 }
 
 // ---------------------------------------------------------------------
@@ -1888,7 +1907,7 @@ struct fot_batch {
   }
 };
 
-extern "C" fot_status fot_batch_create(const fot_functor *f, uint32_t num_problems,
+static fot_status fot_batch_create_impl(const fot_functor *f, uint32_t num_problems,
                                        const fot_problem_desc *descs,
                                        const fot_loop_config *cfgs, uint32_t use_graph,
                                        fot_batch **out) {
@@ -1978,18 +1997,32 @@ extern "C" fot_status fot_batch_create(const fot_functor *f, uint32_t num_proble
   *out = B.release();
   return FOT_OK;
 }
+extern "C" fot_status fot_batch_create(const fot_functor *f, uint32_t num_problems,
+                                       const fot_problem_desc *descs,
+                                       const fot_loop_config *cfgs, uint32_t use_graph,
+                                       fot_batch **out) {
+  return fot::guarded("fot_batch_create", [&] {
+    return fot_batch_create_impl(f, num_problems, descs, cfgs, use_graph, out);
+  });
+}
 
-extern "C" fot_status fot_batch_destroy(fot_batch *b) {
+static fot_status fot_batch_destroy_impl(fot_batch *b) {
   delete b;
   return FOT_OK;
 }
+extern "C" fot_status fot_batch_destroy(fot_batch *b) {
+  return fot::guarded("fot_batch_destroy", [&] { return fot_batch_destroy_impl(b); });
+}
 
-extern "C" fot_status fot_batch_step(fot_batch *b, void *stream) {
+static fot_status fot_batch_step_impl(fot_batch *b, void *stream) {
   if (!b) return fail(FOT_ERR_INVALID_ARGUMENT, "null batch");
   return b->launch_step(static_cast<hipStream_t>(stream));
 }
+extern "C" fot_status fot_batch_step(fot_batch *b, void *stream) {
+  return fot::guarded("fot_batch_step", [&] { return fot_batch_step_impl(b, stream); });
+}
 
-extern "C" fot_status fot_batch_run(fot_batch *b, uint32_t iters, void *stream) {
+static fot_status fot_batch_run_impl(fot_batch *b, uint32_t iters, void *stream) {
   if (!b) return fail(FOT_ERR_INVALID_ARGUMENT, "null batch");
   hipStream_t s = static_cast<hipStream_t>(stream);
   fot_status st = b->launch_copy(b->copy_in, s);
@@ -2008,8 +2041,11 @@ extern "C" fot_status fot_batch_run(fot_batch *b, uint32_t iters, void *stream) 
   FOT_HIP(hipStreamSynchronize(s));
   return FOT_OK;
 }
+extern "C" fot_status fot_batch_run(fot_batch *b, uint32_t iters, void *stream) {
+  return fot::guarded("fot_batch_run", [&] { return fot_batch_run_impl(b, iters, stream); });
+}
 
-extern "C" fot_status fot_batch_stats_get(const fot_batch *b, fot_batch_stats *s) {
+static fot_status fot_batch_stats_get_impl(const fot_batch *b, fot_batch_stats *s) {
   if (!b || !s) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
   s->num_problems = uint32_t(b->loops.size());
   s->num_work_items = b->nwork;
@@ -2018,6 +2054,9 @@ extern "C" fot_status fot_batch_stats_get(const fot_batch *b, fot_batch_stats *s
   s->lds_bytes = b->lds_bytes;
   s->uses_graph = b->exec ? 1u : 0u;
   return FOT_OK;
+}
+extern "C" fot_status fot_batch_stats_get(const fot_batch *b, fot_batch_stats *s) {
+  return fot::guarded("fot_batch_stats_get", [&] { return fot_batch_stats_get_impl(b, s); });
 }
 
 // ---------------------------------------------------------------------
@@ -2033,17 +2072,27 @@ fot_status ctx_stream(fot_ctx *ctx, hipStream_t *s) {
 }
 } // namespace
 
-extern "C" fot_status fot_plan_upload(fot_ctx *ctx, const fot_functor *f,
+static fot_status fot_plan_upload_impl(fot_ctx *ctx, const fot_functor *f,
                                       const fot_problem_desc *desc, const fot_loop_config *cfg,
                                       fot_plan **plan) {
   if (!ctx || !plan) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
   *plan = nullptr;
   return fot_loop_create(f, desc, cfg, plan);
 }
+extern "C" fot_status fot_plan_upload(fot_ctx *ctx, const fot_functor *f,
+                                      const fot_problem_desc *desc, const fot_loop_config *cfg,
+                                      fot_plan **plan) {
+  return fot::guarded("fot_plan_upload", [&] {
+    return fot_plan_upload_impl(ctx, f, desc, cfg, plan);
+  });
+}
 
-extern "C" fot_status fot_plan_free(fot_plan *plan) { return fot_loop_destroy(plan); }
+static fot_status fot_plan_free_impl(fot_plan *plan) { return fot_loop_destroy(plan); }
+extern "C" fot_status fot_plan_free(fot_plan *plan) {
+  return fot::guarded("fot_plan_free", [&] { return fot_plan_free_impl(plan); });
+}
 
-extern "C" fot_status fot_loop_launch(fot_ctx *ctx, const fot_functor *f,
+static fot_status fot_loop_launch_impl(fot_ctx *ctx, const fot_functor *f,
                                       const fot_problem_desc *desc, const fot_loop_config *cfg,
                                       uint32_t iters) {
   hipStream_t s = nullptr;
@@ -2055,6 +2104,13 @@ extern "C" fot_status fot_loop_launch(fot_ctx *ctx, const fot_functor *f,
   fot_loop_destroy(loop);
   return st;
 }
+extern "C" fot_status fot_loop_launch(fot_ctx *ctx, const fot_functor *f,
+                                      const fot_problem_desc *desc, const fot_loop_config *cfg,
+                                      uint32_t iters) {
+  return fot::guarded("fot_loop_launch", [&] {
+    return fot_loop_launch_impl(ctx, f, desc, cfg, iters);
+  });
+}
 
 struct fot_graph {
   hipGraph_t graph = nullptr;
@@ -2065,7 +2121,7 @@ struct fot_graph {
   }
 };
 
-extern "C" fot_status fot_graph_capture(fot_ctx *ctx, fot_plan *plan, fot_graph **out) {
+static fot_status fot_graph_capture_impl(fot_ctx *ctx, fot_plan *plan, fot_graph **out) {
   if (!ctx || !plan || !out) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
   *out = nullptr;
   std::unique_ptr<fot_graph> G(new fot_graph);
@@ -2088,8 +2144,11 @@ extern "C" fot_status fot_graph_capture(fot_ctx *ctx, fot_plan *plan, fot_graph 
   *out = G.release();
   return FOT_OK;
 }
+extern "C" fot_status fot_graph_capture(fot_ctx *ctx, fot_plan *plan, fot_graph **out) {
+  return fot::guarded("fot_graph_capture", [&] { return fot_graph_capture_impl(ctx, plan, out); });
+}
 
-extern "C" fot_status fot_graph_launch(fot_ctx *ctx, fot_graph *graph) {
+static fot_status fot_graph_launch_impl(fot_ctx *ctx, fot_graph *graph) {
   if (!ctx || !graph) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
   if (!graph->exec) return FOT_OK; // a plan over no cells
   hipStream_t s = nullptr;
@@ -2098,25 +2157,34 @@ extern "C" fot_status fot_graph_launch(fot_ctx *ctx, fot_graph *graph) {
   FOT_HIP(hipGraphLaunch(graph->exec, s));
   return FOT_OK;
 }
+extern "C" fot_status fot_graph_launch(fot_ctx *ctx, fot_graph *graph) {
+  return fot::guarded("fot_graph_launch", [&] { return fot_graph_launch_impl(ctx, graph); });
+}
 
-extern "C" fot_status fot_graph_free(fot_graph *graph) {
+static fot_status fot_graph_free_impl(fot_graph *graph) {
   delete graph;
   return FOT_OK;
 }
+extern "C" fot_status fot_graph_free(fot_graph *graph) {
+  return fot::guarded("fot_graph_free", [&] { return fot_graph_free_impl(graph); });
+}
 
-extern "C" fot_status fot_batch_launch(fot_ctx *ctx, fot_batch *batch, uint32_t iters) {
+static fot_status fot_batch_launch_impl(fot_ctx *ctx, fot_batch *batch, uint32_t iters) {
   if (!ctx || !batch) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
   hipStream_t s = nullptr;
   const fot_status st = ctx_stream(ctx, &s);
   if (st != FOT_OK) return st;
   return fot_batch_run(batch, iters, s);
 }
+extern "C" fot_status fot_batch_launch(fot_ctx *ctx, fot_batch *batch, uint32_t iters) {
+  return fot::guarded("fot_batch_launch", [&] { return fot_batch_launch_impl(ctx, batch, iters); });
+}
 
 // ---------------------------------------------------------------------
 // Autotuner measurement: each candidate config planned and timed on a
 // private stream with HIP events (no reference counterpart).
 // ---------------------------------------------------------------------
-extern "C" fot_status fot_autotune_measure(const fot_functor *f, const fot_problem_desc *desc,
+static fot_status fot_autotune_measure_impl(const fot_functor *f, const fot_problem_desc *desc,
                                            const fot_loop_config *cands, uint32_t n,
                                            uint32_t warmup, uint32_t steps, float *times_ms) {
   if (!f || !desc || !cands || !times_ms || n == 0 || steps == 0)
@@ -2126,8 +2194,12 @@ extern "C" fot_status fot_autotune_measure(const fot_functor *f, const fot_probl
   FOT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   std::unique_ptr<std::remove_pointer<hipStream_t>::type, hipError_t (*)(hipStream_t)> sg(
       s, hipStreamDestroy);
+  using EventGuard = std::unique_ptr<std::remove_pointer<hipEvent_t>::type,
+                                     hipError_t (*)(hipEvent_t)>;
   FOT_HIP(hipEventCreate(&e0));
+  EventGuard g0(e0, hipEventDestroy);
   FOT_HIP(hipEventCreate(&e1));
+  EventGuard g1(e1, hipEventDestroy);
   fot_status result = FOT_OK;
   for (uint32_t i = 0; i < n && result == FOT_OK; ++i) {
     times_ms[i] = -1.f;
@@ -2153,7 +2225,12 @@ extern "C" fot_status fot_autotune_measure(const fot_functor *f, const fot_probl
     if (result == FOT_OK) result = fot_loop_check(l);
     times_ms[i] = ms / float(steps);
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   return result;
+}
+extern "C" fot_status fot_autotune_measure(const fot_functor *f, const fot_problem_desc *desc,
+                                           const fot_loop_config *cands, uint32_t n,
+                                           uint32_t warmup, uint32_t steps, float *times_ms) {
+  return fot::guarded("fot_autotune_measure", [&] {
+    return fot_autotune_measure_impl(f, desc, cands, n, warmup, steps, times_ms);
+  });
 }

@@ -46,32 +46,49 @@ void permutation(uint32_t n, uint64_t seed, uint32_t *perm) {
 
 } // namespace
 
-extern "C" fot_status fot_gen_permutation(uint32_t n, uint64_t seed, uint32_t *perm) {
+static fot_status fot_gen_permutation_impl(uint32_t n, uint64_t seed, uint32_t *perm) {
   if (!perm && n) return fail(FOT_ERR_INVALID_ARGUMENT, "null perm");
   permutation(n, seed, perm);
   return FOT_OK;
 }
+extern "C" fot_status fot_gen_permutation(uint32_t n, uint64_t seed, uint32_t *perm) {
+  return fot::guarded("fot_gen_permutation", [&] {
+    return fot_gen_permutation_impl(n, seed, perm);
+  });
+}
 
-extern "C" fot_status fot_gen_uniform(uint64_t n, uint64_t seed, double lo, double hi,
+static fot_status fot_gen_uniform_impl(uint64_t n, uint64_t seed, double lo, double hi,
                                       double *out) {
   if (!out && n) return fail(FOT_ERR_INVALID_ARGUMENT, "null out");
   SplitMix64 g(seed);
   for (uint64_t i = 0; i < n; ++i) out[i] = lo + (hi - lo) * g.uniform();
   return FOT_OK;
 }
+extern "C" fot_status fot_gen_uniform(uint64_t n, uint64_t seed, double lo, double hi,
+                                      double *out) {
+  return fot::guarded("fot_gen_uniform", [&] {
+    return fot_gen_uniform_impl(n, seed, lo, hi, out);
+  });
+}
 
-extern "C" fot_status fot_gen_normal(uint64_t n, uint64_t seed, double mean, double sd,
+static fot_status fot_gen_normal_impl(uint64_t n, uint64_t seed, double mean, double sd,
                                      double *out) {
   if (!out && n) return fail(FOT_ERR_INVALID_ARGUMENT, "null out");
   SplitMix64 g(seed);
   for (uint64_t i = 0; i < n; ++i) out[i] = mean + sd * g.normal();
   return FOT_OK;
 }
+extern "C" fot_status fot_gen_normal(uint64_t n, uint64_t seed, double mean, double sd,
+                                     double *out) {
+  return fot::guarded("fot_gen_normal", [&] {
+    return fot_gen_normal_impl(n, seed, mean, sd, out);
+  });
+}
 
 // Grid::fillEdgeList (grid.hpp:77-104): for each row but the last, a
 // (down, right) edge pair per column but the last, then the last column's
 // down edge (listed lower point first); finally the last row's right edges.
-extern "C" fot_status fot_gen_grid_edges(uint32_t N, uint32_t M, uint32_t *e) {
+static fot_status fot_gen_grid_edges_impl(uint32_t N, uint32_t M, uint32_t *e) {
   if (!e || N < 2 || M < 2) return fail(FOT_ERR_INVALID_ARGUMENT, "grid needs N,M >= 2");
   size_t k = 0;
   uint32_t upper = 0, lower = M;
@@ -92,10 +109,13 @@ extern "C" fot_status fot_gen_grid_edges(uint32_t N, uint32_t M, uint32_t *e) {
   }
   return FOT_OK;
 }
+extern "C" fot_status fot_gen_grid_edges(uint32_t N, uint32_t M, uint32_t *e) {
+  return fot::guarded("fot_gen_grid_edges", [&] { return fot_gen_grid_edges_impl(N, M, e); });
+}
 
 // Grid::fillCellList3D (grid.hpp:415-460): corner order (0,0,0) (0,1,0)
 // (1,1,0) (1,0,0) then the same on the layer above.
-extern "C" fot_status fot_gen_hex_grid(uint32_t N1, uint32_t N2, uint32_t N3,
+static fot_status fot_gen_hex_grid_impl(uint32_t N1, uint32_t N2, uint32_t N3,
                                        uint32_t *cells) {
   if (!cells || N1 < 2 || N2 < 2 || N3 < 2)
     return fail(FOT_ERR_INVALID_ARGUMENT, "hex grid needs N >= 2");
@@ -116,9 +136,13 @@ extern "C" fot_status fot_gen_hex_grid(uint32_t N1, uint32_t N2, uint32_t N3,
       }
   return FOT_OK;
 }
+extern "C" fot_status fot_gen_hex_grid(uint32_t N1, uint32_t N2, uint32_t N3,
+                                       uint32_t *cells) {
+  return fot::guarded("fot_gen_hex_grid", [&] { return fot_gen_hex_grid_impl(N1, N2, N3, cells); });
+}
 
 // Grid::fillCellList (grid.hpp:386-411): quads of an N x M point grid
-extern "C" fot_status fot_gen_quad_grid(uint32_t N, uint32_t M, uint32_t *cells) {
+static fot_status fot_gen_quad_grid_impl(uint32_t N, uint32_t M, uint32_t *cells) {
   if (!cells || N < 2 || M < 2) return fail(FOT_ERR_INVALID_ARGUMENT, "quad grid needs N,M >= 2");
   size_t k = 0;
   for (uint32_t r = 0; r + 1 < N; ++r)
@@ -131,11 +155,14 @@ extern "C" fot_status fot_gen_quad_grid(uint32_t N, uint32_t M, uint32_t *cells)
     }
   return FOT_OK;
 }
+extern "C" fot_status fot_gen_quad_grid(uint32_t N, uint32_t M, uint32_t *cells) {
+  return fot::guarded("fot_gen_quad_grid", [&] { return fot_gen_quad_grid_impl(N, M, cells); });
+}
 
 // Airfoil-like structured quad mesh with interior edges only.  Edge order
 // before shuffling follows the cells row by row: for cell (i, j) the edge
 // below it (j > 0), then the edge left of it (i > 0).
-extern "C" fot_status fot_gen_airfoil(uint32_t nx, uint32_t ny, uint64_t seed,
+static fot_status fot_gen_airfoil_impl(uint32_t nx, uint32_t ny, uint64_t seed,
                                       uint32_t shuffle, uint32_t *pedge,
                                       uint32_t *pecell, double *x, double *q,
                                       double *adt) {
@@ -202,6 +229,14 @@ extern "C" fot_status fot_gen_airfoil(uint32_t nx, uint32_t ny, uint64_t seed,
   }
   return FOT_OK;
 }
+extern "C" fot_status fot_gen_airfoil(uint32_t nx, uint32_t ny, uint64_t seed,
+                                      uint32_t shuffle, uint32_t *pedge,
+                                      uint32_t *pecell, double *x, double *q,
+                                      double *adt) {
+  return fot::guarded("fot_gen_airfoil", [&] {
+    return fot_gen_airfoil_impl(nx, ny, seed, shuffle, pedge, pecell, x, q, adt);
+  });
+}
 
 namespace {
 struct Pt {
@@ -217,7 +252,7 @@ struct Pt {
 // vertical, diagonal.  An interior edge maps to (left, right) with its unit
 // normal pointing from left to right; a boundary edge maps to (cell, cell)
 // with is_boundary 1 and the outward normal.
-extern "C" fot_status fot_gen_tri_mesh(uint32_t nx, uint32_t ny, uint64_t seed, uint32_t shuffle,
+static fot_status fot_gen_tri_mesh_impl(uint32_t nx, uint32_t ny, uint64_t seed, uint32_t shuffle,
                                        uint32_t *edge_cells, double *is_boundary,
                                        double *normals, double *fluxes, double *bathy,
                                        double *volume, double *state) {
@@ -347,9 +382,18 @@ extern "C" fot_status fot_gen_tri_mesh(uint32_t nx, uint32_t ny, uint64_t seed, 
   }
   return FOT_OK;
 }
+extern "C" fot_status fot_gen_tri_mesh(uint32_t nx, uint32_t ny, uint64_t seed, uint32_t shuffle,
+                                       uint32_t *edge_cells, double *is_boundary,
+                                       double *normals, double *fluxes, double *bathy,
+                                       double *volume, double *state) {
+  return fot::guarded("fot_gen_tri_mesh", [&] {
+    return fot_gen_tri_mesh_impl(nx, ny, seed, shuffle, edge_cells, is_boundary, normals, fluxes,
+                                 bathy, volume, state);
+  });
+}
 
 // mini_aero-style hex face mesh (declared in fot_capi.h)
-extern "C" fot_status fot_gen_hex_faces(uint32_t n1, uint32_t n2, uint32_t n3, uint64_t seed,
+static fot_status fot_gen_hex_faces_impl(uint32_t n1, uint32_t n2, uint32_t n3, uint64_t seed,
                                         uint32_t shuffle, uint32_t *face_cells, double *face,
                                         double *cells) {
   if (n1 < 1 || n2 < 1 || n3 < 1) return fail(FOT_ERR_INVALID_ARGUMENT, "hex faces need n >= 1");
@@ -409,4 +453,11 @@ extern "C" fot_status fot_gen_hex_faces(uint32_t n1, uint32_t n2, uint32_t n3, u
         }
   }
   return FOT_OK;
+}
+extern "C" fot_status fot_gen_hex_faces(uint32_t n1, uint32_t n2, uint32_t n3, uint64_t seed,
+                                        uint32_t shuffle, uint32_t *face_cells, double *face,
+                                        double *cells) {
+  return fot::guarded("fot_gen_hex_faces", [&] {
+    return fot_gen_hex_faces_impl(n1, n2, n3, seed, shuffle, face_cells, face, cells);
+  });
 }

@@ -264,7 +264,7 @@ static float mask_count(const float *mask, size_t n) {
   return float(std::max(c, 1.0));
 }
 
-extern "C" fot_status fot_mlp_create(uint32_t in, uint32_t hidden, uint32_t out, uint64_t seed,
+static fot_status fot_mlp_create_impl(uint32_t in, uint32_t hidden, uint32_t out, uint64_t seed,
                                      fot_mlp **mlp) {
   if (!mlp || in == 0 || hidden == 0 || out == 0 || in > 4096 || hidden > 4096 || out > 4096)
     return fail(FOT_ERR_INVALID_ARGUMENT, "bad MLP shape");
@@ -298,25 +298,40 @@ extern "C" fot_status fot_mlp_create(uint32_t in, uint32_t hidden, uint32_t out,
   *mlp = M.release();
   return FOT_OK;
 }
+extern "C" fot_status fot_mlp_create(uint32_t in, uint32_t hidden, uint32_t out, uint64_t seed,
+                                     fot_mlp **mlp) {
+  return fot::guarded("fot_mlp_create", [&] {
+    return fot_mlp_create_impl(in, hidden, out, seed, mlp);
+  });
+}
 
-extern "C" fot_status fot_mlp_destroy(fot_mlp *mlp) {
+static fot_status fot_mlp_destroy_impl(fot_mlp *mlp) {
   delete mlp;
   return FOT_OK;
 }
+extern "C" fot_status fot_mlp_destroy(fot_mlp *mlp) {
+  return fot::guarded("fot_mlp_destroy", [&] { return fot_mlp_destroy_impl(mlp); });
+}
 
-extern "C" fot_status fot_mlp_num_params(const fot_mlp *mlp, uint64_t *n) {
+static fot_status fot_mlp_num_params_impl(const fot_mlp *mlp, uint64_t *n) {
   if (!mlp || !n) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
   *n = mlp->nparam;
   return FOT_OK;
 }
+extern "C" fot_status fot_mlp_num_params(const fot_mlp *mlp, uint64_t *n) {
+  return fot::guarded("fot_mlp_num_params", [&] { return fot_mlp_num_params_impl(mlp, n); });
+}
 
-extern "C" fot_status fot_mlp_get_params(const fot_mlp *mlp, float *host, uint64_t n) {
+static fot_status fot_mlp_get_params_impl(const fot_mlp *mlp, float *host, uint64_t n) {
   if (!mlp || !host || n != mlp->nparam) return fail(FOT_ERR_INVALID_ARGUMENT, "bad params");
   FOT_HIP(hipMemcpy(host, mlp->p, n * sizeof(float), hipMemcpyDeviceToHost));
   return FOT_OK;
 }
+extern "C" fot_status fot_mlp_get_params(const fot_mlp *mlp, float *host, uint64_t n) {
+  return fot::guarded("fot_mlp_get_params", [&] { return fot_mlp_get_params_impl(mlp, host, n); });
+}
 
-extern "C" fot_status fot_mlp_set_params(fot_mlp *mlp, const float *host, uint64_t n) {
+static fot_status fot_mlp_set_params_impl(fot_mlp *mlp, const float *host, uint64_t n) {
   if (!mlp || !host || n != mlp->nparam) return fail(FOT_ERR_INVALID_ARGUMENT, "bad params");
   FOT_HIP(hipMemcpy(mlp->p, host, n * sizeof(float), hipMemcpyHostToDevice));
   FOT_HIP(hipMemset(mlp->m, 0, mlp->nparam * sizeof(float)));
@@ -324,8 +339,11 @@ extern "C" fot_status fot_mlp_set_params(fot_mlp *mlp, const float *host, uint64
   mlp->step = 0;
   return FOT_OK;
 }
+extern "C" fot_status fot_mlp_set_params(fot_mlp *mlp, const float *host, uint64_t n) {
+  return fot::guarded("fot_mlp_set_params", [&] { return fot_mlp_set_params_impl(mlp, host, n); });
+}
 
-extern "C" fot_status fot_mlp_forward(fot_mlp *mlp, const float *x, uint32_t rows, float *y) {
+static fot_status fot_mlp_forward_impl(fot_mlp *mlp, const float *x, uint32_t rows, float *y) {
   if (!mlp || !x || !y) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
   if (rows == 0) return FOT_OK;
   fot_status st;
@@ -334,8 +352,11 @@ extern "C" fot_status fot_mlp_forward(fot_mlp *mlp, const float *x, uint32_t row
   FOT_HIP(hipMemcpy(y, mlp->Y, sizeof(float) * rows * mlp->O, hipMemcpyDeviceToHost));
   return FOT_OK;
 }
+extern "C" fot_status fot_mlp_forward(fot_mlp *mlp, const float *x, uint32_t rows, float *y) {
+  return fot::guarded("fot_mlp_forward", [&] { return fot_mlp_forward_impl(mlp, x, rows, y); });
+}
 
-extern "C" fot_status fot_mlp_loss_grad(fot_mlp *mlp, const float *x, const float *t,
+static fot_status fot_mlp_loss_grad_impl(fot_mlp *mlp, const float *x, const float *t,
                                         const float *mask, uint32_t rows, float *loss,
                                         float *grad) {
   if (!mlp || !x || !t || !loss || !grad || rows == 0)
@@ -348,8 +369,15 @@ extern "C" fot_status fot_mlp_loss_grad(fot_mlp *mlp, const float *x, const floa
   FOT_HIP(hipMemcpy(grad, mlp->g, mlp->nparam * sizeof(float), hipMemcpyDeviceToHost));
   return FOT_OK;
 }
+extern "C" fot_status fot_mlp_loss_grad(fot_mlp *mlp, const float *x, const float *t,
+                                        const float *mask, uint32_t rows, float *loss,
+                                        float *grad) {
+  return fot::guarded("fot_mlp_loss_grad", [&] {
+    return fot_mlp_loss_grad_impl(mlp, x, t, mask, rows, loss, grad);
+  });
+}
 
-extern "C" fot_status fot_mlp_train(fot_mlp *mlp, const float *x, const float *t,
+static fot_status fot_mlp_train_impl(fot_mlp *mlp, const float *x, const float *t,
                                     const float *mask, uint32_t rows, uint32_t epochs, float lr,
                                     float weight_decay, float *loss_per_epoch) {
   if (!mlp || !x || !t || rows == 0) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
@@ -375,10 +403,17 @@ extern "C" fot_status fot_mlp_train(fot_mlp *mlp, const float *x, const float *t
   FOT_HIP(hipDeviceSynchronize());
   return FOT_OK;
 }
+extern "C" fot_status fot_mlp_train(fot_mlp *mlp, const float *x, const float *t,
+                                    const float *mask, uint32_t rows, uint32_t epochs, float lr,
+                                    float weight_decay, float *loss_per_epoch) {
+  return fot::guarded("fot_mlp_train", [&] {
+    return fot_mlp_train_impl(mlp, x, t, mask, rows, epochs, lr, weight_decay, loss_per_epoch);
+  });
+}
 
 // The MFMA GEMM on device buffers: C[M][N] = op(A) op(B) with row-major A
 // (M x K, or K x M when trans_a) and B (K x N, or N x K when trans_b).
-extern "C" fot_status fot_gemm_f32(const float *A, const float *B, float *C, uint32_t M,
+static fot_status fot_gemm_f32_impl(const float *A, const float *B, float *C, uint32_t M,
                                    uint32_t N, uint32_t K, int trans_a, int trans_b,
                                    void *stream) {
   if ((!A || !B || !C) && M && N) return fail(FOT_ERR_INVALID_ARGUMENT, "null matrix");
@@ -386,4 +421,11 @@ extern "C" fot_status fot_gemm_f32(const float *A, const float *B, float *C, uin
   const uint32_t sbk = trans_b ? 1 : N, sbn = trans_b ? K : 1;
   return gemm(A, B, C, nullptr, M, N, K, sam, sak, sbk, sbn, N, false, false,
               static_cast<hipStream_t>(stream));
+}
+extern "C" fot_status fot_gemm_f32(const float *A, const float *B, float *C, uint32_t M,
+                                   uint32_t N, uint32_t K, int trans_a, int trans_b,
+                                   void *stream) {
+  return fot::guarded("fot_gemm_f32", [&] {
+    return fot_gemm_f32_impl(A, B, C, M, N, K, trans_a, trans_b, stream);
+  });
 }

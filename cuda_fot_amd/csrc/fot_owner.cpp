@@ -219,7 +219,7 @@ struct fot_oplan {
   fot::OwnerPlan plan;
 };
 
-extern "C" fot_status fot_oplan_build(const uint32_t *c2n0, uint32_t num_cells,
+static fot_status fot_oplan_build_impl(const uint32_t *c2n0, uint32_t num_cells,
                                       uint32_t mesh_dim, uint32_t num_points,
                                       const uint32_t *partition, uint32_t block_size,
                                       const uint32_t *point_partition, fot_oplan **out) {
@@ -236,10 +236,19 @@ extern "C" fot_status fot_oplan_build(const uint32_t *c2n0, uint32_t num_cells,
   *out = p;
   return FOT_OK;
 }
+extern "C" fot_status fot_oplan_build(const uint32_t *c2n0, uint32_t num_cells,
+                                      uint32_t mesh_dim, uint32_t num_points,
+                                      const uint32_t *partition, uint32_t block_size,
+                                      const uint32_t *point_partition, fot_oplan **out) {
+  return fot::guarded("fot_oplan_build", [&] {
+    return fot_oplan_build_impl(c2n0, num_cells, mesh_dim, num_points, partition, block_size,
+                                point_partition, out);
+  });
+}
 
 extern "C" void fot_oplan_free(fot_oplan *plan) { delete plan; }
 
-extern "C" fot_status fot_oplan_get(const fot_oplan *plan, fot_oplan_view *v) {
+static fot_status fot_oplan_get_impl(const fot_oplan *plan, fot_oplan_view *v) {
   if (!plan || !v) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
   const fot::OwnerPlan &p = plan->plan;
   v->mesh_dim = p.mesh_dim;
@@ -255,4 +264,7 @@ extern "C" fot_status fot_oplan_get(const fot_oplan *plan, fot_oplan_view *v) {
   v->colour = p.colour.data();
   v->blk_ncol = p.blk_ncol.data();
   return FOT_OK;
+}
+extern "C" fot_status fot_oplan_get(const fot_oplan *plan, fot_oplan_view *v) {
+  return fot::guarded("fot_oplan_get", [&] { return fot_oplan_get_impl(plan, v); });
 }

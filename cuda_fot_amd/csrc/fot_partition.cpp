@@ -74,7 +74,7 @@ private:
 
 } // namespace
 
-extern "C" fot_status fot_heuristic_partition(const uint32_t *ci,
+static fot_status fot_heuristic_partition_impl(const uint32_t *ci,
                                               const uint32_t *ce, uint32_t n,
                                               const uint32_t *coarse_in,
                                               uint32_t block_size,
@@ -160,6 +160,16 @@ extern "C" fot_status fot_heuristic_partition(const uint32_t *ci,
   for (uint32_t i = 0; i < n; ++i) result_out[inv[i]] = result[i];
   *num_blocks = block_id;
   return FOT_OK;
+}
+extern "C" fot_status fot_heuristic_partition(const uint32_t *ci,
+                                              const uint32_t *ce, uint32_t n,
+                                              const uint32_t *coarse_in,
+                                              uint32_t block_size,
+                                              uint32_t *result_out,
+                                              uint32_t *num_blocks) {
+  return fot::guarded("fot_heuristic_partition", [&] {
+    return fot_heuristic_partition_impl(ci, ce, n, coarse_in, block_size, result_out, num_blocks);
+  });
 }
 
 namespace {
@@ -248,7 +258,7 @@ void cuthill_mckee(const uint32_t *ci, const uint32_t *ce, uint32_t n,
 
 } // namespace
 
-extern "C" fot_status fot_rcm_order(const uint32_t *ci, const uint32_t *ce,
+static fot_status fot_rcm_order_impl(const uint32_t *ci, const uint32_t *ce,
                                     uint32_t n, uint32_t *perm) {
   if (!ci || !perm || (n && !ce && ci[n])) return fail(FOT_ERR_INVALID_ARGUMENT, "null");
   std::vector<uint32_t> order;
@@ -256,8 +266,12 @@ extern "C" fot_status fot_rcm_order(const uint32_t *ci, const uint32_t *ce,
   for (uint32_t i = 0; i < n; ++i) perm[order[i]] = n - 1 - i; // reversed
   return FOT_OK;
 }
+extern "C" fot_status fot_rcm_order(const uint32_t *ci, const uint32_t *ce,
+                                    uint32_t n, uint32_t *perm) {
+  return fot::guarded("fot_rcm_order", [&] { return fot_rcm_order_impl(ci, ce, n, perm); });
+}
 
-extern "C" fot_status fot_gps_order(const uint32_t *ci, const uint32_t *ce,
+static fot_status fot_gps_order_impl(const uint32_t *ci, const uint32_t *ce,
                                     uint32_t n, uint32_t *perm) {
   if (!ci || !perm || (n && !ce && ci[n])) return fail(FOT_ERR_INVALID_ARGUMENT, "null");
   // Level-structure numbering from a pseudo-diameter endpoint (the first
@@ -267,8 +281,12 @@ extern "C" fot_status fot_gps_order(const uint32_t *ci, const uint32_t *ce,
   for (uint32_t i = 0; i < n; ++i) perm[order[i]] = i;
   return FOT_OK;
 }
+extern "C" fot_status fot_gps_order(const uint32_t *ci, const uint32_t *ce,
+                                    uint32_t n, uint32_t *perm) {
+  return fot::guarded("fot_gps_order", [&] { return fot_gps_order_impl(ci, ce, n, perm); });
+}
 
-extern "C" fot_status fot_kway_partition(const uint32_t *ci, const uint32_t *ce,
+static fot_status fot_kway_partition_impl(const uint32_t *ci, const uint32_t *ce,
                                          uint32_t n, uint32_t max_part,
                                          uint32_t *part, uint32_t *num_parts) {
   if (!ci || !part || !num_parts || max_part == 0)
@@ -318,4 +336,11 @@ extern "C" fot_status fot_kway_partition(const uint32_t *ci, const uint32_t *ce,
   }
   *num_parts = pid;
   return FOT_OK;
+}
+extern "C" fot_status fot_kway_partition(const uint32_t *ci, const uint32_t *ce,
+                                         uint32_t n, uint32_t max_part,
+                                         uint32_t *part, uint32_t *num_parts) {
+  return fot::guarded("fot_kway_partition", [&] {
+    return fot_kway_partition_impl(ci, ce, n, max_part, part, num_parts);
+  });
 }

@@ -166,7 +166,7 @@ namespace {
 using fot::colour_block;
 } // namespace
 
-extern "C" fot_status fot_colour_cells(const uint32_t *c2n, uint32_t num_cells,
+static fot_status fot_colour_cells_impl(const uint32_t *c2n, uint32_t num_cells,
                                        uint32_t mesh_dim, uint32_t num_points,
                                        uint8_t *colour_out,
                                        uint32_t *num_colours) {
@@ -202,8 +202,16 @@ extern "C" fot_status fot_colour_cells(const uint32_t *c2n, uint32_t num_cells,
   *num_colours = ncol;
   return FOT_OK;
 }
+extern "C" fot_status fot_colour_cells(const uint32_t *c2n, uint32_t num_cells,
+                                       uint32_t mesh_dim, uint32_t num_points,
+                                       uint8_t *colour_out,
+                                       uint32_t *num_colours) {
+  return fot::guarded("fot_colour_cells", [&] {
+    return fot_colour_cells_impl(c2n, num_cells, mesh_dim, num_points, colour_out, num_colours);
+  });
+}
 
-extern "C" fot_status fot_colour_cells_with_ordering(const uint32_t *mapping,
+static fot_status fot_colour_cells_with_ordering_impl(const uint32_t *mapping,
                                                      uint32_t num_cells,
                                                      uint32_t mapping_dim,
                                                      uint8_t *colours_out,
@@ -214,6 +222,16 @@ extern "C" fot_status fot_colour_cells_with_ordering(const uint32_t *mapping,
   if (n < 0) return fail(FOT_ERR_COLOUR_OVERFLOW, "more than 128 thread colours");
   *num_colours = uint32_t(n);
   return FOT_OK;
+}
+extern "C" fot_status fot_colour_cells_with_ordering(const uint32_t *mapping,
+                                                     uint32_t num_cells,
+                                                     uint32_t mapping_dim,
+                                                     uint8_t *colours_out,
+                                                     uint32_t *num_colours) {
+  return fot::guarded("fot_colour_cells_with_ordering", [&] {
+    return fot_colour_cells_with_ordering_impl(mapping, num_cells, mapping_dim, colours_out,
+                                               num_colours);
+  });
 }
 
 namespace fot {
@@ -325,7 +343,7 @@ fot_status build_hplan(const uint32_t *const *c2n, const uint32_t *mesh_dim,
 
 } // namespace fot
 
-extern "C" fot_status fot_hplan_build(const uint32_t *const *c2n,
+static fot_status fot_hplan_build_impl(const uint32_t *const *c2n,
                                       const uint32_t *mesh_dim,
                                       uint32_t num_mappings, uint32_t num_cells,
                                       uint32_t num_points0,
@@ -345,10 +363,21 @@ extern "C" fot_status fot_hplan_build(const uint32_t *const *c2n,
   *plan = p;
   return FOT_OK;
 }
+extern "C" fot_status fot_hplan_build(const uint32_t *const *c2n,
+                                      const uint32_t *mesh_dim,
+                                      uint32_t num_mappings, uint32_t num_cells,
+                                      uint32_t num_points0,
+                                      const uint32_t *partition,
+                                      uint32_t block_size, fot_hplan **plan) {
+  return fot::guarded("fot_hplan_build", [&] {
+    return fot_hplan_build_impl(c2n, mesh_dim, num_mappings, num_cells, num_points0, partition,
+                                block_size, plan);
+  });
+}
 
 extern "C" void fot_hplan_free(fot_hplan *plan) { delete plan; }
 
-extern "C" fot_status fot_hplan_info_get(const fot_hplan *plan,
+static fot_status fot_hplan_info_get_impl(const fot_hplan *plan,
                                          fot_hplan_info *info) {
   if (!plan || !info) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
   *info = fot_hplan_info{};
@@ -369,8 +398,12 @@ extern "C" fot_status fot_hplan_info_get(const fot_hplan *plan,
   info->avg_thread_colours = info->num_blocks ? tc / info->num_blocks : 0.0;
   return FOT_OK;
 }
+extern "C" fot_status fot_hplan_info_get(const fot_hplan *plan,
+                                         fot_hplan_info *info) {
+  return fot::guarded("fot_hplan_info_get", [&] { return fot_hplan_info_get_impl(plan, info); });
+}
 
-extern "C" fot_status fot_hplan_colour_get(const fot_hplan *plan, uint32_t colour,
+static fot_status fot_hplan_colour_get_impl(const fot_hplan *plan, uint32_t colour,
                                            fot_hplan_colour *out) {
   if (!plan || !out || colour >= plan->colours.size())
     return fail(FOT_ERR_INVALID_ARGUMENT, "fot_hplan_colour_get: bad colour");
@@ -389,8 +422,14 @@ extern "C" fot_status fot_hplan_colour_get(const fot_hplan *plan, uint32_t colou
   out->cell_inv_permutation = c.cell_inv_permutation.data();
   return FOT_OK;
 }
+extern "C" fot_status fot_hplan_colour_get(const fot_hplan *plan, uint32_t colour,
+                                           fot_hplan_colour *out) {
+  return fot::guarded("fot_hplan_colour_get", [&] {
+    return fot_hplan_colour_get_impl(plan, colour, out);
+  });
+}
 
-extern "C" fot_status fot_reorder_to_partition(uint32_t *partition,
+static fot_status fot_reorder_to_partition_impl(uint32_t *partition,
                                                uint32_t num_cells,
                                                uint32_t *inv_perm) {
   if ((!partition || !inv_perm) && num_cells)
@@ -405,8 +444,15 @@ extern "C" fot_status fot_reorder_to_partition(uint32_t *partition,
   }
   return FOT_OK;
 }
+extern "C" fot_status fot_reorder_to_partition(uint32_t *partition,
+                                               uint32_t num_cells,
+                                               uint32_t *inv_perm) {
+  return fot::guarded("fot_reorder_to_partition", [&] {
+    return fot_reorder_to_partition_impl(partition, num_cells, inv_perm);
+  });
+}
 
-extern "C" fot_status fot_point_renumbering_by_partition(
+static fot_status fot_point_renumbering_by_partition_impl(
     const uint32_t *c2n, uint32_t num_cells, uint32_t mesh_dim,
     uint32_t num_points, const uint32_t *partition, uint32_t *perm) {
   if ((!c2n || !partition) && num_cells) return fail(FOT_ERR_INVALID_ARGUMENT, "null");
@@ -442,8 +488,16 @@ extern "C" fot_status fot_point_renumbering_by_partition(
   for (uint32_t i = 0; i < num_points; ++i) perm[inv[i]] = i;
   return FOT_OK;
 }
+extern "C" fot_status fot_point_renumbering_by_partition(
+    const uint32_t *c2n, uint32_t num_cells, uint32_t mesh_dim,
+    uint32_t num_points, const uint32_t *partition, uint32_t *perm) {
+  return fot::guarded("fot_point_renumbering_by_partition", [&] {
+    return fot_point_renumbering_by_partition_impl(c2n, num_cells, mesh_dim, num_points, partition,
+                                                   perm);
+  });
+}
 
-extern "C" fot_status fot_point_renumbering_first_touch(const uint32_t *c2n,
+static fot_status fot_point_renumbering_first_touch_impl(const uint32_t *c2n,
                                                         uint32_t num_cells,
                                                         uint32_t mesh_dim,
                                                         uint32_t num_points,
@@ -462,8 +516,17 @@ extern "C" fot_status fot_point_renumbering_first_touch(const uint32_t *c2n,
     if (perm[p] == num_points) perm[p] = next++;
   return FOT_OK;
 }
+extern "C" fot_status fot_point_renumbering_first_touch(const uint32_t *c2n,
+                                                        uint32_t num_cells,
+                                                        uint32_t mesh_dim,
+                                                        uint32_t num_points,
+                                                        uint32_t *perm) {
+  return fot::guarded("fot_point_renumbering_first_touch", [&] {
+    return fot_point_renumbering_first_touch_impl(c2n, num_cells, mesh_dim, num_points, perm);
+  });
+}
 
-extern "C" fot_status fot_mesh_reorder_cells(uint32_t *c2n0, uint32_t num_cells,
+static fot_status fot_mesh_reorder_cells_impl(uint32_t *c2n0, uint32_t num_cells,
                                              uint32_t mesh_dim,
                                              const uint32_t *point_perm,
                                              uint32_t *inv_perm) {
@@ -487,8 +550,16 @@ extern "C" fot_status fot_mesh_reorder_cells(uint32_t *c2n0, uint32_t num_cells,
     std::copy_n(&old[size_t(inv_perm[i]) * mesh_dim], mesh_dim, c2n0 + size_t(i) * mesh_dim);
   return FOT_OK;
 }
+extern "C" fot_status fot_mesh_reorder_cells(uint32_t *c2n0, uint32_t num_cells,
+                                             uint32_t mesh_dim,
+                                             const uint32_t *point_perm,
+                                             uint32_t *inv_perm) {
+  return fot::guarded("fot_mesh_reorder_cells", [&] {
+    return fot_mesh_reorder_cells_impl(c2n0, num_cells, mesh_dim, point_perm, inv_perm);
+  });
+}
 
-extern "C" fot_status fot_graph_csr(const uint32_t *c2n, uint32_t num_cells,
+static fot_status fot_graph_csr_impl(const uint32_t *c2n, uint32_t num_cells,
                                     uint32_t mesh_dim, uint32_t num_points,
                                     uint32_t *indices, uint32_t *endpoints,
                                     uint32_t *num_arcs) {
@@ -518,8 +589,16 @@ extern "C" fot_status fot_graph_csr(const uint32_t *c2n, uint32_t num_cells,
   *num_arcs = arcs;
   return FOT_OK;
 }
+extern "C" fot_status fot_graph_csr(const uint32_t *c2n, uint32_t num_cells,
+                                    uint32_t mesh_dim, uint32_t num_points,
+                                    uint32_t *indices, uint32_t *endpoints,
+                                    uint32_t *num_arcs) {
+  return fot::guarded("fot_graph_csr", [&] {
+    return fot_graph_csr_impl(c2n, num_cells, mesh_dim, num_points, indices, endpoints, num_arcs);
+  });
+}
 
-extern "C" fot_status fot_cell_to_cell_csr(const uint32_t *const *c2n,
+static fot_status fot_cell_to_cell_csr_impl(const uint32_t *const *c2n,
                                            const uint32_t *mesh_dim,
                                            uint32_t num_mappings,
                                            uint32_t num_cells,
@@ -557,13 +636,25 @@ extern "C" fot_status fot_cell_to_cell_csr(const uint32_t *const *c2n,
   *num_arcs = arcs;
   return FOT_OK;
 }
+extern "C" fot_status fot_cell_to_cell_csr(const uint32_t *const *c2n,
+                                           const uint32_t *mesh_dim,
+                                           uint32_t num_mappings,
+                                           uint32_t num_cells,
+                                           const uint32_t *num_points,
+                                           uint32_t *indices, uint32_t *endpoints,
+                                           uint64_t *num_arcs) {
+  return fot::guarded("fot_cell_to_cell_csr", [&] {
+    return fot_cell_to_cell_csr_impl(c2n, mesh_dim, num_mappings, num_cells, num_points, indices,
+                                     endpoints, num_arcs);
+  });
+}
 
 // countCacheLinesForBlock (problem.hpp:654-672) with the line size as a
 // parameter: distinct lines of `line_bytes` holding the points' data (AOS:
 // the dim values of each point; SOA: one line set per component, so dim x
 // the lines of the ids).  The reference counts 32-byte sectors; CDNA's L2
 // line is 128 bytes.
-extern "C" fot_status fot_count_cache_lines(const uint32_t *points, uint32_t n, uint32_t dim,
+static fot_status fot_count_cache_lines_impl(const uint32_t *points, uint32_t n, uint32_t dim,
                                             uint32_t type_size, uint32_t line_bytes, int soa,
                                             uint64_t *lines) {
   if ((!points && n) || !lines || dim == 0 || type_size == 0 || line_bytes < type_size)
@@ -580,4 +671,11 @@ extern "C" fot_status fot_count_cache_lines(const uint32_t *points, uint32_t n, 
   const uint64_t distinct = uint64_t(std::unique(ids.begin(), ids.end()) - ids.begin());
   *lines = (soa ? dim : 1) * distinct;
   return FOT_OK;
+}
+extern "C" fot_status fot_count_cache_lines(const uint32_t *points, uint32_t n, uint32_t dim,
+                                            uint32_t type_size, uint32_t line_bytes, int soa,
+                                            uint64_t *lines) {
+  return fot::guarded("fot_count_cache_lines", [&] {
+    return fot_count_cache_lines_impl(points, n, dim, type_size, line_bytes, soa, lines);
+  });
 }

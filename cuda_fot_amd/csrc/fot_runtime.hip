@@ -48,7 +48,7 @@ extern "C" const char *fot_status_string(fot_status s) {
 extern "C" const char *fot_last_error_message(void) { return g_message.c_str(); }
 extern "C" int fot_version(void) { return 1; }
 
-extern "C" fot_status fot_device_count(int *count) {
+static fot_status fot_device_count_impl(int *count) {
   if (!count) return fail(FOT_ERR_INVALID_ARGUMENT, "null count");
   *count = 0;
   const hipError_t e = hipGetDeviceCount(count);
@@ -58,40 +58,58 @@ extern "C" fot_status fot_device_count(int *count) {
   }
   return FOT_OK;
 }
+extern "C" fot_status fot_device_count(int *count) {
+  return fot::guarded("fot_device_count", [&] { return fot_device_count_impl(count); });
+}
 
-extern "C" fot_status fot_set_device(int device) {
+static fot_status fot_set_device_impl(int device) {
   FOT_HIP(hipSetDevice(device));
   return FOT_OK;
 }
+extern "C" fot_status fot_set_device(int device) {
+  return fot::guarded("fot_set_device", [&] { return fot_set_device_impl(device); });
+}
 
-extern "C" fot_status fot_malloc(void **p, size_t bytes) {
+static fot_status fot_malloc_impl(void **p, size_t bytes) {
   if (!p) return fail(FOT_ERR_INVALID_ARGUMENT, "null pointer");
   *p = nullptr;
   if (bytes == 0) return FOT_OK;
   FOT_HIP(hipMalloc(p, bytes));
   return FOT_OK;
 }
+extern "C" fot_status fot_malloc(void **p, size_t bytes) {
+  return fot::guarded("fot_malloc", [&] { return fot_malloc_impl(p, bytes); });
+}
 
-extern "C" fot_status fot_free(void *p) {
+static fot_status fot_free_impl(void *p) {
   if (p) FOT_HIP(hipFree(p));
   return FOT_OK;
 }
+extern "C" fot_status fot_free(void *p) {
+  return fot::guarded("fot_free", [&] { return fot_free_impl(p); });
+}
 
-extern "C" fot_status fot_memcpy_htod(void *dst, const void *src, size_t bytes) {
+static fot_status fot_memcpy_htod_impl(void *dst, const void *src, size_t bytes) {
   if (bytes == 0) return FOT_OK;
   if (!dst || !src) return fail(FOT_ERR_INVALID_ARGUMENT, "null pointer");
   FOT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
   return FOT_OK;
 }
+extern "C" fot_status fot_memcpy_htod(void *dst, const void *src, size_t bytes) {
+  return fot::guarded("fot_memcpy_htod", [&] { return fot_memcpy_htod_impl(dst, src, bytes); });
+}
 
-extern "C" fot_status fot_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+static fot_status fot_memcpy_dtoh_impl(void *dst, const void *src, size_t bytes) {
   if (bytes == 0) return FOT_OK;
   if (!dst || !src) return fail(FOT_ERR_INVALID_ARGUMENT, "null pointer");
   FOT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
   return FOT_OK;
 }
+extern "C" fot_status fot_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+  return fot::guarded("fot_memcpy_dtoh", [&] { return fot_memcpy_dtoh_impl(dst, src, bytes); });
+}
 
-extern "C" fot_status fot_memcpy_dtod(void *dst, const void *src, size_t bytes,
+static fot_status fot_memcpy_dtod_impl(void *dst, const void *src, size_t bytes,
                                       void *stream) {
   if (bytes == 0) return FOT_OK;
   if (!dst || !src) return fail(FOT_ERR_INVALID_ARGUMENT, "null pointer");
@@ -99,13 +117,22 @@ extern "C" fot_status fot_memcpy_dtod(void *dst, const void *src, size_t bytes,
                          static_cast<hipStream_t>(stream)));
   return FOT_OK;
 }
+extern "C" fot_status fot_memcpy_dtod(void *dst, const void *src, size_t bytes,
+                                      void *stream) {
+  return fot::guarded("fot_memcpy_dtod", [&] {
+    return fot_memcpy_dtod_impl(dst, src, bytes, stream);
+  });
+}
 
-extern "C" fot_status fot_device_synchronize(void) {
+static fot_status fot_device_synchronize_impl(void) {
   FOT_HIP(hipDeviceSynchronize());
   return FOT_OK;
 }
+extern "C" fot_status fot_device_synchronize(void) {
+  return fot::guarded("fot_device_synchronize", [&] { return fot_device_synchronize_impl(); });
+}
 
-extern "C" fot_status fot_stream_create(void **stream) {
+static fot_status fot_stream_create_impl(void **stream) {
   if (!stream) return fail(FOT_ERR_INVALID_ARGUMENT, "null stream");
   hipStream_t s;
   // default priority: streams of other priorities are not used anywhere
@@ -113,15 +140,26 @@ extern "C" fot_status fot_stream_create(void **stream) {
   *stream = s;
   return FOT_OK;
 }
+extern "C" fot_status fot_stream_create(void **stream) {
+  return fot::guarded("fot_stream_create", [&] { return fot_stream_create_impl(stream); });
+}
 
-extern "C" fot_status fot_stream_destroy(void *stream) {
+static fot_status fot_stream_destroy_impl(void *stream) {
   if (stream) FOT_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream)));
   return FOT_OK;
 }
+extern "C" fot_status fot_stream_destroy(void *stream) {
+  return fot::guarded("fot_stream_destroy", [&] { return fot_stream_destroy_impl(stream); });
+}
 
-extern "C" fot_status fot_stream_synchronize(void *stream) {
+static fot_status fot_stream_synchronize_impl(void *stream) {
   FOT_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
   return FOT_OK;
+}
+extern "C" fot_status fot_stream_synchronize(void *stream) {
+  return fot::guarded("fot_stream_synchronize", [&] {
+    return fot_stream_synchronize_impl(stream);
+  });
 }
 
 // copyKernel (data_t.cu:76-93): 16-byte vectors, a scalar tail.
@@ -137,7 +175,7 @@ __global__ __launch_bounds__(256) void fot_copy_kernel(const uint4 *__restrict__
   if (blockIdx.x == 0 && threadIdx.x < tail) tail_dst[threadIdx.x] = tail_src[threadIdx.x];
 }
 
-extern "C" fot_status fot_copy(void *dst, const void *src, size_t bytes, void *stream) {
+static fot_status fot_copy_impl(void *dst, const void *src, size_t bytes, void *stream) {
   if (bytes == 0) return FOT_OK;
   if (!dst || !src) return fail(FOT_ERR_INVALID_ARGUMENT, "null pointer");
   if ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 15u)
@@ -153,12 +191,15 @@ extern "C" fot_status fot_copy(void *dst, const void *src, size_t bytes, void *s
   FOT_HIP(hipGetLastError());
   return FOT_OK;
 }
+extern "C" fot_status fot_copy(void *dst, const void *src, size_t bytes, void *stream) {
+  return fot::guarded("fot_copy", [&] { return fot_copy_impl(dst, src, bytes, stream); });
+}
 
 struct fot_timer {
   hipEvent_t start = nullptr, stop = nullptr;
 };
 
-extern "C" fot_status fot_timer_create(fot_timer **t) {
+static fot_status fot_timer_create_impl(fot_timer **t) {
   if (!t) return fail(FOT_ERR_INVALID_ARGUMENT, "null timer");
   fot_timer *p = new fot_timer;
   hipError_t e = hipEventCreate(&p->start);
@@ -171,32 +212,47 @@ extern "C" fot_status fot_timer_create(fot_timer **t) {
   *t = p;
   return FOT_OK;
 }
+extern "C" fot_status fot_timer_create(fot_timer **t) {
+  return fot::guarded("fot_timer_create", [&] { return fot_timer_create_impl(t); });
+}
 
-extern "C" fot_status fot_timer_destroy(fot_timer *t) {
+static fot_status fot_timer_destroy_impl(fot_timer *t) {
   if (!t) return FOT_OK;
   (void)hipEventDestroy(t->start);
   (void)hipEventDestroy(t->stop);
   delete t;
   return FOT_OK;
 }
+extern "C" fot_status fot_timer_destroy(fot_timer *t) {
+  return fot::guarded("fot_timer_destroy", [&] { return fot_timer_destroy_impl(t); });
+}
 
-extern "C" fot_status fot_timer_start(fot_timer *t, void *stream) {
+static fot_status fot_timer_start_impl(fot_timer *t, void *stream) {
   if (!t) return fail(FOT_ERR_INVALID_ARGUMENT, "null timer");
   FOT_HIP(hipEventRecord(t->start, static_cast<hipStream_t>(stream)));
   return FOT_OK;
 }
+extern "C" fot_status fot_timer_start(fot_timer *t, void *stream) {
+  return fot::guarded("fot_timer_start", [&] { return fot_timer_start_impl(t, stream); });
+}
 
-extern "C" fot_status fot_timer_stop(fot_timer *t, void *stream) {
+static fot_status fot_timer_stop_impl(fot_timer *t, void *stream) {
   if (!t) return fail(FOT_ERR_INVALID_ARGUMENT, "null timer");
   FOT_HIP(hipEventRecord(t->stop, static_cast<hipStream_t>(stream)));
   return FOT_OK;
 }
+extern "C" fot_status fot_timer_stop(fot_timer *t, void *stream) {
+  return fot::guarded("fot_timer_stop", [&] { return fot_timer_stop_impl(t, stream); });
+}
 
-extern "C" fot_status fot_timer_elapsed_ms(fot_timer *t, float *ms) {
+static fot_status fot_timer_elapsed_ms_impl(fot_timer *t, float *ms) {
   if (!t || !ms) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
   FOT_HIP(hipEventSynchronize(t->stop));
   FOT_HIP(hipEventElapsedTime(ms, t->start, t->stop));
   return FOT_OK;
+}
+extern "C" fot_status fot_timer_elapsed_ms(fot_timer *t, float *ms) {
+  return fot::guarded("fot_timer_elapsed_ms", [&] { return fot_timer_elapsed_ms_impl(t, ms); });
 }
 
 // ---------------------------------------------------------------------
@@ -207,7 +263,7 @@ struct fot_ctx {
   std::vector<void *> bufs;
 };
 
-extern "C" fot_status fot_ctx_create(fot_ctx **ctx, int device) {
+static fot_status fot_ctx_create_impl(fot_ctx **ctx, int device) {
   if (!ctx) return fail(FOT_ERR_INVALID_ARGUMENT, "null context");
   *ctx = nullptr;
   int n = 0;
@@ -227,8 +283,11 @@ extern "C" fot_status fot_ctx_create(fot_ctx **ctx, int device) {
   *ctx = c;
   return FOT_OK;
 }
+extern "C" fot_status fot_ctx_create(fot_ctx **ctx, int device) {
+  return fot::guarded("fot_ctx_create", [&] { return fot_ctx_create_impl(ctx, device); });
+}
 
-extern "C" fot_status fot_ctx_destroy(fot_ctx *ctx) {
+static fot_status fot_ctx_destroy_impl(fot_ctx *ctx) {
   if (!ctx) return FOT_OK;
   (void)hipSetDevice(ctx->device);
   const hipError_t e = hipStreamSynchronize(ctx->stream);
@@ -238,20 +297,29 @@ extern "C" fot_status fot_ctx_destroy(fot_ctx *ctx) {
   if (e != hipSuccess) return fail(FOT_ERR_HIP, "context stream: %s", hipGetErrorString(e));
   return FOT_OK;
 }
+extern "C" fot_status fot_ctx_destroy(fot_ctx *ctx) {
+  return fot::guarded("fot_ctx_destroy", [&] { return fot_ctx_destroy_impl(ctx); });
+}
 
-extern "C" fot_status fot_ctx_stream(const fot_ctx *ctx, void **stream) {
+static fot_status fot_ctx_stream_impl(const fot_ctx *ctx, void **stream) {
   if (!ctx || !stream) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
   *stream = ctx->stream;
   return FOT_OK;
 }
+extern "C" fot_status fot_ctx_stream(const fot_ctx *ctx, void **stream) {
+  return fot::guarded("fot_ctx_stream", [&] { return fot_ctx_stream_impl(ctx, stream); });
+}
 
-extern "C" fot_status fot_ctx_synchronize(fot_ctx *ctx) {
+static fot_status fot_ctx_synchronize_impl(fot_ctx *ctx) {
   if (!ctx) return fail(FOT_ERR_INVALID_ARGUMENT, "null context");
   FOT_HIP(hipStreamSynchronize(ctx->stream));
   return FOT_OK;
 }
+extern "C" fot_status fot_ctx_synchronize(fot_ctx *ctx) {
+  return fot::guarded("fot_ctx_synchronize", [&] { return fot_ctx_synchronize_impl(ctx); });
+}
 
-extern "C" fot_status fot_buf_alloc(fot_ctx *ctx, size_t bytes, void **device_ptr) {
+static fot_status fot_buf_alloc_impl(fot_ctx *ctx, size_t bytes, void **device_ptr) {
   if (!ctx || !device_ptr) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
   *device_ptr = nullptr;
   FOT_HIP(hipSetDevice(ctx->device));
@@ -265,8 +333,11 @@ extern "C" fot_status fot_buf_alloc(fot_ctx *ctx, size_t bytes, void **device_pt
   *device_ptr = p;
   return FOT_OK;
 }
+extern "C" fot_status fot_buf_alloc(fot_ctx *ctx, size_t bytes, void **device_ptr) {
+  return fot::guarded("fot_buf_alloc", [&] { return fot_buf_alloc_impl(ctx, bytes, device_ptr); });
+}
 
-extern "C" fot_status fot_buf_upload(fot_ctx *ctx, void *device_dst, const void *host_src,
+static fot_status fot_buf_upload_impl(fot_ctx *ctx, void *device_dst, const void *host_src,
                                      size_t bytes) {
   if (!ctx || (bytes && (!device_dst || !host_src)))
     return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
@@ -275,8 +346,14 @@ extern "C" fot_status fot_buf_upload(fot_ctx *ctx, void *device_dst, const void 
   FOT_HIP(hipStreamSynchronize(ctx->stream));
   return FOT_OK;
 }
+extern "C" fot_status fot_buf_upload(fot_ctx *ctx, void *device_dst, const void *host_src,
+                                     size_t bytes) {
+  return fot::guarded("fot_buf_upload", [&] {
+    return fot_buf_upload_impl(ctx, device_dst, host_src, bytes);
+  });
+}
 
-extern "C" fot_status fot_buf_download(fot_ctx *ctx, void *host_dst, const void *device_src,
+static fot_status fot_buf_download_impl(fot_ctx *ctx, void *host_dst, const void *device_src,
                                        size_t bytes) {
   if (!ctx || (bytes && (!host_dst || !device_src)))
     return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
@@ -285,8 +362,14 @@ extern "C" fot_status fot_buf_download(fot_ctx *ctx, void *host_dst, const void 
   FOT_HIP(hipStreamSynchronize(ctx->stream));
   return FOT_OK;
 }
+extern "C" fot_status fot_buf_download(fot_ctx *ctx, void *host_dst, const void *device_src,
+                                       size_t bytes) {
+  return fot::guarded("fot_buf_download", [&] {
+    return fot_buf_download_impl(ctx, host_dst, device_src, bytes);
+  });
+}
 
-extern "C" fot_status fot_buf_free(fot_ctx *ctx, void *device_ptr) {
+static fot_status fot_buf_free_impl(fot_ctx *ctx, void *device_ptr) {
   if (!ctx) return fail(FOT_ERR_INVALID_ARGUMENT, "null context");
   if (!device_ptr) return FOT_OK;
   auto it = std::find(ctx->bufs.begin(), ctx->bufs.end(), device_ptr);
@@ -297,4 +380,7 @@ extern "C" fot_status fot_buf_free(fot_ctx *ctx, void *device_ptr) {
   FOT_HIP(hipStreamSynchronize(ctx->stream));
   FOT_HIP(hipFree(device_ptr));
   return FOT_OK;
+}
+extern "C" fot_status fot_buf_free(fot_ctx *ctx, void *device_ptr) {
+  return fot::guarded("fot_buf_free", [&] { return fot_buf_free_impl(ctx, device_ptr); });
 }

@@ -152,7 +152,7 @@ fot_status launched() {
 
 } // namespace
 
-extern "C" fot_status fot_functor_init(fot_functor *f, fot_kernel kernel, uint32_t mesh_dim,
+static fot_status fot_functor_init_impl(fot_functor *f, fot_kernel kernel, uint32_t mesh_dim,
                                        uint32_t point_dim, uint32_t cell_dim) {
   if (!f) return fail(FOT_ERR_INVALID_ARGUMENT, "null functor");
   f->kernel = kernel;
@@ -184,6 +184,12 @@ extern "C" fot_status fot_functor_init(fot_functor *f, fot_kernel kernel, uint32
   }
   return fail(FOT_ERR_INVALID_ARGUMENT, "unknown kernel %d", int(kernel));
 }
+extern "C" fot_status fot_functor_init(fot_functor *f, fot_kernel kernel, uint32_t mesh_dim,
+                                       uint32_t point_dim, uint32_t cell_dim) {
+  return fot::guarded("fot_functor_init", [&] {
+    return fot_functor_init_impl(f, kernel, mesh_dim, point_dim, cell_dim);
+  });
+}
 
 static fot_status step_direct(bool atomic, const fot_functor *f, int soa,
                               const void *const *pd, void *out, void *const *cd,
@@ -205,7 +211,7 @@ static fot_status step_direct(bool atomic, const fot_functor *f, int soa,
   });
 }
 
-extern "C" fot_status fot_step_gpu_global(const fot_functor *f, int soa,
+static fot_status fot_step_gpu_global_impl(const fot_functor *f, int soa,
                                           const void *const *pd, void *out,
                                           void *const *cd, const uint32_t *const *c2n,
                                           uint32_t num_cells, const uint32_t *ps,
@@ -213,8 +219,18 @@ extern "C" fot_status fot_step_gpu_global(const fot_functor *f, int soa,
                                           void *stream) {
   return step_direct(false, f, soa, pd, out, cd, c2n, num_cells, ps, cs, nb, bs, stream);
 }
+extern "C" fot_status fot_step_gpu_global(const fot_functor *f, int soa,
+                                          const void *const *pd, void *out,
+                                          void *const *cd, const uint32_t *const *c2n,
+                                          uint32_t num_cells, const uint32_t *ps,
+                                          uint32_t cs, uint32_t nb, uint32_t bs,
+                                          void *stream) {
+  return fot::guarded("fot_step_gpu_global", [&] {
+    return fot_step_gpu_global_impl(f, soa, pd, out, cd, c2n, num_cells, ps, cs, nb, bs, stream);
+  });
+}
 
-extern "C" fot_status fot_step_gpu_atomic(const fot_functor *f, int soa,
+static fot_status fot_step_gpu_atomic_impl(const fot_functor *f, int soa,
                                           const void *const *pd, void *out,
                                           void *const *cd, const uint32_t *const *c2n,
                                           uint32_t num_cells, const uint32_t *ps,
@@ -222,8 +238,18 @@ extern "C" fot_status fot_step_gpu_atomic(const fot_functor *f, int soa,
                                           void *stream) {
   return step_direct(true, f, soa, pd, out, cd, c2n, num_cells, ps, cs, nb, bs, stream);
 }
+extern "C" fot_status fot_step_gpu_atomic(const fot_functor *f, int soa,
+                                          const void *const *pd, void *out,
+                                          void *const *cd, const uint32_t *const *c2n,
+                                          uint32_t num_cells, const uint32_t *ps,
+                                          uint32_t cs, uint32_t nb, uint32_t bs,
+                                          void *stream) {
+  return fot::guarded("fot_step_gpu_atomic", [&] {
+    return fot_step_gpu_atomic_impl(f, soa, pd, out, cd, c2n, num_cells, ps, cs, nb, bs, stream);
+  });
+}
 
-extern "C" fot_status fot_step_gpu_hierarchical(
+static fot_status fot_step_gpu_hierarchical_impl(
     const fot_functor *f, int soa, const void *const *pd, void *out,
     const uint32_t *ptbc, const uint32_t *ptbc_off, void *const *cd,
     const uint32_t *const *c2n, const uint8_t *ncc, const uint8_t *cc,
@@ -243,8 +269,19 @@ extern "C" fot_status fot_step_gpu_hierarchical(
     return launched();
   });
 }
+extern "C" fot_status fot_step_gpu_hierarchical(
+    const fot_functor *f, int soa, const void *const *pd, void *out,
+    const uint32_t *ptbc, const uint32_t *ptbc_off, void *const *cd,
+    const uint32_t *const *c2n, const uint8_t *ncc, const uint8_t *cc,
+    const uint32_t *bo, uint32_t num_cells, const uint32_t *ps, uint32_t cs,
+    uint32_t nb, uint32_t bs, uint32_t cache_size, void *stream) {
+  return fot::guarded("fot_step_gpu_hierarchical", [&] {
+    return fot_step_gpu_hierarchical_impl(f, soa, pd, out, ptbc, ptbc_off, cd, c2n, ncc, cc, bo,
+                                          num_cells, ps, cs, nb, bs, cache_size, stream);
+  });
+}
 
-extern "C" fot_status fot_lds_overflow_flag(uint32_t *flag, int clear) {
+static fot_status fot_lds_overflow_flag_impl(uint32_t *flag, int clear) {
   if (!flag) return fail(FOT_ERR_INVALID_ARGUMENT, "null flag");
   hipError_t e = hipMemcpyFromSymbol(flag, HIP_SYMBOL(g_lds_overflow), sizeof(uint32_t));
   if (e == hipSuccess && clear) {
@@ -254,10 +291,15 @@ extern "C" fot_status fot_lds_overflow_flag(uint32_t *flag, int clear) {
   if (e != hipSuccess) return fail(FOT_ERR_HIP, "%s", hipGetErrorString(e));
   return FOT_OK;
 }
+extern "C" fot_status fot_lds_overflow_flag(uint32_t *flag, int clear) {
+  return fot::guarded("fot_lds_overflow_flag", [&] {
+    return fot_lds_overflow_flag_impl(flag, clear);
+  });
+}
 
 // The lulesh element function on the host (the code the kernels run): the
 // C++ drop-in's lulesh::StepSeq calls it per element.
-extern "C" fot_status fot_lulesh_element(const double *x, const double *y, const double *z,
+static fot_status fot_lulesh_element_impl(const double *x, const double *y, const double *z,
                                          const double *sig, uint32_t sig_stride, double *f,
                                          double *determ) {
   if (!x || !y || !z || !sig || !f || !determ)
@@ -265,13 +307,27 @@ extern "C" fot_status fot_lulesh_element(const double *x, const double *y, const
   fot::lu::element(x, y, z, sig[0], sig[sig_stride], sig[2 * size_t(sig_stride)], f, determ);
   return FOT_OK;
 }
+extern "C" fot_status fot_lulesh_element(const double *x, const double *y, const double *z,
+                                         const double *sig, uint32_t sig_stride, double *f,
+                                         double *determ) {
+  return fot::guarded("fot_lulesh_element", [&] {
+    return fot_lulesh_element_impl(x, y, z, sig, sig_stride, f, determ);
+  });
+}
 
 // The mini_aero user function on the host (the code the kernels run): the
 // C++ drop-in's mini_aero::StepSeq calls it per face.
-extern "C" fot_status fot_mini_aero_face_flux(const double *left, const double *right,
+static fot_status fot_mini_aero_face_flux_impl(const double *left, const double *right,
                                               uint32_t cell_stride, const double *face,
                                               uint32_t face_stride, double *flux) {
   if (!left || !right || !face || !flux) return fail(FOT_ERR_INVALID_ARGUMENT, "null argument");
   fot::ma::face_flux(left, right, cell_stride, face, face_stride, flux);
   return FOT_OK;
+}
+extern "C" fot_status fot_mini_aero_face_flux(const double *left, const double *right,
+                                              uint32_t cell_stride, const double *face,
+                                              uint32_t face_stride, double *flux) {
+  return fot::guarded("fot_mini_aero_face_flux", [&] {
+    return fot_mini_aero_face_flux_impl(left, right, cell_stride, face, face_stride, flux);
+  });
 }

@@ -18,6 +18,17 @@ def test_exports_every_declared_symbol():
     assert not set(names) - set(capi._PROTOS)
 
 
+def test_exceptions_become_statuses():
+    # every entry point runs through the library's exception guard: a C++
+    # exception (out of host memory or any other) returns a status and a
+    # message instead of crossing the C ABI (tests/cpp/guard_test.cpp)
+    import os
+    import subprocess
+    binary = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "bin", "guard_test")
+    r = subprocess.run([binary], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr
+
+
 def test_status_strings():
     for s in range(9):
         assert capi.lib.fot_status_string(s)
