@@ -2,7 +2,7 @@
 like the reference's Problem<SOA> by tests/cpp/dropin_test.cpp; its outputs
 are checked here against the oracle on the same seeded inputs.  The CPU test
 also covers the mesh / point / cell / partition stream formats (self-checks
-inside the driver)."""
+inside the test program)."""
 import os
 import subprocess
 
@@ -29,7 +29,7 @@ def _load(out, name, dim):
 
 
 def _airfoil_oracle():
-    # the driver's airfoil: 40 x 24 cells, seed 3, shuffled, res0 U[-1,1) seed 11
+    # the test program's airfoil: 40 x 24 cells, seed 3, shuffled, res0 U[-1,1) seed 11
     m = meshgen.airfoil(40, 24, seed=3, shuffle=7)
     res0 = meshgen.uniform(m["num_cells"] * 4, 11, -1.0, 1.0).reshape(-1, 4)
     maps = [m["pecell"], m["pedge"], m["pecell"], m["pecell"]]
@@ -65,7 +65,7 @@ def _mini_aero_oracle():
 
 
 def _lulesh_oracle():
-    # the driver's lulesh: 5 x 4 x 3 node lattice, jitter seed 21, forces
+    # the test program's lulesh: 5 x 4 x 3 node lattice, jitter seed 21, forces
     # seed 22, sigma seed 23, volumes zero
     n1, n2, n3 = 5, 4, 3
     c = meshgen.hex_grid(n1, n2, n3)
