@@ -29,8 +29,11 @@ template <bool SOA = false> struct Problem {
   std::vector<MY_SIZE> partition_vector;
   std::vector<MY_SIZE> applied_permutation;
   // loopGPUHierarchical's plan: FOT_LOOP_* flags (default: owner compute, two
-  // cells per thread); 0 gives the reference's per-block-colour launches
-  std::uint32_t hierarchical_flags = FOT_LOOP_OWNER_COMPUTE | FOT_LOOP_TWO_PER_THREAD;
+  // cells per thread, and for res_calc the gather accumulation); 0 gives the
+  // reference's per-block-colour launches, the faster choice for 3D element
+  // loops on a reorder()ed mesh
+  std::uint32_t hierarchical_flags =
+      FOT_LOOP_OWNER_COMPUTE | FOT_LOOP_TWO_PER_THREAD | FOT_LOOP_OWNER_GATHER;
 
   // problem.hpp:59-79
   Problem(const std::vector<std::istream *> &mesh_is, const std::vector<MY_SIZE> &mesh_dim,
