@@ -258,43 +258,54 @@ __global__ __launch_bounds__(1024) void res_calc_hier_staged_kernel(EngineArgs a
   extern __shared__ double lds[];
   uint32_t gen;
   const uint32_t b = pick_block<DEP>(dd, blk_begin, nbc, gen);
-  const uint32_t tid = threadIdx.x;
+  const uint32_t tid = threadIdx.x, bd = blockDim.x;
   const uint32_t e0 = h.blk_cell_off[b], ne = h.blk_cell_off[b + 1] - e0;
   const uint32_t c0 = h.blk_cache_off[b], m = h.blk_cache_off[b + 1] - c0;
   double *res_s = lds, *q_s = lds + 4 * m, *adt_s = lds + 8 * m;
   const double *x = a.point_data[1], *q = a.point_data[2], *adt = a.point_data[3];
-  for (uint32_t i = tid; i < m; i += blockDim.x) {
-    const uint32_t g = h.cached[c0 + i];
-    if (SOA) {
-      const uint32_t nq = a.num_points[2];
+  const uint32_t np = a.num_points[0], nq = a.num_points[2], nx = a.num_points[1];
+  // Two memory round trips before the barrier, not four: first the plan
+  // entries (this thread's edge and its first staged cell), then every value
+  // they name (x of the edge, q and adt of the cell).  Indices are clamped
+  // into the block so the loads are unconditional; a block with more cached
+  // cells than threads stages the rest in further rounds.  (Reading the res
+  // to write back this early as well measured slower: 42.6 vs 39.3 us per
+  // step on the GPS-ordered 1200x600 airfoil with 1024-edge blocks.)
+  const uint32_t pos = e0 + min(tid, ne - 1);
+  const uint32_t i0 = min(tid, m - 1);
+  const uint32_t colour_e = h.colour[pos];
+  const uint32_t lv = reinterpret_cast<const uint32_t *>(h.loc)[pos];
+  const uint2 nn = reinterpret_cast<const uint2 *>(a.maps[1])[pos];
+  const uint32_t g0 = h.cached[c0 + i0];
+  double x1[2], x2[2], q0[4], adt0;
+  if (SOA) {
+    x1[0] = x[nn.x], x1[1] = x[nx + nn.x], x2[0] = x[nn.y], x2[1] = x[nx + nn.y];
 #pragma unroll
-      for (uint32_t d = 0; d < 4; ++d) q_s[d * m + i] = q[size_t(d) * nq + g];
-    } else {
-      const double2 *q2p = reinterpret_cast<const double2 *>(q);
-      const double2 v0 = q2p[2 * size_t(g)], v1 = q2p[2 * size_t(g) + 1];
-      q_s[0 * m + i] = v0.x, q_s[1 * m + i] = v0.y, q_s[2 * m + i] = v1.x, q_s[3 * m + i] = v1.y;
-    }
+    for (uint32_t d = 0; d < 4; ++d) q0[d] = q[size_t(d) * nq + g0];
+  } else {
+    const double2 xa = reinterpret_cast<const double2 *>(x)[nn.x];
+    const double2 xb = reinterpret_cast<const double2 *>(x)[nn.y];
+    x1[0] = xa.x, x1[1] = xa.y, x2[0] = xb.x, x2[1] = xb.y;
+    const double2 *q2p = reinterpret_cast<const double2 *>(q);
+    const double2 v0 = q2p[2 * size_t(g0)], v1 = q2p[2 * size_t(g0) + 1];
+    q0[0] = v0.x, q0[1] = v0.y, q0[2] = v1.x, q0[3] = v1.y;
+  }
+  adt0 = adt[g0];
+  if (tid < m) {
+#pragma unroll
+    for (uint32_t d = 0; d < 4; ++d) q_s[d * m + tid] = q0[d], res_s[d * m + tid] = 0.0;
+    adt_s[tid] = adt0;
+  }
+  for (uint32_t i = tid + bd; i < m; i += bd) {
+    const uint32_t g = h.cached[c0 + i];
+#pragma unroll
+    for (uint32_t d = 0; d < 4; ++d) q_s[d * m + i] = q[pidx<SOA>(nq, g, 4, d)];
     adt_s[i] = adt[g];
 #pragma unroll
     for (uint32_t d = 0; d < 4; ++d) res_s[d * m + i] = 0.0;
   }
-  uint32_t colour = 0xffu, l0 = 0, l1 = 0;
-  double x1[2] = {0, 0}, x2[2] = {0, 0};
-  if (tid < ne) {
-    const uint32_t pos = e0 + tid;
-    colour = h.colour[pos];
-    const uint32_t lv = reinterpret_cast<const uint32_t *>(h.loc)[pos];
-    l0 = lv & 0xffffu, l1 = lv >> 16;
-    const uint2 nn = reinterpret_cast<const uint2 *>(a.maps[1])[pos];
-    if (SOA) {
-      const uint32_t nx = a.num_points[1];
-      x1[0] = x[nn.x], x1[1] = x[nx + nn.x], x2[0] = x[nn.y], x2[1] = x[nx + nn.y];
-    } else {
-      const double2 xa = reinterpret_cast<const double2 *>(x)[nn.x];
-      const double2 xb = reinterpret_cast<const double2 *>(x)[nn.y];
-      x1[0] = xa.x, x1[1] = xa.y, x2[0] = xb.x, x2[1] = xb.y;
-    }
-  }
+  const uint32_t colour = tid < ne ? colour_e : 0xffu;
+  const uint32_t l0 = lv & 0xffffu, l1 = lv >> 16;
   __syncthreads();
   double inc[8];
   if (tid < ne) {
@@ -316,24 +327,26 @@ __global__ __launch_bounds__(1024) void res_calc_hier_staged_kernel(EngineArgs a
     __syncthreads();
   }
   wait_predecessors<DEP>(dd, b, gen);
-  const uint32_t np = a.num_points[0];
-  for (uint32_t i = tid; i < m; i += blockDim.x) {
-    const uint32_t g = h.cached[c0 + i];
+  for (uint32_t i = tid; i < m; i += bd) {
+    const uint32_t g = i == tid ? g0 : h.cached[c0 + i];
+    double r[4];
     if (SOA) {
 #pragma unroll
-      for (uint32_t d = 0; d < 4; ++d) {
-        double *p = a.out + size_t(d) * np + g;
-        *p = *p + res_s[d * m + i];
-      }
+      for (uint32_t d = 0; d < 4; ++d) r[d] = a.out[size_t(d) * np + g];
+    } else {
+      const double2 *o = reinterpret_cast<const double2 *>(a.out) + 2 * size_t(g);
+      const double2 u0 = o[0], u1 = o[1];
+      r[0] = u0.x, r[1] = u0.y, r[2] = u1.x, r[3] = u1.y;
+    }
+#pragma unroll
+    for (uint32_t d = 0; d < 4; ++d) r[d] = r[d] + res_s[d * m + i];
+    if (SOA) {
+#pragma unroll
+      for (uint32_t d = 0; d < 4; ++d) a.out[size_t(d) * np + g] = r[d];
     } else {
       double2 *o = reinterpret_cast<double2 *>(a.out) + 2 * size_t(g);
-      double2 v0 = o[0], v1 = o[1];
-      v0.x = v0.x + res_s[0 * m + i];
-      v0.y = v0.y + res_s[1 * m + i];
-      v1.x = v1.x + res_s[2 * m + i];
-      v1.y = v1.y + res_s[3 * m + i];
-      o[0] = v0;
-      o[1] = v1;
+      o[0] = make_double2(r[0], r[1]);
+      o[1] = make_double2(r[2], r[3]);
     }
   }
   publish<DEP>(dd, b, gen);
