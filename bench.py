@@ -76,6 +76,9 @@ def parse():
     ap.add_argument("--persistent", action="store_true",
                     help="owner compute, gather: persistent workgroups that load the next "
                          "block's plan entries while computing the current one")
+    ap.add_argument("--hbm-lists", action="store_true",
+                    help="owner compute, gather: load the owned cells' edge lists from the "
+                         "plan in HBM instead of building them in LDS")
     ap.add_argument("--point-terms", action="store_true",
                     help="owner compute: 1/rho and pressure once per staged cell")
     ap.add_argument("--nodes", default="gather", choices=["gather", "stage"],
@@ -119,7 +122,7 @@ def build_problem(args, partition_kind):
 
 
 def make_loop(p, strategy, block_size, stage, graph, order="launch", ept=1, gather=False,
-              point_terms=False, owner_gather=False, persistent=False):
+              point_terms=False, owner_gather=False, persistent=False, hbm_lists=False):
     from cuda_fot_amd import DeviceArray, Loop, capi
     strat = {"atomic": capi.STRATEGY_ATOMIC, "global": capi.STRATEGY_GLOBAL_COLOUR,
              "hierarchical": capi.STRATEGY_HIERARCHICAL}[strategy]
@@ -137,6 +140,8 @@ def make_loop(p, strategy, block_size, stage, graph, order="launch", ept=1, gath
         flags |= capi.LOOP_OWNER_GATHER
     if persistent:
         flags |= capi.LOOP_PERSISTENT
+    if hbm_lists:
+        flags |= capi.LOOP_LISTS_IN_HBM
     part = p.partition_vector if strat == capi.STRATEGY_HIERARCHICAL else None
     if strat == capi.STRATEGY_HIERARCHICAL and order == "owner" and p.point_partition is not None:
         part = p.point_partition
@@ -217,6 +222,7 @@ def load_traffic(args):
             args.point_terms == c.get("point_terms", False) and
             args.accumulate == c.get("accumulate", "colours") and
             args.persistent == c.get("persistent", False) and
+            ("hbm" if args.hbm_lists else "lds") == c.get("lists", "hbm") and
             args.soa == c.get("soa", False))
     return t["traffic_bytes"] if same else None
 
@@ -590,7 +596,7 @@ def main():
     mesh, p = build_problem(args, args.partition)
     loop, dev, out = make_loop(p, args.strategy, args.block_size, not args.no_stage,
                                args.graph, args.order, args.ept, args.nodes == "gather", args.point_terms,
-                               args.accumulate == "gather", args.persistent)
+                               args.accumulate == "gather", args.persistent, args.hbm_lists)
     plan_s = time.perf_counter() - t_plan
     stats = loop.stats()
     wall, ev = time_loop(loop, args.steps, args.warmup)
@@ -635,6 +641,7 @@ def main():
                    "threads_per_block": stats["threads_per_block"],
                    "stage_per_thread": stats["stage_per_thread"],
                    "persistent": args.persistent, "launch_grid": stats["launch_grid"],
+                   "lists": "hbm" if args.hbm_lists else "lds",
                    "plan_seconds": round(plan_s, 2), "tune": tune},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,

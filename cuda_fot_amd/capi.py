@@ -58,6 +58,7 @@ LOOP_GATHER_NODES = 32
 LOOP_POINT_TERMS = 64
 LOOP_OWNER_GATHER = 128
 LOOP_PERSISTENT = 256
+LOOP_LISTS_IN_HBM = 512
 
 
 class FotError(RuntimeError):

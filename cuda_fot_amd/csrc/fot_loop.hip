@@ -700,7 +700,7 @@ FOT_DEV void owner3_load_ids(Owner3Ids<EPT, KS, KO> &id, const OwnerMeta &mt, co
     const uint32_t i = tid + k * bd;
     const uint32_t pc = pcached[i];
     id.gq[k] = (contig && i < nw) ? mt.own_start + i : (i < m ? pc : 0u);
-    if (k < KO) {
+    if (LW > 0 && k < KO) {
       const list_t l = lists[mt.pad0 + i];
       id.lst[k][0] = l.x, id.lst[k][1] = l.y;
       if constexpr (LW == 4) id.lst[k][2] = l.z, id.lst[k][3] = l.w;
@@ -769,6 +769,16 @@ FOT_DEV void owner3_body(double *lds, Owner3Data<EPT, KS, KO> &d, const Owner3Id
   double *q_s = lds, *adt_s = q_s + 4 * m, *ri_s = adt_s + m, *p_s = ri_s + m;
   double2 *q2_s = reinterpret_cast<double2 *>(q_s);
   double2 *f2_s = reinterpret_cast<double2 *>(lds);
+  // LW == 0: the owned cells' edge lists are built here instead of loaded:
+  // slot_s[c * nw + i] is the position (| 0x8000 for the edge's second cell)
+  // of owned cell i's edge of thread colour c, 0xffff where it has none.  A
+  // cell has at most one edge per colour (the plan checks it), so reading its
+  // slots in colour order is the order of the loaded lists.
+  const uint32_t ncol = mt.ncol;
+  uint16_t *slot_s = reinterpret_cast<uint16_t *>(
+      lds + (PRE ? 7u * m > 4u * ne ? 7u * m : 4u * ne : 5u * m > 4u * ne ? 5u * m : 4u * ne));
+  if (LW == 0)
+    for (uint32_t i = tid; i < ncol * nw; i += bd) slot_s[i] = 0xffffu;
   // one wait for all the loads (see res_calc_owner2_kernel)
 #pragma unroll
   for (int k = 0; k < KS; ++k) {
@@ -812,6 +822,12 @@ FOT_DEV void owner3_body(double *lds, Owner3Data<EPT, KS, KO> &d, const Owner3Id
     if (t < ne) {
       f2_s[t] = make_double2(fl[e][0], fl[e][1]);
       f2_s[ne + t] = make_double2(fl[e][2], fl[e][3]);
+      if (LW == 0) {
+        const uint32_t l0 = id.pk[e] & 0xfffu, l1 = (id.pk[e] >> 12) & 0xfffu;
+        const uint32_t col = id.pk[e] >> 24;
+        if (l0 < nw) slot_s[col * nw + l0] = uint16_t(t);
+        if (l1 < nw) slot_s[col * nw + l1] = uint16_t(t | 0x8000u);
+      }
     }
   }
   __syncthreads();
@@ -820,16 +836,26 @@ FOT_DEV void owner3_body(double *lds, Owner3Data<EPT, KS, KO> &d, const Owner3Id
     const uint32_t i = tid + k * bd;
     if (i < nw) {
       double r[4] = {d.vr0[k].x, d.vr0[k].y, d.vr1[k].x, d.vr1[k].y};
-#pragma unroll
-      for (int j = 0; j < 2 * LW; ++j) {
-        const uint32_t ent = (id.lst[k][j >> 1] >> (16 * (j & 1))) & 0xffffu;
-        if (ent == 0xffffu) break;
+      auto add = [&](uint32_t ent) {
         const uint32_t pos = ent & 0x7fffu;
         const double2 f01 = f2_s[pos], f23 = f2_s[ne + pos];
         if (ent & 0x8000u) { // the edge's second cell: res2 = -f, and r + (-f) == r - f
           r[0] -= f01.x, r[1] -= f01.y, r[2] -= f23.x, r[3] -= f23.y;
         } else {
           r[0] += f01.x, r[1] += f01.y, r[2] += f23.x, r[3] += f23.y;
+        }
+      };
+      if (LW == 0) {
+        for (uint32_t c = 0; c < ncol; ++c) {
+          const uint32_t ent = slot_s[c * nw + i];
+          if (ent != 0xffffu) add(ent);
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < 2 * LW; ++j) {
+          const uint32_t ent = (id.lst[k][j >> 1] >> (16 * (j & 1))) & 0xffffu;
+          if (ent == 0xffffu) break;
+          add(ent);
         }
       }
       const uint32_t g = id.gq[k];
@@ -915,6 +941,9 @@ const void *owner3_pick(bool persistent, bool pre) {
 }
 template <bool SOA, int E, int K>
 const void *owner3_pick_w(bool persistent, bool pre, uint32_t lw, bool ko1) {
+  if (lw == 0)
+    return ko1 ? owner3_pick<SOA, E, K, 0, 1>(persistent, pre)
+               : owner3_pick<SOA, E, K, 0, K>(persistent, pre);
   if (lw == 2)
     return ko1 ? owner3_pick<SOA, E, K, 2, 1>(persistent, pre)
                : owner3_pick<SOA, E, K, 2, K>(persistent, pre);
@@ -998,6 +1027,7 @@ struct fot_loop {
   bool owner_gather = false;     // owner res_calc: gather accumulation kernel
   const uint32_t *owner_plist = nullptr;
   uint32_t list_words = 4;       // owner gather: 32-bit words per owned-cell list
+  bool lists_in_lds = false;     // owner gather: the kernel builds the lists in LDS
   uint32_t stage_per_thread = 1; // owner res_calc: staged points per thread
   bool owned_first_round = false; // owner res_calc: every block owns <= threads cells
   const void *owner3_fn = nullptr; // owner gather: the kernel instantiation to launch
@@ -1353,6 +1383,7 @@ fot_status fot_loop::build_hierarchical() {
         for (const OwnerMeta &mt : meta) total_owned += mt.nowned;
         std::vector<uint16_t> plist((size_t(total_owned) + kOwnerPad) * 8, 0xffffu);
         uint32_t longest = 0;
+        bool one_per_colour = true; // every owned cell has <= 1 edge per colour
         std::vector<std::vector<std::pair<uint32_t, uint16_t>>> lists;
         uint32_t own_off = 0;
         for (size_t b = 0; b < meta.size() && owner_gather; ++b) {
@@ -1371,6 +1402,8 @@ fot_status fot_loop::build_hierarchical() {
               break;
             }
             std::sort(lists[i].begin(), lists[i].end());
+            for (size_t j = 1; j < lists[i].size(); ++j)
+              if (lists[i][j].first == lists[i][j - 1].first) one_per_colour = false;
             longest = std::max(longest, uint32_t(lists[i].size()));
             for (size_t j = 0; j < lists[i].size(); ++j)
               plist[(size_t(own_off) + i) * 8 + j] = lists[i][j].second;
@@ -1378,8 +1411,10 @@ fot_status fot_loop::build_hierarchical() {
           own_off += mt.nowned;
         }
         if (owner_gather) {
-          // 4 entries (8 bytes) per owned cell when every list fits, else 8
+          // 4 entries (8 bytes) per owned cell when every list fits, else 8;
+          // none (0) when the kernel builds them in LDS
           list_words = longest <= 4 ? 2 : 4;
+          lists_in_lds = one_per_colour && !(cfg.flags & FOT_LOOP_LISTS_IN_HBM);
           if (list_words == 2) {
             const size_t n = plist.size() / 8;
             for (size_t i = 0; i < n; ++i)
@@ -1496,7 +1531,9 @@ fot_status fot_loop::build_hierarchical() {
       for (size_t b = 0; b + 1 < blk_cache_off.size(); ++b) {
         const uint32_t m = blk_cache_off[b + 1] - blk_cache_off[b];
         const uint32_t ne = blk_cell_off[b + 1] - blk_cell_off[b];
-        lds_bytes = std::max(lds_bytes, 8u * std::max(per_cell * m, 4u * ne));
+        // + the edge-list slots (2 bytes per owned cell and thread colour)
+        const uint32_t slots = lists_in_lds ? 2u * uint32_t(blk_ncol[b]) * blk_owned[b] : 0u;
+        lds_bytes = std::max(lds_bytes, 8u * std::max(per_cell * m, 4u * ne) + slots);
       }
     }
     if (f.kernel == FOT_KERNEL_RES_CALC && point_terms && !owner_gather) {
@@ -1539,12 +1576,11 @@ fot_status fot_loop::build_hierarchical() {
   launch_grid = (dep_order || owner) ? nb_total : 0;
   if (owner && f.kernel == FOT_KERNEL_RES_CALC && owner_gather) {
     const bool persistent = (cfg.flags & FOT_LOOP_PERSISTENT) != 0;
+    const uint32_t lw = lists_in_lds ? 0u : list_words;
     owner3_fn = desc.soa ? owner3_pick_kernel<true>(persistent, point_terms, ept,
-                                                    stage_per_thread, list_words,
-                                                    owned_first_round)
+                                                    stage_per_thread, lw, owned_first_round)
                          : owner3_pick_kernel<false>(persistent, point_terms, ept,
-                                                     stage_per_thread, list_words,
-                                                     owned_first_round);
+                                                     stage_per_thread, lw, owned_first_round);
     if (persistent && nb_total) {
       // as many workgroups as are resident at once, a multiple of the 8 XCDs
       int dev = 0, cus = 0, per_cu = 0;

@@ -497,6 +497,11 @@ typedef struct {
  * block's plan entries while it computes the current one; the same bits.
  * Ignored by the other kernels. */
 #define FOT_LOOP_PERSISTENT 256u
+/* owner compute, res_calc gather: load each owned cell's edge list (8 bytes
+ * per cell) from the plan instead of building it in LDS from the edges'
+ * positions (the default when every cell has at most one edge per thread
+ * colour); the same bits */
+#define FOT_LOOP_LISTS_IN_HBM 512u
 
 typedef struct fot_loop fot_loop;
 /* Builds the plan (host), uploads it, permutes cell data. */
