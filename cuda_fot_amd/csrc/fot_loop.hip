@@ -698,7 +698,11 @@ FOT_DEV void owner3_load_ids(Owner3Ids<EPT, KS, KO> &id, const OwnerMeta &mt, co
 #pragma unroll
   for (int k = 0; k < KS; ++k) {
     const uint32_t i = tid + k * bd;
-    const uint32_t pc = pcached[i];
+    // a contiguous block's owned ids are own_start + i: their lanes, and the
+    // lanes past the block's end, read one shared word (the first halo id,
+    // the last id) instead of loading ids they do not use
+    const uint32_t j = contig ? (i < nw ? nw : (i < m ? i : m - 1)) : i;
+    const uint32_t pc = pcached[j];
     id.gq[k] = (contig && i < nw) ? mt.own_start + i : (i < m ? pc : 0u);
     if (LW > 0 && k < KO) {
       const list_t l = lists[mt.pad0 + i];
