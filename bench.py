@@ -93,7 +93,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-atomic-ref", action="store_true")
     ap.add_argument("--sweep-block-sizes", action="store_true",
-                    help="mini_aero / hex / lulesh / mine: also time plain atomics and the "
+                    help="mini_aero / hex / lulesh / getacc / mine: also time plain atomics "
+                         "and the "
                          "reference-shaped hierarchical loop (GPS-ordered mesh, one launch per "
                          "block colour) at block sizes 128-1024")
     ap.add_argument("--soa", action="store_true",
