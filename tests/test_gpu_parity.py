@@ -7,6 +7,8 @@ order (oracle_loop_colour / oracle_loop_hier_replay).  The atomics loop adds
 in arbitrary order and is held to |gpu - serial| <= 1e-12 * max(|serial|, S)
 with S the oracle's sum of absolute increments.
 """
+import copy
+
 import numpy as np
 import pytest
 
@@ -63,6 +65,7 @@ TWO = capi.LOOP_TWO_PER_THREAD
 GATHER = capi.LOOP_GATHER_NODES
 TERMS = capi.LOOP_POINT_TERMS
 OG = capi.LOOP_OWNER_GATHER
+HBM = capi.LOOP_LISTS_IN_HBM
 PERS = capi.LOOP_PERSISTENT
 
 
@@ -100,7 +103,15 @@ def _owner_replay(p, kernel, block_size, partition=None, iters=ITERS, point_part
                                                  (False, 192, OWN | TWO | OG | TERMS | PERS,
                                                   "points"),
                                                  (True, 128, OWN | GATHER, "points"),
-                                                 (False, 256, OWN | TWO, None)])
+                                                 (False, 256, OWN | TWO, None),
+                                                 # the owned cells' edge lists loaded from
+                                                 # the plan instead of built in LDS
+                                                 (False, 192, OWN | TWO | OG | HBM, "points"),
+                                                 (True, 224, OWN | TWO | OG | HBM, "points"),
+                                                 (False, 128, OWN | OG | PERS | HBM, "points"),
+                                                 (True, 192, OWN | TWO | OG | TERMS | HBM,
+                                                  "points"),
+                                                 (True, 512, OWN | OG | HBM, "kway")])
 def test_res_calc_owner_compute_bitwise(gpu, soa, bs, flags, method):
     p = airfoil_problem(nx=64, ny=32, soa=soa, res0_seed=10, block_size=bs)
     orig = airfoil_problem(nx=64, ny=32, soa=soa, res0_seed=10, block_size=bs)
@@ -119,6 +130,22 @@ def test_res_calc_owner_compute_bitwise(gpu, soa, bs, flags, method):
     p.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, ITERS, flags=flags)
     assert_bitwise(p.point_data[0], ref, "owner compute")
     assert_close_scaled(p.target_in_original_order(), serial, S, what="owner compute vs serial")
+
+
+def test_res_calc_gather_edge_on_one_cell_twice(gpu):
+    # an edge whose two cells are the same cell gives that cell two edge-list
+    # entries of one thread colour: the lists cannot be built one slot per
+    # colour, so the plan loads them (the same bits as the oracle's replay,
+    # which adds the edge's two sides in slot order)
+    p = airfoil_problem(nx=24, ny=16, res0_seed=12, block_size=128)
+    for m in (0, 2, 3):
+        p.maps[m][5, 1] = p.maps[m][5, 0]
+    p.partition_points(128)
+    ref = _owner_replay(p, O.RES_CALC, 128, None, point_partition=p.point_partition)
+    for flags in (OWN | TWO | OG, OWN | TWO | OG | HBM):
+        q = copy.deepcopy(p)
+        q.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, ITERS, flags=flags)
+        assert_bitwise(q.point_data[0], ref, f"edge on one cell twice, flags {flags}")
 
 
 def test_res_calc_owner_engine_partition(gpu):
