@@ -359,32 +359,79 @@ __global__ __launch_bounds__(1024) void res_calc_hier_staged_kernel(EngineArgs a
 // another block and are skipped) and the cache is stored back.  No block
 // shares a written point with another, so there is no order between blocks.
 
+// Whether an owner block loads its plan entries and first staged values
+// before the functor's arithmetic, overlapping their round trips.  Off where
+// that measured slower: mini_aero's face flux already fills the 128 registers
+// a 1024-thread workgroup allows and spills, and getacc lost 1 %.
+template <class F>
+constexpr bool kEarlyStaging = !std::is_same_v<F, fot::MiniAeroF> && !std::is_same_v<F, fot::GetaccF>;
+
 template <class F, bool SOA>
 __device__ __forceinline__ void owner_block(const EngineArgs &a, const HierDev &h, uint32_t b,
                                             double *lds) {
-  const uint32_t tid = threadIdx.x;
+  const uint32_t tid = threadIdx.x, bd = blockDim.x;
   const uint32_t e0 = h.blk_cell_off[b], ne = h.blk_cell_off[b + 1] - e0;
   const uint32_t c0 = h.blk_cache_off[b];
   const uint32_t nw = h.blk_owned[b];
   double *cache = lds;
   uint32_t *g_s = reinterpret_cast<uint32_t *>(lds + size_t(F::TD) * nw);
   const uint32_t np = a.num_points[0];
-  for (uint32_t i = tid; i < nw; i += blockDim.x) {
-    const uint32_t g = h.cached[c0 + i];
-    g_s[i] = g;
-#pragma unroll
-    for (uint32_t d = 0; d < F::TD; ++d) cache[d * nw + i] = a.out[pidx<SOA>(np, g, F::TD, d)];
-  }
   uint32_t loc[F::MD], tgt[F::MD];
   double inc[F::MD * F::TD];
   uint32_t colour = 0xffu;
-  if (tid < ne) {
-    const uint32_t pos = e0 + tid;
-    colour = h.colour[pos];
+  if constexpr (kEarlyStaging<F>) {
+    // Plan entries first (the first staging round's owned id, this thread's
+    // cell and its local ids), then the values they name (the owned point's
+    // current value, the cell's target ids), then the functor's own loads:
+    // three dependent round trips instead of five.  Indices are clamped into
+    // the block, so the loads are unconditional; the staged value is in LDS
+    // before the functor computes, so it holds no register across it.
+    // (a block owns at least one point; it may run no cell, when its points
+    // are touched by none: then it reads position 0 and local id 0, both in
+    // bounds, and computes nothing)
+    const uint32_t g0 = h.cached[c0 + min(tid, nw - 1)];
+    const bool cells = ne > 0;
+    const uint32_t pos = cells ? e0 + min(tid, ne - 1) : 0u;
+    const uint32_t colour_c = h.colour[pos];
     load_loc<F::MD>(h.loc + size_t(pos) * F::MD, loc);
 #pragma unroll
+    for (uint32_t k = 0; k < F::MD; ++k) loc[k] = cells ? loc[k] : 0u;
+    double v0[F::TD];
+#pragma unroll
+    for (uint32_t d = 0; d < F::TD; ++d) v0[d] = a.out[pidx<SOA>(np, g0, F::TD, d)];
+#pragma unroll
     for (uint32_t k = 0; k < F::MD; ++k) tgt[k] = h.cached[c0 + loc[k]];
-    F::template compute<SOA>(a, pos, tgt, inc);
+    if (tid < nw) {
+      g_s[tid] = g0;
+#pragma unroll
+      for (uint32_t d = 0; d < F::TD; ++d) cache[d * nw + tid] = v0[d];
+    }
+    for (uint32_t i = tid + bd; i < nw; i += bd) {
+      const uint32_t g = h.cached[c0 + i];
+      g_s[i] = g;
+#pragma unroll
+      for (uint32_t d = 0; d < F::TD; ++d) cache[d * nw + i] = a.out[pidx<SOA>(np, g, F::TD, d)];
+    }
+    if (tid < ne) {
+      colour = colour_c;
+      F::template compute<SOA>(a, pos, tgt, inc);
+    }
+  } else {
+    // stage the owned points, then load the cell's plan entries and compute
+    for (uint32_t i = tid; i < nw; i += bd) {
+      const uint32_t g = h.cached[c0 + i];
+      g_s[i] = g;
+#pragma unroll
+      for (uint32_t d = 0; d < F::TD; ++d) cache[d * nw + i] = a.out[pidx<SOA>(np, g, F::TD, d)];
+    }
+    if (tid < ne) {
+      const uint32_t pos = e0 + tid;
+      colour = h.colour[pos];
+      load_loc<F::MD>(h.loc + size_t(pos) * F::MD, loc);
+#pragma unroll
+      for (uint32_t k = 0; k < F::MD; ++k) tgt[k] = h.cached[c0 + loc[k]];
+      F::template compute<SOA>(a, pos, tgt, inc);
+    }
   }
   __syncthreads();
   const uint32_t ncol = h.blk_ncol[b];
