@@ -204,9 +204,16 @@ FOT_DEV void publish(const DepDev &dd, uint32_t b, uint32_t gen) {
   }
 }
 
-// MODE: 0 one launch per block colour, 1 dependency order
-template <class F, bool SOA, int MODE>
-__global__ __launch_bounds__(1024) void eng_hier_kernel(EngineArgs a, HierDev h, DepDev dd,
+// Functors whose hierarchical blocks of at most 256 cells run a 256-thread
+// instantiation: mini_aero's face flux spills at the 128 registers a
+// 1024-thread bound allows.
+template <class F> constexpr bool kNarrowHier = std::is_same_v<F, fot::MiniAeroF>;
+
+// MODE: 0 one launch per block colour, 1 dependency order.  LB: the most
+// threads a launch may use (a lower bound leaves the compiler more registers
+// per thread).
+template <class F, bool SOA, int MODE, int LB = 1024>
+__global__ __launch_bounds__(LB) void eng_hier_kernel(EngineArgs a, HierDev h, DepDev dd,
                                                         uint32_t blk_begin, uint32_t nbc) {
   constexpr bool DEP = MODE == 1;
   extern __shared__ double cache[];
@@ -1740,6 +1747,13 @@ fot_status fot_loop::enqueue(hipStream_t s) {
           done = true;
         }
       }
+      if constexpr (kNarrowHier<F>) {
+        if (!done && threads <= 256) {
+          hipLaunchKernelGGL((eng_hier_kernel<F, SOA, 1, 256>), dim3(nb), dim3(threads),
+                             lds_bytes, s, plan_args, hier, dep, 0u, nb);
+          done = true;
+        }
+      }
       if (!done)
         hipLaunchKernelGGL((eng_hier_kernel<F, SOA, 1>), dim3(nb), dim3(threads), lds_bytes, s,
                            plan_args, hier, dep, 0u, nb);
@@ -1751,6 +1765,13 @@ fot_status fot_loop::enqueue(hipStream_t s) {
           if (staged) {
             hipLaunchKernelGGL((res_calc_hier_staged_kernel<SOA, 0>), dim3(nbc),
                                dim3(threads), lds_bytes, s, plan_args, hier, dep, b0, nbc);
+            continue;
+          }
+        }
+        if constexpr (kNarrowHier<F>) {
+          if (threads <= 256) {
+            hipLaunchKernelGGL((eng_hier_kernel<F, SOA, 0, 256>), dim3(nbc), dim3(threads),
+                               lds_bytes, s, plan_args, hier, dep, b0, nbc);
             continue;
           }
         }
