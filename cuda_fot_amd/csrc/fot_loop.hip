@@ -63,9 +63,18 @@ FOT_DEV void load_loc(const uint16_t *__restrict__ p, uint32_t *loc) {
   }
 }
 
+// Launch bound of the instantiations that run a functor's small blocks.
+// Under a 1024-thread bound a kernel may use 128 VGPRs, where mini_aero's
+// face flux spills about 200 B per lane.  Hierarchical blocks of at most this
+// many threads run a kernel bounded at it, which may use more registers, and
+// direct launches (whose results do not depend on the block size) always do.
+// (lulesh's element forces, at the cap too, measured the same at 512.)
+template <class F>
+constexpr int kSmallBound = std::is_same_v<F, fot::MiniAeroF> ? 256 : 1024;
+
 // ---- direct launches (atomics, global colour) -------------------------
-template <class F, bool SOA, bool ATOMIC>
-__global__ __launch_bounds__(1024) void eng_direct_kernel(EngineArgs a) {
+template <class F, bool SOA, bool ATOMIC, int LB = 1024>
+__global__ __launch_bounds__(LB) void eng_direct_kernel(EngineArgs a) {
   const uint32_t ind = blockIdx.x * blockDim.x + threadIdx.x;
   if (ind >= a.num_cells) return;
   uint32_t tgt[F::MD];
@@ -203,11 +212,6 @@ FOT_DEV void publish(const DepDev &dd, uint32_t b, uint32_t gen) {
     }
   }
 }
-
-// Functors whose hierarchical blocks of at most 256 cells run a 256-thread
-// instantiation: mini_aero's face flux spills at the 128 registers a
-// 1024-thread bound allows.
-template <class F> constexpr bool kNarrowHier = std::is_same_v<F, fot::MiniAeroF>;
 
 // MODE: 0 one launch per block colour, 1 dependency order.  LB: the most
 // threads a launch may use (a lower bound leaves the compiler more registers
@@ -1682,17 +1686,21 @@ fot_status fot_loop::enqueue(hipStream_t s) {
   if (desc.num_cells == 0) return FOT_OK; // nothing to launch
   const fot_status st = dispatch(&f, desc.soa, [&]<class F, bool SOA>() -> fot_status {
     const uint32_t bs = cfg.block_size;
+    // direct launches: one thread per cell, whatever the block size
+    constexpr int LB = kSmallBound<F>;
+    const uint32_t dbs = std::min<uint32_t>(bs, LB);
     if (cfg.strategy == FOT_STRATEGY_ATOMIC) {
-      const uint32_t nb = (desc.num_cells + bs - 1) / bs;
       if constexpr (std::is_same<F, ResCalcF>::value)
-        hipLaunchKernelGGL((res_calc_atomic_kernel<SOA>), dim3(nb), dim3(bs), 0, s, base);
+        hipLaunchKernelGGL((res_calc_atomic_kernel<SOA>), dim3((desc.num_cells + bs - 1) / bs),
+                           dim3(bs), 0, s, base);
       else
-        hipLaunchKernelGGL((eng_direct_kernel<F, SOA, true>), dim3(nb), dim3(bs), 0, s, base);
+        hipLaunchKernelGGL((eng_direct_kernel<F, SOA, true, LB>),
+                           dim3((desc.num_cells + dbs - 1) / dbs), dim3(dbs), 0, s, base);
     } else if (cfg.strategy == FOT_STRATEGY_GLOBAL_COLOUR) {
       for (const EngineArgs &a : colour_args) {
         if (a.num_cells == 0) continue;
-        hipLaunchKernelGGL((eng_direct_kernel<F, SOA, false>), dim3((a.num_cells + bs - 1) / bs),
-                           dim3(bs), 0, s, a);
+        hipLaunchKernelGGL((eng_direct_kernel<F, SOA, false, LB>),
+                           dim3((a.num_cells + dbs - 1) / dbs), dim3(dbs), 0, s, a);
       }
     } else if (owner) {
       const uint32_t nb = colour_blk_off.back();
@@ -1747,9 +1755,9 @@ fot_status fot_loop::enqueue(hipStream_t s) {
           done = true;
         }
       }
-      if constexpr (kNarrowHier<F>) {
-        if (!done && threads <= 256) {
-          hipLaunchKernelGGL((eng_hier_kernel<F, SOA, 1, 256>), dim3(nb), dim3(threads),
+      if constexpr (kSmallBound<F> < 1024) {
+        if (!done && threads <= kSmallBound<F>) {
+          hipLaunchKernelGGL((eng_hier_kernel<F, SOA, 1, kSmallBound<F>>), dim3(nb), dim3(threads),
                              lds_bytes, s, plan_args, hier, dep, 0u, nb);
           done = true;
         }
@@ -1768,9 +1776,9 @@ fot_status fot_loop::enqueue(hipStream_t s) {
             continue;
           }
         }
-        if constexpr (kNarrowHier<F>) {
-          if (threads <= 256) {
-            hipLaunchKernelGGL((eng_hier_kernel<F, SOA, 0, 256>), dim3(nbc), dim3(threads),
+        if constexpr (kSmallBound<F> < 1024) {
+          if (threads <= kSmallBound<F>) {
+            hipLaunchKernelGGL((eng_hier_kernel<F, SOA, 0, kSmallBound<F>>), dim3(nbc), dim3(threads),
                                lds_bytes, s, plan_args, hier, dep, b0, nbc);
             continue;
           }
