@@ -3,6 +3,8 @@
 #ifndef FOT_DISPATCH_HPP
 #define FOT_DISPATCH_HPP
 
+#include <type_traits>
+
 #include "fot_functors.hpp"
 #include "fot_internal.hpp"
 
@@ -51,6 +53,15 @@ fot_status dispatch(const fot_functor *f, int soa, Body &&body) {
   }
 #undef FOT_SOA
 }
+
+// Launch bound of the instantiations that run a functor's small blocks.
+// Under a 1024-thread bound a kernel may use 128 VGPRs, where mini_aero's
+// face flux spills about 200 B per lane.  Hierarchical blocks of at most this
+// many threads run a kernel bounded at it, which may use more registers, and
+// direct launches (whose results do not depend on the block size) always do.
+// (lulesh's element forces, at the cap too, measured the same at 512.)
+template <class F>
+constexpr int kSmallBound = std::is_same_v<F, MiniAeroF> ? 256 : 1024;
 
 } // namespace fot
 

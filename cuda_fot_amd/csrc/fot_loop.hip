@@ -63,15 +63,6 @@ FOT_DEV void load_loc(const uint16_t *__restrict__ p, uint32_t *loc) {
   }
 }
 
-// Launch bound of the instantiations that run a functor's small blocks.
-// Under a 1024-thread bound a kernel may use 128 VGPRs, where mini_aero's
-// face flux spills about 200 B per lane.  Hierarchical blocks of at most this
-// many threads run a kernel bounded at it, which may use more registers, and
-// direct launches (whose results do not depend on the block size) always do.
-// (lulesh's element forces, at the cap too, measured the same at 512.)
-template <class F>
-constexpr int kSmallBound = std::is_same_v<F, fot::MiniAeroF> ? 256 : 1024;
-
 // ---- direct launches (atomics, global colour) -------------------------
 template <class F, bool SOA, bool ATOMIC, int LB = 1024>
 __global__ __launch_bounds__(LB) void eng_direct_kernel(EngineArgs a) {

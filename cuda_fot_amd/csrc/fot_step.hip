@@ -10,8 +10,8 @@ using namespace fot;
 namespace {
 
 // Global-colouring / atomics step: one thread per cell of the launch.
-template <class F, bool SOA, bool ATOMIC>
-__global__ __launch_bounds__(1024) void step_direct_kernel(StepArgs a) {
+template <class F, bool SOA, bool ATOMIC, int LB = 1024>
+__global__ __launch_bounds__(LB) void step_direct_kernel(StepArgs a) {
   const uint32_t ind = blockIdx.x * blockDim.x + threadIdx.x;
   if (ind >= a.num_cells) return;
   uint32_t tgt[F::MD];
@@ -96,8 +96,8 @@ __device__ uint32_t g_lds_overflow; // set when a block's cache exceeds cache_si
 // colour at a time, then the cache is added to the output.  Blocks of one
 // launch share no target point (block colouring), so the writeback is a
 // plain read-modify-write.
-template <class F, bool SOA>
-__global__ __launch_bounds__(1024) void step_hier_kernel(
+template <class F, bool SOA, int LB = 1024>
+__global__ __launch_bounds__(LB) void step_hier_kernel(
     StepArgs a, const uint32_t *__restrict__ ptbc, const uint32_t *__restrict__ ptbc_off,
     const uint8_t *__restrict__ num_cell_colours, const uint8_t *__restrict__ cell_colours,
     const uint32_t *__restrict__ block_offsets, uint32_t cache_size) {
@@ -203,10 +203,15 @@ static fot_status step_direct(bool atomic, const fot_functor *f, int soa,
   const StepArgs a{pd, static_cast<double *>(out), cd, c2n, num_cells, ps, cs};
   const hipStream_t s = static_cast<hipStream_t>(stream);
   return dispatch(f, soa, [&]<class F, bool SOA>() {
+    // one thread per cell: the cells a launch covers do not depend on its
+    // shape, so a functor with a smaller bound runs blocks of that size
+    constexpr int LB = kSmallBound<F>;
+    const uint32_t dbs = bs <= uint32_t(LB) ? bs : uint32_t(LB);
+    const uint32_t dnb = dbs == bs ? nb : (num_cells + dbs - 1) / dbs;
     if (atomic)
-      hipLaunchKernelGGL((step_direct_kernel<F, SOA, true>), dim3(nb), dim3(bs), 0, s, a);
+      hipLaunchKernelGGL((step_direct_kernel<F, SOA, true, LB>), dim3(dnb), dim3(dbs), 0, s, a);
     else
-      hipLaunchKernelGGL((step_direct_kernel<F, SOA, false>), dim3(nb), dim3(bs), 0, s, a);
+      hipLaunchKernelGGL((step_direct_kernel<F, SOA, false, LB>), dim3(dnb), dim3(dbs), 0, s, a);
     return launched();
   });
 }
@@ -264,6 +269,13 @@ static fot_status fot_step_gpu_hierarchical_impl(
   const StepArgs a{pd, static_cast<double *>(out), cd, c2n, num_cells, ps, cs};
   const hipStream_t s = static_cast<hipStream_t>(stream);
   return dispatch(f, soa, [&]<class F, bool SOA>() {
+    if constexpr (kSmallBound<F> < 1024) {
+      if (bs <= uint32_t(kSmallBound<F>)) {
+        hipLaunchKernelGGL((step_hier_kernel<F, SOA, kSmallBound<F>>), dim3(nb), dim3(bs),
+                           cache_size, s, a, ptbc, ptbc_off, ncc, cc, bo, cache_size);
+        return launched();
+      }
+    }
     hipLaunchKernelGGL((step_hier_kernel<F, SOA>), dim3(nb), dim3(bs), cache_size, s, a, ptbc,
                        ptbc_off, ncc, cc, bo, cache_size);
     return launched();

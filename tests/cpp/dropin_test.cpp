@@ -342,6 +342,16 @@ template <bool SOA> void gpu_cases(const char *tag) {
     dump(p, "ma_owner_" + t);
   }
   {
+    auto p = mini_aero_mesh<SOA>();
+    p.template loopGPUCellCentred<mini_aero::StepGPUGlobal>(2);
+    dump(p, "ma_global_" + t);
+  }
+  {
+    auto p = mini_aero_mesh<SOA>();
+    p.template loopGPUEdgeCentred<mini_aero::StepGPUAtomic>(2);
+    dump(p, "ma_atomic_" + t);
+  }
+  {
     auto p = lulesh_grid<SOA>();
     p.template loopGPUCellCentred<lulesh::StepGPUGlobal>(2);
     dump(p, "lulesh_global_" + t);

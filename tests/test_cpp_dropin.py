@@ -141,6 +141,7 @@ def test_cpp_dropin_gpu_loops(gpu, tmp_path):
     hc = _colour(hexp, ITERS_HEX)
     vs, vS = _serial(_volna_oracle(), 2)
     mas, maS = _serial(_mini_aero_oracle(), 2)
+    mac = _colour(_mini_aero_oracle(), 2)
     ls, lS = _serial(_lulesh_oracle(), 2)
     lc = _colour(_lulesh_oracle(), 2)
     for tag in ("aos", "soa"):
@@ -157,3 +158,6 @@ def test_cpp_dropin_gpu_loops(gpu, tmp_path):
                             what="volna owner " + tag)
         assert_close_scaled(_load(tmp_path, "ma_owner_" + tag, 5), mas, maS,
                             what="mini_aero owner " + tag)
+        assert_bitwise(_load(tmp_path, "ma_global_" + tag, 5), mac, "mini_aero global " + tag)
+        assert_close_scaled(_load(tmp_path, "ma_atomic_" + tag, 5), mas, maS,
+                            what="mini_aero atomic " + tag)
