@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="res_calc",
                     choices=["res_calc", "volna-batch", "mini_aero", "hex", "lulesh", "mine",
-                             "getacc"],
+                             "getacc", "copy"],
                     help="res_calc: the airfoil edge loop (the headline line); the others "
                          "print their own line: 64 batched Volna meshes, mini_aero 96^3, "
                          "hex scatter 128^3, lulesh element forces 128^3, getacc corner "
@@ -186,23 +186,43 @@ def launch_durations(loop, steps):
     return float(ms.mean()), float(np.median(ms))
 
 
-def copy_bandwidth(nbytes=256 << 20, reps=10):
-    """Bytes/s a plain device copy (fot_copy, uint4 loads and stores) moves,
-    read + write, over a buffer larger than the 256 MB Infinity Cache: what
-    DRAM delivers to a streaming kernel on this GPU, for context beside the
-    nominal peak."""
+def copy_bandwidth(nbytes=256 << 20, reps=10, how="kernel"):
+    """Bytes/s a device copy moves, read + write: fot_copy (the streaming
+    copy kernel, 16-byte nontemporal loads and stores, four in flight per
+    lane) or hipMemcpyAsync device to device ("runtime").  Beyond the 256 MB
+    Infinity Cache this is what DRAM delivers to a streaming kernel on this
+    GPU, for context beside the nominal peak."""
     from cuda_fot_amd import DeviceArray, Timer, capi, synchronize
+    fn = {"kernel": "fot_copy", "runtime": "fot_memcpy_dtod"}[how]
     a = DeviceArray((nbytes // 8,), np.float64)
     b = DeviceArray((nbytes // 8,), np.float64)
-    capi.call("fot_copy", b.ptr, a.ptr, nbytes, None)
+    capi.call(fn, b.ptr, a.ptr, nbytes, None)
     synchronize()
     t = Timer()
     t.start()
     for _ in range(reps):
-        capi.call("fot_copy", b.ptr, a.ptr, nbytes, None)
+        capi.call(fn, b.ptr, a.ptr, nbytes, None)
     t.stop()
     synchronize()
     return 2.0 * nbytes * reps / (t.elapsed_ms() / 1e3)
+
+
+def bench_copy(args):
+    """The DRAM ceiling the loops are compared with: device copies of 32 MiB
+    to 1 GiB each way by the streaming copy kernel and by hipMemcpyAsync,
+    best of three timings of 10 copies each."""
+    sizes = [32 << 20, 64 << 20, 128 << 20, 256 << 20, 512 << 20, 1 << 30]
+    rows = {}
+    for n in sizes:
+        rows[str(n >> 20)] = {how: round(max(copy_bandwidth(n, 10, how) for _ in range(3)) / 1e9, 1)
+                              for how in ("kernel", "runtime")}
+    best_dram = max(max(r.values()) for k, r in rows.items() if int(k) >= 512)
+    return _line("device copy GB/s (read + write) by size each way", best_dram * 1e9, "B/s",
+                 10, 1, 2.0 * (1 << 30) / (best_dram * 1e9) * 1e3,
+                 "device copy, 32 MiB - 1 GiB each way",
+                 copy_gbps_by_mib=rows,
+                 note="value: the best rate at 512 MiB - 1 GiB each way (beyond the 256 MB "
+                      "Infinity Cache)")
 
 
 def load_traffic(args):
@@ -226,6 +246,17 @@ def load_traffic(args):
             ("hbm" if args.hbm_lists else "lds") == c.get("lists", "hbm") and
             args.soa == c.get("soa", False))
     return t["traffic_bytes"] if same else None
+
+
+def load_workload_traffic(workload: str, variant: str):
+    """HBM bytes per step of a secondary workload's line from its committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE summed over the
+    step's launches, scaled by the copy-kernel calibration), or None when no
+    summary was taken for this variant."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_{variant}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    return json.load(open(path))["traffic_bytes"]
 
 
 def cpu_model() -> str:
@@ -529,7 +560,7 @@ def bench_single(args, workload):
         l.close()
         return (wall, ev, st) if with_stats else (wall, ev)
 
-    wa, _ = run(p, capi.STRATEGY_ATOMIC, 0, 256)
+    wa = run(p, capi.STRATEGY_ATOMIC, 0, 256)[0] if not args.no_atomic_ref else None
     sweep = None
     if args.sweep_block_sizes:
         # atomics on the mesh as generated; the hierarchical loop (LDS-cached
@@ -551,21 +582,37 @@ def bench_single(args, workload):
                 row["hierarchical_error"] = str(e)
             sweep[str(bs)] = row
         del g
-    p.partition_points(args.points_per_block, args.numbering)
-    wall, ev = run(p, capi.STRATEGY_HIERARCHICAL, capi.LOOP_OWNER_COMPUTE,
-                   args.points_per_block)
+    if args.partition == "gps":
+        # the reference-shaped hierarchical loop (one launch per block
+        # colour) on the GPS-ordered mesh, consecutive blocks of
+        # --block-size cells: the autotuner's pick on these meshes
+        p.reorder("gps")
+        wall, ev, st = run(p, capi.STRATEGY_HIERARCHICAL, 0, args.block_size, with_stats=True)
+        variant = f"gps_b{args.block_size}"
+        cfg = {"strategy": "hierarchical", "partition": "gps", "block_size": args.block_size,
+               "launches_per_step": st["num_launches_per_iter"]}
+    else:
+        p.partition_points(args.points_per_block, args.numbering)
+        wall, ev, st = run(p, capi.STRATEGY_HIERARCHICAL, capi.LOOP_OWNER_COMPUTE,
+                           args.points_per_block, with_stats=True)
+        variant = f"owner_ppb{args.points_per_block}"
+        cfg = {"strategy": "owner compute", "points_per_block": args.points_per_block,
+               "redundancy": round(st["redundancy"], 4)}
     unit = {"mini_aero": "faces/s", "mine": "edges/s"}.get(workload, "elements/s")
     extra = {}
+    if wa is not None:
+        extra.update(atomic_per_s=units * args.steps / wa, speedup_vs_atomic=wa / wall)
     if workload == "mine" and not args.no_cpu:
         extra["cpu_baseline"] = cpu_baseline_mine(p, f, args.cpu_seconds)
     line = _line(f"{workload} {unit}", units * args.steps / wall, unit, args.steps, args.warmup,
                  wall / args.steps * 1e3, name,
                  roofline={"bound": "hbm", "achieved": ubytes * args.steps / ev / 1e9,
                            "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                           "frac": ubytes * args.steps / ev / PEAK_HBM, "traffic": None,
+                           "frac": ubytes * args.steps / ev / PEAK_HBM,
+                           "traffic": load_workload_traffic(workload, variant),
                            "algorithmic_bytes_per_step": ubytes},
-                 atomic_per_s=units * args.steps / wa,
-                 speedup_vs_atomic=wa / wall, **extra)
+                 **extra)
+    line["config"].update(cfg)
     if sweep is not None:
         line["block_size_sweep"] = sweep
     return line
@@ -582,6 +629,9 @@ def main():
     if capi.device_count() < 1:
         print("no HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "copy":
+        print(json.dumps(bench_copy(args)), flush=True)
+        return
     if args.workload == "volna-batch":
         args.tune = args.tune or "none"
         print(json.dumps(bench_volna_batch(args)), flush=True)

@@ -41,9 +41,13 @@ fot_status dispatch(const fot_functor *f, int soa, Body &&body) {
     FOT_MINE_PC(2, 4)
     FOT_MINE_PC(2, 8)
     FOT_MINE_PC(4, 1)
+    FOT_MINE_PC(4, 2)
     FOT_MINE_PC(4, 4)
+    FOT_MINE_PC(4, 8)
     FOT_MINE_PC(8, 1)
+    FOT_MINE_PC(8, 2)
     FOT_MINE_PC(8, 4)
+    FOT_MINE_PC(8, 8)
 #undef FOT_MINE_PC
     return fail(FOT_ERR_UNSUPPORTED, "mine: MeshDim %u PointDim %u CellDim %u not built",
                 f->mesh_dim, f->point_dim, f->cell_dim);

@@ -162,17 +162,46 @@ extern "C" fot_status fot_stream_synchronize(void *stream) {
   });
 }
 
-// copyKernel (data_t.cu:76-93): 16-byte vectors, a scalar tail.
-__global__ __launch_bounds__(256) void fot_copy_kernel(const uint4 *__restrict__ src,
-                                                       uint4 *__restrict__ dst,
-                                                       size_t n16,
-                                                       const unsigned char *__restrict__ tail_src,
-                                                       unsigned char *__restrict__ tail_dst,
-                                                       unsigned tail) {
-  const size_t stride = size_t(gridDim.x) * blockDim.x;
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n16; i += stride)
-    dst[i] = src[i];
+// copyKernel (data_t.cu:76-93), streaming form: each workgroup moves tiles
+// of kCopyUnroll x 256 16-byte vectors, all loads of a tile issued before
+// its stores (four 16-byte loads in flight per lane), with nontemporal hints
+// so a copy larger than the caches does not evict what the loop keeps
+// there.  Lanes of one load instruction touch consecutive vectors.  The
+// scalar tail (bytes % 16) is copied by workgroup 0.
+constexpr unsigned kCopyThreads = 256;
+constexpr unsigned kCopyUnroll = 4;
+constexpr size_t kCopyTile = size_t(kCopyThreads) * kCopyUnroll;
+// the nontemporal builtins take native vector types, not HIP's uint4 struct
+typedef unsigned int copy_vec __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(kCopyThreads) void fot_copy_kernel(
+    const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16,
+    const unsigned char *__restrict__ tail_src, unsigned char *__restrict__ tail_dst,
+    unsigned tail) {
+  const size_t full = n16 / kCopyTile;
+  for (size_t t = blockIdx.x; t < full; t += gridDim.x) {
+    const size_t base = t * kCopyTile + threadIdx.x;
+    const copy_vec *s = reinterpret_cast<const copy_vec *>(src) + base;
+    copy_vec *d = reinterpret_cast<copy_vec *>(dst) + base;
+    copy_vec v[kCopyUnroll];
+#pragma unroll
+    for (unsigned k = 0; k < kCopyUnroll; ++k) v[k] = __builtin_nontemporal_load(s + k * kCopyThreads);
+#pragma unroll
+    for (unsigned k = 0; k < kCopyUnroll; ++k) __builtin_nontemporal_store(v[k], d + k * kCopyThreads);
+  }
+  // the vectors after the last whole tile: fewer than kCopyTile
+  if (blockIdx.x == gridDim.x - 1) {
+    for (size_t i = full * kCopyTile + threadIdx.x; i < n16; i += kCopyThreads)
+      dst[i] = src[i];
+  }
   if (blockIdx.x == 0 && threadIdx.x < tail) tail_dst[threadIdx.x] = tail_src[threadIdx.x];
+}
+
+// Workgroups of one copy launch: enough for every tile up to 8 per CU of
+// the 256 CUs at once, then the tiles are taken grid-stride.
+static unsigned copy_grid(size_t n16) {
+  const size_t tiles = n16 / kCopyTile;
+  return unsigned(std::max<size_t>(1, std::min<size_t>(tiles, 2048)));
 }
 
 static fot_status fot_copy_impl(void *dst, const void *src, size_t bytes, void *stream) {
@@ -182,8 +211,7 @@ static fot_status fot_copy_impl(void *dst, const void *src, size_t bytes, void *
     return fot_memcpy_dtod(dst, src, bytes, stream);
   const size_t n16 = bytes / 16;
   const unsigned tail = unsigned(bytes % 16);
-  const unsigned blocks = unsigned(std::min<size_t>((n16 + 255) / 256 + 1, 8192));
-  hipLaunchKernelGGL(fot_copy_kernel, dim3(blocks), dim3(256), 0,
+  hipLaunchKernelGGL(fot_copy_kernel, dim3(copy_grid(n16)), dim3(kCopyThreads), 0,
                      static_cast<hipStream_t>(stream), static_cast<const uint4 *>(src),
                      static_cast<uint4 *>(dst), n16,
                      static_cast<const unsigned char *>(src) + n16 * 16,

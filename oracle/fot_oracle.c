@@ -17,6 +17,26 @@
 #include <omp.h>
 #endif
 
+/* Planted mistakes.  ORACLE_PLANT is 0 in every normal build.  A nonzero
+ * value flips exactly one rule of the restatement, the kind of slip a
+ * restatement could plausibly make; tests/test_oracle_planted.py builds each
+ * variant into oracle/lib/planted/ and shows that a known-answer test fails
+ * on it, so every such rule is pinned by a test that notices it flipped. */
+#ifndef ORACLE_PLANT
+#define ORACLE_PLANT 0
+#endif
+enum {
+  PLANT_NONE = 0,
+  PLANT_BLOCK_COLOUR_FIRST_FIT = 1, /* first available block colour, not least used */
+  PLANT_PART_CHANGE_KEEPS_TID = 2,  /* no tid reset where the partition changes */
+  PLANT_THREAD_ORDER_REVERSED = 3,  /* positions descending inside a thread colour */
+  PLANT_CACHE_DESCENDING = 4,       /* points_to_be_cached in descending order */
+  PLANT_MINE_NO_FACTOR_2 = 5,       /* mine_func<2> without its factor 2 */
+  PLANT_MINE_SAME_SIDE = 6,         /* mine_func<2>: out_L from L, not from R */
+  PLANT_MINE_RING_NEIGHBOUR = 7,    /* mine_func<4/8>: neighbours k+1 and k+2 */
+  PLANT_GLOBAL_COLOUR_FIRST_FIT = 8 /* Mesh::colourCells first fit */
+};
+
 static double now_seconds(void) {
   struct timespec ts;
   clock_gettime(CLOCK_MONOTONIC, &ts);
@@ -429,11 +449,15 @@ static void compute_inc(const oracle_problem *p, uint32_t ind, double *inc) {
       const uint32_t cd = (C == P) ? d : 0;
       const double wd = w[(size_t)cd * cs];
       if (md == 2) {
-        inc[0 * P + d] = 2 * pt[1][d * ps] * wd;
-        inc[1 * P + d] = 2 * pt[0][d * ps] * wd;
+        const double f = ORACLE_PLANT == PLANT_MINE_NO_FACTOR_2 ? 1 : 2;
+        const int same = ORACLE_PLANT == PLANT_MINE_SAME_SIDE;
+        inc[0 * P + d] = f * pt[same ? 0 : 1][d * ps] * wd;
+        inc[1 * P + d] = f * pt[same ? 1 : 0][d * ps] * wd;
       } else {
         for (uint32_t k = 0; k < md; ++k) {
-          const uint32_t a = (k + 1) % md, b = (k + md - 1) % md;
+          const uint32_t a = (k + 1) % md;
+          const uint32_t b = ORACLE_PLANT == PLANT_MINE_RING_NEIGHBOUR ? (k + 2) % md
+                                                                      : (k + md - 1) % md;
           inc[k * P + d] = (pt[a][d * ps] + pt[b][d * ps]) * wd;
         }
       }
@@ -800,7 +824,8 @@ int oracle_colour_cells(const uint32_t *c2n, uint32_t num_cells,
       ++ncol;
       avail = cset_andnot(used, occ);
     }
-    colour = avail_min_size(avail, set_sizes, 64);
+    colour = ORACLE_PLANT == PLANT_GLOBAL_COLOUR_FIRST_FIT ? avail_first(avail)
+                                                           : avail_min_size(avail, set_sizes, 64);
     if (colour >= 64 || ncol > 64) {
       free(pc);
       return -1; /* the reference warns "not enough colour" and overruns */
@@ -910,7 +935,8 @@ typedef struct {
 
 static int cmp_ptref(const void *a, const void *b) {
   const pt_ref *x = (const pt_ref *)a, *y = (const pt_ref *)b;
-  if (x->point != y->point) return x->point < y->point ? -1 : 1;
+  const int asc = ORACLE_PLANT == PLANT_CACHE_DESCENDING ? 1 : -1;
+  if (x->point != y->point) return x->point < y->point ? asc : -asc;
   if (x->local_cell != y->local_cell) return x->local_cell < y->local_cell ? -1 : 1;
   if (x->slot != y->slot) return x->slot < y->slot ? -1 : 1;
   return 0;
@@ -952,7 +978,7 @@ int oracle_hplan_build(const uint32_t *const *c2n, const uint32_t *mesh_dim,
         continue;
       }
       if ((pairs[i - 1] >> 32) != (pairs[i] >> 32)) {
-        tid = 0;
+        if (ORACLE_PLANT != PLANT_PART_CHANGE_KEEPS_TID) tid = 0;
         blocks[nblk++] = i;
       }
     }
@@ -1002,7 +1028,9 @@ int oracle_hplan_build(const uint32_t *const *c2n, const uint32_t *mesh_dim,
       set_sizes[ncol++] = 0;
       avail = cset_andnot(used, occ);
     }
-    const unsigned colour = avail_min_size(avail, set_sizes, ncol);
+    const unsigned colour = ORACLE_PLANT == PLANT_BLOCK_COLOUR_FIRST_FIT
+                                ? avail_first(avail)
+                                : avail_min_size(avail, set_sizes, ncol);
     ++set_sizes[colour];
     bcol[b] = (uint8_t)colour;
     /* colourBlock: colour written points, sorted unique points, local ids */
@@ -1036,6 +1064,13 @@ int oracle_hplan_build(const uint32_t *const *c2n, const uint32_t *mesh_dim,
       for (uint32_t i = 0; i < bs; ++i) cnt[tmp_col[i] + 1]++;
       for (int k = 0; k < 256; ++k) cnt[k + 1] += cnt[k];
       for (uint32_t i = 0; i < bs; ++i) perm[cnt[tmp_col[i]]++] = i;
+      if (ORACLE_PLANT == PLANT_THREAD_ORDER_REVERSED)
+        for (uint32_t k = 0, lo = 0; k < 256 && lo < bs; lo = cnt[k++])
+          for (uint32_t a = lo, z = cnt[k]; a + 1 < z; ++a, --z) {
+            const uint32_t t = perm[a];
+            perm[a] = perm[z - 1];
+            perm[z - 1] = t;
+          }
     }
     for (uint32_t i = 0; i < bs; ++i) {
       tcol[from + i] = tmp_col[perm[i]];

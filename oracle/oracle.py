@@ -51,10 +51,13 @@ class Hplan(C.Structure):
     ]
 
 
-def _load():
-    if not os.path.exists(LIB_PATH):
+def load(path: str = LIB_PATH):
+    """The oracle library at `path` with its signatures set: the normal
+    build, or a planted-mistake variant (oracle/lib/planted/, made by
+    `make planted`) for the tests that show a known answer catches it."""
+    if path == LIB_PATH and not os.path.exists(LIB_PATH):
         build()
-    lib = C.CDLL(LIB_PATH)
+    lib = C.CDLL(path)
     V, U = C.c_void_p, C.c_uint32
     sig = {
         "oracle_res_calc_func": [V] * 8 + [C.c_uint, C.c_uint],
@@ -88,7 +91,7 @@ def _load():
     return lib
 
 
-lib = _load()
+lib = load()
 
 
 def _p(a):
@@ -107,7 +110,9 @@ class OracleProblem:
     (dim, C) arrays.
     """
 
-    def __init__(self, kernel, maps, num_points, point_data, cell_data=(), soa=False):
+    def __init__(self, kernel, maps, num_points, point_data, cell_data=(), soa=False,
+                 library=None):
+        self.lib = library or lib
         self.maps = [_u32(m) for m in maps]
         self.point_data = [None if d is None else np.array(d, dtype=np.float64, order="C")
                            for d in point_data]
@@ -143,7 +148,7 @@ class OracleProblem:
         """Serial loop; returns S (the error scale) when with_scale."""
         scale = np.zeros_like(self.point_data[0]) if with_scale else None
         secs = C.c_double(0.0)
-        if lib.oracle_loop_serial(C.byref(self.s), iters, _p(scale), C.byref(secs)) != 0:
+        if self.lib.oracle_loop_serial(C.byref(self.s), iters, _p(scale), C.byref(secs)) != 0:
             raise RuntimeError("oracle_loop_serial failed")
         self.last_loop_seconds = secs.value
         return scale
@@ -151,13 +156,13 @@ class OracleProblem:
     def loop_colour(self, iters: int, nthreads: int = 1) -> float:
         """Colour-major loop; returns the iteration loop's seconds."""
         secs = C.c_double(0.0)
-        if lib.oracle_loop_colour(C.byref(self.s), iters, nthreads, C.byref(secs)) != 0:
+        if self.lib.oracle_loop_colour(C.byref(self.s), iters, nthreads, C.byref(secs)) != 0:
             raise RuntimeError("oracle_loop_colour failed")
         self.last_loop_seconds = secs.value
         return secs.value
 
     def loop_hier_replay(self, plan: "OraclePlan", iters: int) -> None:
-        if lib.oracle_loop_hier_replay(C.byref(self.s), C.byref(plan.h), iters) != 0:
+        if self.lib.oracle_loop_hier_replay(C.byref(self.s), C.byref(plan.h), iters) != 0:
             raise RuntimeError("oracle_loop_hier_replay failed")
 
 
@@ -185,13 +190,14 @@ def loop_owner_replay(problem: "OracleProblem", plan: dict, iters: int) -> None:
 class OraclePlan:
     """oracle_hplan_build: the hierarchical plan as one dict per colour."""
 
-    def __init__(self, maps, num_points0, block_size, partition=None):
+    def __init__(self, maps, num_points0, block_size, partition=None, library=None):
+        self.lib = library or lib
         self.maps = [_u32(m) for m in maps]
         arr = (C.c_void_p * len(self.maps))(*[_p(m) for m in self.maps])
         dims = np.array([m.shape[1] for m in self.maps], dtype=np.uint32)
         part = None if partition is None else _u32(partition)
         self.h = Hplan()
-        rc = lib.oracle_hplan_build(arr, _p(dims), len(self.maps), self.maps[0].shape[0],
+        rc = self.lib.oracle_hplan_build(arr, _p(dims), len(self.maps), self.maps[0].shape[0],
                                     num_points0, _p(part), block_size, C.byref(self.h))
         if rc != 0:
             raise RuntimeError("oracle_hplan_build failed")
@@ -232,7 +238,7 @@ class OraclePlan:
 
     def __del__(self):
         try:
-            lib.oracle_hplan_free(C.byref(self.h))
+            self.lib.oracle_hplan_free(C.byref(self.h))
         except Exception:
             pass
 
@@ -260,10 +266,11 @@ def lulesh_func(xyz, sig):
     return f.reshape(3, 8).T.copy(), float(det[0])
 
 
-def colour_cells(maps0, num_points):
+def colour_cells(maps0, num_points, library=None):
     m = _u32(maps0)
     col = np.empty(m.shape[0], dtype=np.uint8)
-    n = lib.oracle_colour_cells(_p(m), m.shape[0], m.shape[1], num_points, _p(col))
+    n = (library or lib).oracle_colour_cells(_p(m), m.shape[0], m.shape[1], num_points,
+                                             _p(col))
     if n < 0:
         raise RuntimeError("colour overflow")
     return col, n

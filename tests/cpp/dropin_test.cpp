@@ -5,8 +5,12 @@
 // <outdir>/<name>.bin for the Python side to check against the oracle.
 //   dropin_test cpu <outdir>   CPU loops, pre-passes and IO round trips
 //   dropin_test gpu <outdir>   GPU loops (needs a device)
+//   dropin_test step <outdir>  the StepGPU*::call<SOA> functors driven per
+//                              colour as the reference's Problem drives them
 // Exit status 0 on success; a failed self-check prints and exits 1.
+#include <algorithm>
 #include <cstdio>
+#include <cstring>
 #include <cstdlib>
 #include <fstream>
 #include <sstream>
@@ -368,11 +372,256 @@ template <bool SOA> void gpu_cases(const char *tag) {
   }
 }
 
+
+// ---- the functor-level step API driven as the reference's Problem drives it
+// A device buffer that frees itself (the reference's device_data_t,
+// data_t.cu:58-74).
+struct DevBuf {
+  void *p = nullptr;
+  DevBuf() = default;
+  DevBuf(const void *host, std::size_t bytes) {
+    if (!bytes) return;
+    fot::check(fot_malloc(&p, bytes), "fot_malloc");
+    fot::check(fot_memcpy_htod(p, host, bytes), "fot_memcpy_htod");
+  }
+  DevBuf(DevBuf &&o) noexcept : p(o.p) { o.p = nullptr; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    std::swap(p, o.p);
+    return *this;
+  }
+  DevBuf(const DevBuf &) = delete;
+  ~DevBuf() {
+    if (p) fot_free(p);
+  }
+};
+template <class T> DevBuf dev_vec(const std::vector<T> &v) {
+  return DevBuf(v.data(), v.size() * sizeof(T));
+}
+// a device array of device pointers
+DevBuf dev_ptrs(const std::vector<const void *> &v) { return dev_vec(v); }
+
+// cell data entry cw gathered for `cells` (SOA, stride cells.size()), as
+// problem.hpp:124-143 and copyCellWeights (colouring.hpp:310-335) lay it out
+template <bool SOA>
+std::vector<char> gather_cells(const Problem<SOA> &p, unsigned cw, const MY_SIZE *cells,
+                               MY_SIZE n) {
+  const data_t &w = p.cell_weights[cw];
+  const MY_SIZE dim = w.getDim(), ts = w.getTypeSize(), nc = p.mesh.numCells();
+  std::vector<char> out(std::size_t(n) * dim * ts);
+  for (MY_SIZE i = 0; i < n; ++i)
+    for (MY_SIZE d = 0; d < dim; ++d)
+      std::memcpy(out.data() + std::size_t(index<true>(n, i, dim, d)) * ts,
+                  w.cbegin() + std::size_t(index<true>(nc, cells[i], dim, d)) * ts, ts);
+  return out;
+}
+
+// The point data's device mirrors and strides; `out` starts as a copy of
+// point data 0 (problem.hpp:92-95, 203-209).
+template <bool SOA> struct StepPoints {
+  std::vector<const void *> host_ptrs;
+  DevBuf ptrs, stride, out;
+  std::size_t out_bytes = 0;
+  explicit StepPoints(Problem<SOA> &p) {
+    for (data_t &pw : p.point_weights) {
+      pw.initDeviceMemory();
+      host_ptrs.push_back(pw.getDeviceData());
+    }
+    ptrs = dev_ptrs(host_ptrs);
+    stride = dev_vec(p.mesh.numPoints());
+    out_bytes = p.point_weights[0].bytes();
+    out = DevBuf(p.point_weights[0].cbegin(), out_bytes);
+  }
+  // after each iteration point data 0 = out (problem.hpp:179-185, 320-327)
+  void copy_back(Problem<SOA> &p) {
+    fot::check(fot_memcpy_dtod(p.point_weights[0].getDeviceData(), out.p, out_bytes, nullptr),
+               "fot_memcpy_dtod");
+    fot::check(fot_device_synchronize(), "fot_device_synchronize");
+  }
+};
+
+// Problem::loopGPUCellCentred (problem.hpp:90-194) with the launches made
+// by Step::call<SOA> itself: per colour, compacted AOS cell lists and SOA
+// cell data of the colour's cells, device arrays of device pointers, a
+// synchronisation after each launch.  `atomic` runs every cell as one
+// colour through the atomics functor (the launch behind loopGPUEdgeCentred).
+template <class Step, bool SOA>
+void step_loop_global(Problem<SOA> &p, MY_SIZE num, MY_SIZE block_size, bool atomic) {
+  std::vector<std::vector<MY_SIZE>> colours;
+  if (atomic) {
+    colours.emplace_back(p.mesh.numCells());
+    for (MY_SIZE i = 0; i < p.mesh.numCells(); ++i) colours[0][i] = i;
+  } else {
+    colours = p.mesh.colourCells();
+  }
+  std::vector<std::vector<DevBuf>> lists(colours.size()), weights(colours.size());
+  std::vector<DevBuf> list_ptrs, weight_ptrs;
+  for (std::size_t c = 0; c < colours.size(); ++c) {
+    const std::vector<MY_SIZE> &cells = colours[c];
+    std::vector<const void *> lp, wp;
+    for (unsigned m = 0; m < p.mesh.numMappings(); ++m) {
+      const MY_SIZE dim = p.mesh.cell_to_node[m].getDim();
+      std::vector<MY_SIZE> l(std::size_t(cells.size()) * dim);
+      for (std::size_t i = 0; i < cells.size(); ++i)
+        std::copy_n(p.mesh.cell_to_node[m].template cbegin<MY_SIZE>() + dim * cells[i], dim,
+                    l.begin() + dim * i);
+      lists[c].push_back(dev_vec(l));
+      lp.push_back(lists[c].back().p);
+    }
+    for (unsigned cw = 0; cw < p.cell_weights.size(); ++cw) {
+      weights[c].push_back(dev_vec(gather_cells(p, cw, cells.data(), MY_SIZE(cells.size()))));
+      wp.push_back(weights[c].back().p);
+    }
+    list_ptrs.push_back(dev_ptrs(lp));
+    weight_ptrs.push_back(dev_ptrs(wp.empty() ? std::vector<const void *>{nullptr} : wp));
+  }
+  StepPoints<SOA> pts(p);
+  for (MY_SIZE it = 0; it < num; ++it) {
+    for (std::size_t c = 0; c < colours.size(); ++c) {
+      const MY_SIZE n = MY_SIZE(colours[c].size());
+      const MY_SIZE nb = (n + block_size - 1) / block_size;
+      Step::template call<SOA>(static_cast<const void **>(pts.ptrs.p), pts.out.p,
+                               static_cast<void **>(weight_ptrs[c].p),
+                               static_cast<const MY_SIZE **>(list_ptrs[c].p), n,
+                               static_cast<MY_SIZE *>(pts.stride.p), n, nb, block_size);
+      fot::check(fot_device_synchronize(), "fot_device_synchronize");
+    }
+    pts.copy_back(p);
+  }
+  p.point_weights[0].flushToHost();
+}
+
+// One block colour of a HierarchicalColourMemory on the device
+// (getDeviceMemoryOfOneColour, colouring.hpp:341-393).
+struct DevColour {
+  MY_SIZE num_cells = 0, num_blocks = 0, shared_size = 0;
+  DevBuf ptbc, ptbc_off, ncc, cc, bo, list_ptrs, weight_ptrs;
+  std::vector<DevBuf> lists, weights;
+};
+
+// Problem::loopGPUHierarchical (problem.hpp:198-339) with the launches made
+// by Step::call<SOA>: the plan of fot_hplan_build, one launch per block
+// colour with the reference's cache size (shared_size + 32) * Dim *
+// sizeof(double), a synchronisation after each.
+template <class Step, bool SOA>
+void step_loop_hier(Problem<SOA> &p, MY_SIZE num, MY_SIZE block_size) {
+  std::vector<const std::uint32_t *> maps;
+  std::vector<std::uint32_t> dims;
+  for (unsigned m = 0; m < p.mesh.numMappings(); ++m) {
+    maps.push_back(p.mesh.cell_to_node[m].template cbegin<MY_SIZE>());
+    dims.push_back(p.mesh.cell_to_node[m].getDim());
+  }
+  fot_hplan *plan = nullptr;
+  fot::check(fot_hplan_build(maps.data(), dims.data(), MY_SIZE(maps.size()), p.mesh.numCells(),
+                             p.mesh.numPoints(0), nullptr, block_size, &plan),
+             "fot_hplan_build");
+  fot_hplan_info info{};
+  fot::check(fot_hplan_info_get(plan, &info), "fot_hplan_info_get");
+  std::vector<DevColour> cols(info.num_colours);
+  for (MY_SIZE c = 0; c < info.num_colours; ++c) {
+    fot_hplan_colour v{};
+    fot::check(fot_hplan_colour_get(plan, c, &v), "fot_hplan_colour_get");
+    DevColour &d = cols[c];
+    d.num_cells = v.num_cells, d.num_blocks = v.num_blocks, d.shared_size = v.shared_size;
+    d.ptbc = DevBuf(v.points_to_be_cached, sizeof(MY_SIZE) * v.num_cached);
+    d.ptbc_off = DevBuf(v.points_to_be_cached_offsets, sizeof(MY_SIZE) * (v.num_blocks + 1));
+    d.ncc = DevBuf(v.num_cell_colours, v.num_blocks);
+    d.cc = DevBuf(v.cell_colours, v.num_cells);
+    d.bo = DevBuf(v.block_offsets, sizeof(MY_SIZE) * (v.num_blocks + 1));
+    std::vector<const void *> lp, wp;
+    for (unsigned m = 0; m < maps.size(); ++m) {
+      d.lists.emplace_back(v.cell_list[m], sizeof(MY_SIZE) * v.num_cells * dims[m]);
+      lp.push_back(d.lists.back().p);
+    }
+    for (unsigned cw = 0; cw < p.cell_weights.size(); ++cw) {
+      d.weights.push_back(dev_vec(gather_cells(p, cw, v.cell_inv_permutation, v.num_cells)));
+      wp.push_back(d.weights.back().p);
+    }
+    d.list_ptrs = dev_ptrs(lp);
+    d.weight_ptrs = dev_ptrs(wp.empty() ? std::vector<const void *>{nullptr} : wp);
+  }
+  fot_hplan_free(plan);
+  StepPoints<SOA> pts(p);
+  const MY_SIZE mult = MY_SIZE(sizeof(double)) * p.point_weights[0].getDim();
+  std::uint32_t flag = 0;
+  fot::check(fot_lds_overflow_flag(&flag, 1), "fot_lds_overflow_flag");
+  for (MY_SIZE it = 0; it < num; ++it) {
+    for (const DevColour &d : cols) {
+      Step::template call<SOA>(
+          static_cast<const void **>(pts.ptrs.p), pts.out.p, static_cast<MY_SIZE *>(d.ptbc.p),
+          static_cast<MY_SIZE *>(d.ptbc_off.p), static_cast<void **>(d.weight_ptrs.p),
+          static_cast<const MY_SIZE **>(d.list_ptrs.p), static_cast<std::uint8_t *>(d.ncc.p),
+          static_cast<std::uint8_t *>(d.cc.p), static_cast<MY_SIZE *>(d.bo.p), d.num_cells,
+          static_cast<MY_SIZE *>(pts.stride.p), d.num_cells, d.num_blocks, block_size,
+          (d.shared_size + 32) * mult);
+      fot::check(fot_device_synchronize(), "fot_device_synchronize");
+    }
+    pts.copy_back(p);
+  }
+  fot::check(fot_lds_overflow_flag(&flag, 0), "fot_lds_overflow_flag");
+  CHECK(flag == 0);
+  // an 8-byte cache holds no block's points (every target has three or more
+  // values): every block skips its work and raises the flag, every block
+  // exits, and the output is untouched
+  const DevColour &d0 = cols[0];
+  std::vector<char> before(pts.out_bytes), after(pts.out_bytes);
+  fot::check(fot_memcpy_dtoh(before.data(), pts.out.p, pts.out_bytes), "fot_memcpy_dtoh");
+  Step::template call<SOA>(
+      static_cast<const void **>(pts.ptrs.p), pts.out.p, static_cast<MY_SIZE *>(d0.ptbc.p),
+      static_cast<MY_SIZE *>(d0.ptbc_off.p), static_cast<void **>(d0.weight_ptrs.p),
+      static_cast<const MY_SIZE **>(d0.list_ptrs.p), static_cast<std::uint8_t *>(d0.ncc.p),
+      static_cast<std::uint8_t *>(d0.cc.p), static_cast<MY_SIZE *>(d0.bo.p), d0.num_cells,
+      static_cast<MY_SIZE *>(pts.stride.p), d0.num_cells, d0.num_blocks, block_size, 8);
+  fot::check(fot_device_synchronize(), "fot_device_synchronize");
+  fot::check(fot_memcpy_dtoh(after.data(), pts.out.p, pts.out_bytes), "fot_memcpy_dtoh");
+  fot::check(fot_lds_overflow_flag(&flag, 1), "fot_lds_overflow_flag");
+  CHECK(flag == 1);
+  CHECK(after == before);
+  fot::check(fot_lds_overflow_flag(&flag, 0), "fot_lds_overflow_flag");
+  CHECK(flag == 0);
+  p.point_weights[0].flushToHost();
+}
+
+template <class Glob, class Atom, class Hier, bool SOA, class Make>
+void step_cases(const std::string &name, Make make, MY_SIZE iters, const std::string &tag) {
+  for (MY_SIZE bs : {128u, 256u, 512u}) {
+    const std::string b = "_b" + std::to_string(bs) + "_" + tag;
+    {
+      auto p = make();
+      step_loop_global<Glob>(p, iters, bs, false);
+      dump(p, "step_" + name + "_global" + b);
+    }
+    {
+      auto p = make();
+      step_loop_global<Atom>(p, iters, bs, true);
+      dump(p, "step_" + name + "_atomic" + b);
+    }
+    {
+      auto p = make();
+      step_loop_hier<Hier>(p, iters, bs);
+      dump(p, "step_" + name + "_hier" + b);
+    }
+  }
+}
+
+template <bool SOA> void step_api_cases(const char *tag) {
+  const std::string t(tag);
+  step_cases<res_calc::StepGPUGlobal, res_calc::StepGPUAtomic, res_calc::StepGPUHierarchical,
+             SOA>("res", [] { return airfoil<SOA>(128); }, 3, t);
+  step_cases<mine::StepGPUGlobal<2, 4, 4>, mine::StepGPUAtomic<2, 4, 4>,
+             mine::StepGPUHierarchical<2, 4, 4>, SOA>("mine", [] { return mine_grid<SOA>(); },
+                                                      3, t);
+  step_cases<hex_scatter::StepGPUGlobal, hex_scatter::StepGPUAtomic,
+             hex_scatter::StepGPUHierarchical, SOA>("hex", [] { return hex_grid<SOA>(); }, 2, t);
+  step_cases<mini_aero::StepGPUGlobal, mini_aero::StepGPUAtomic,
+             mini_aero::StepGPUHierarchical, SOA>("ma", [] { return mini_aero_mesh<SOA>(); }, 2,
+                                                  t);
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
   if (argc != 3) {
-    std::fprintf(stderr, "usage: %s cpu|gpu <outdir>\n", argv[0]);
+    std::fprintf(stderr, "usage: %s cpu|gpu|step <outdir>\n", argv[0]);
     return 2;
   }
   g_out = argv[2];
@@ -385,6 +634,9 @@ int main(int argc, char **argv) {
     } else if (mode == "gpu") {
       gpu_cases<false>("aos");
       gpu_cases<true>("soa");
+    } else if (mode == "step") {
+      step_api_cases<false>("aos");
+      step_api_cases<true>("soa");
     } else {
       return 2;
     }

@@ -126,3 +126,76 @@ def test_batch_with_an_empty_mesh(gpu):
         stats = gpu_batch([only], RES_CALC, iters, use_graph=graph)
         assert stats["num_work_items"] == 0 and stats["uses_graph"] == 0
         assert_bitwise(only.point_data[0], before, f"batch of one empty mesh, graph {graph}")
+
+
+def _with_untouched_points(p, extra, part_size, seed):
+    """`p` with `extra` points that no cell touches appended to every point
+    set read through mapping 0's ids, and an owner partition: parts of
+    `part_size` consecutive touched points; the first third of the new
+    points join part 0 (a block that runs cells and owns untouched points),
+    the rest form two parts of their own (blocks that own points and run no
+    cell)."""
+    n0 = p.num_points[0]
+    same = [k for k in range(len(p.maps)) if np.array_equal(p.maps[k], p.maps[0])]
+    for k in same:
+        d = p.point_data[k]
+        if d is None:
+            continue
+        dim = 1 if d.ndim == 1 else (d.shape[0] if p.soa else d.shape[1])
+        new = meshgen.uniform(extra * dim, seed + k, 0.5, 1.5)
+        if d.ndim == 1:
+            p.point_data[k] = np.concatenate([d, new])
+        elif p.soa:
+            p.point_data[k] = np.concatenate([d, new.reshape(dim, extra)], axis=1)
+        else:
+            p.point_data[k] = np.concatenate([d, new.reshape(extra, dim)], axis=0)
+        p.num_points[k] = n0 + extra
+    touched = (np.arange(n0) // part_size).astype(np.uint32)
+    nparts = int(touched.max()) + 1
+    third = extra // 3
+    tail = np.where(np.arange(extra) < third, 0,
+                    np.where(np.arange(extra) < 2 * third, nparts, nparts + 1)).astype(np.uint32)
+    p.point_partition = np.concatenate([touched, tail])
+    p.block_size = part_size
+    return p
+
+
+@pytest.mark.parametrize("soa", [False, True])
+@pytest.mark.parametrize("kind", ["mine", "volna"])
+def test_owner_blocks_that_run_no_cell(gpu, kind, soa):
+    """Owner-compute blocks that own points no cell touches run no cell:
+    the owner loop and the batch leave those points as they were (mine's
+    Jacobi copy included) and match the oracle's owner replay bit for bit."""
+    from helpers import mine_problem
+    iters = 3
+    if kind == "mine":
+        f, k = capi.functor(capi.KERNEL_MINE, 2, 4, 4), O.MINE2
+
+        def make(i):
+            return _with_untouched_points(mine_problem(N=20, M=14, seed=3 + i, soa=soa),
+                                          30, 64, 40 + i)
+    else:
+        f, k = VOLNA, O.VOLNA
+
+        def make(i):
+            return _with_untouched_points(
+                volna_problem(meshgen.tri_mesh(14, 10, seed=5 + i), soa=soa), 30, 64, 50 + i)
+    for i in range(2):
+        p = make(i)
+        n0 = p.num_points[0] - 30
+        before = p.target_in_original_order().copy()
+        o = oracle_of(make(i), k)
+        O.loop_owner_replay(o, plan.owner_plan(p.maps[0], p.num_points[0], p.block_size, None,
+                                               p.point_partition), iters)
+        p.gpu_loop(f, capi.STRATEGY_HIERARCHICAL, iters, flags=capi.LOOP_OWNER_COMPUTE)
+        assert_bitwise(p.point_data[0], o.target, f"{kind} owner loop, mesh {i}")
+        got = p.point_data[0][:, n0:] if soa else p.point_data[0][n0:]
+        old = before[:, n0:] if soa else before[n0:]
+        assert_bitwise(got, old, f"{kind}: untouched points changed")
+    batch = [make(0), make(1)]
+    gpu_batch(batch, f, iters)
+    for i, b in enumerate(batch):
+        o = oracle_of(make(i), k)
+        O.loop_owner_replay(o, plan.owner_plan(b.maps[0], b.num_points[0], b.block_size, None,
+                                               b.point_partition), iters)
+        assert_bitwise(b.point_data[0], o.target, f"{kind} batch, mesh {i}")

@@ -161,3 +161,40 @@ def test_cpp_dropin_gpu_loops(gpu, tmp_path):
         assert_bitwise(_load(tmp_path, "ma_global_" + tag, 5), mac, "mini_aero global " + tag)
         assert_close_scaled(_load(tmp_path, "ma_atomic_" + tag, 5), mas, maS,
                             what="mini_aero atomic " + tag)
+
+
+def _replay(make, block_size, iters):
+    o = make()
+    o.loop_hier_replay(O.OraclePlan(o.maps, int(o.s.num_points[0]), block_size), iters)
+    return o.target.copy()
+
+
+@pytest.mark.gpu
+def test_cpp_dropin_step_api(gpu, tmp_path):
+    """The reference-signature functors (StepGPUGlobal / StepGPUAtomic /
+    StepGPUHierarchical::call<SOA>, res_calc.hpp:104-117, 197-216), driven by
+    the test program per colour and per block colour exactly as
+    Problem::loopGPUCellCentred / loopGPUHierarchical drive them
+    (problem.hpp:90-194, 198-339), at block sizes 128, 256 and 512, AOS and
+    SOA: the global-colour launches bit for bit against the oracle's colour
+    loop, the hierarchical launches against the oracle's replay of the same
+    plan, the atomics within 1e-12 * S of the serial loop.  The program also
+    checks the LDS-overflow flag: a cache too small for every block skips
+    every block, raises the flag and leaves the output untouched."""
+    assert os.path.exists(BIN), "tests/cpp/bin/dropin_test not built (run build())"
+    _run("step", tmp_path)
+    cases = [("res", _airfoil_oracle(), ITERS_RES, 4), ("mine", _mine_oracle(), ITERS_MINE, 4),
+             ("hex", _hex_oracle(), ITERS_HEX, 3), ("ma", _mini_aero_oracle(), 2, 5)]
+    for name, make, iters, dim in cases:
+        serial, S = _serial(make, iters)
+        colour = _colour(make, iters)
+        for bs in (128, 256, 512):
+            hier = _replay(make, bs, iters)
+            for tag in ("aos", "soa"):
+                b = f"_b{bs}_{tag}"
+                assert_bitwise(_load(tmp_path, f"step_{name}_global{b}", dim), colour,
+                               f"{name} StepGPUGlobal{b}")
+                assert_bitwise(_load(tmp_path, f"step_{name}_hier{b}", dim), hier,
+                               f"{name} StepGPUHierarchical{b}")
+                assert_close_scaled(_load(tmp_path, f"step_{name}_atomic{b}", dim), serial, S,
+                                    what=f"{name} StepGPUAtomic{b}")

@@ -111,3 +111,25 @@ def test_mini_aero_gpu_strategies(gpu, soa):
         assert_close_scaled(p.point_data[0], ref.target, S, what=f"mini_aero {strategy}/{flags}")
         if expect is not None:
             assert_bitwise(p.point_data[0], expect, f"mini_aero {strategy}/{flags} bits")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("soa", [False, True])
+@pytest.mark.parametrize("bs", [128, 256, 512])
+@pytest.mark.parametrize("flags", [0, capi.LOOP_DEPENDENCY_ORDER],
+                         ids=["per-colour", "dependency-order"])
+def test_mini_aero_hierarchical_bitwise(gpu, soa, bs, flags):
+    """The reference-shaped hierarchical loop, one launch per block colour or
+    one dependency-ordered launch, at block sizes that run the 256-thread
+    instantiation (128, 256) and the 1024-thread one (512): bit for bit
+    against the oracle's replay of the same plan.  fot_loop_run reports a
+    dependency wait that ran out as an error, so a pass also shows that every
+    wait ended."""
+    m = meshgen.hex_faces(12, 10, 9, seed=3)
+    iters = 2
+    base = mini_aero_problem(m, block_size=bs, soa=soa)
+    ref = oracle_of(base, O.MINI_AERO)
+    ref.loop_hier_replay(O.OraclePlan(base.maps, base.num_points[0], bs), iters)
+    p = mini_aero_problem(m, block_size=bs, soa=soa)
+    p.gpu_loop(MINI_AERO, capi.STRATEGY_HIERARCHICAL, iters, flags=flags)
+    assert_bitwise(p.point_data[0], ref.target, f"mini_aero hierarchical b{bs} flags {flags}")

@@ -332,3 +332,16 @@ def test_count_cache_lines_matches_reference_rule(soa):
         capi.call("fot_count_cache_lines", capi.ptr(pts), pts.size, dim, 8, line, int(soa),
                   C.byref(out))
         assert out.value == _cache_lines_reference(pts.tolist(), dim, 8, line, soa)
+
+
+# ---- the product's planner against the hand-derived known answers ------
+from kats import HPLAN_KATS, kat_global_colour_least_used  # noqa: E402
+
+
+@pytest.mark.parametrize("kat", HPLAN_KATS, ids=[k.__name__ for k in HPLAN_KATS])
+def test_hplan_known_answers(kat):
+    kat(lambda maps, npts, bs, part: plan.hplan(maps, npts, bs, part))
+
+
+def test_global_colouring_known_answer():
+    kat_global_colour_least_used(plan.colour_cells)
