@@ -271,6 +271,35 @@ def cpu_model() -> str:
     return "unknown"
 
 
+def host_cpu() -> dict:
+    """The host's CPUs (lscpu's model name, sockets, cores, threads) and the
+    threads the CPU legs run: OMP_NUM_THREADS when the environment sets it
+    (a shared GPU host gives each GPU's jobs that share of its cores), else
+    every CPU this process may run on."""
+    import subprocess
+    info = {"cpu_model": cpu_model(), "host_logical_cpus": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core"):
+                info["lscpu_" + k.lower().replace("(s)", "s").replace(" ", "_")] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else \
+        (os.cpu_count() or 1)
+    info["affinity_cpus"] = affinity
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        info["threads"] = min(int(env), affinity)
+        info["threads_from"] = "OMP_NUM_THREADS (this host's CPU share for the job)"
+    else:
+        info["threads"] = affinity
+        info["threads_from"] = "every CPU in the process's affinity mask"
+    return info
+
+
 def cpu_baseline(mesh, seconds):
     """The oracle's colour-parallel loop (problem.hpp:405-437 restated in C,
     OpenMP over all host cores) timed on a bounded number of iterations of
@@ -280,12 +309,8 @@ def cpu_baseline(mesh, seconds):
     from cuda_fot_amd import res_calc_problem
     p = res_calc_problem(mesh)
     o = O.OracleProblem(O.RES_CALC, p.maps, p.num_points, p.point_data)
-    cores = os.cpu_count() or 1
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        pass
-    cores = min(cores, 16)
+    host = host_cpu()
+    cores = host["threads"]
     # size the sample from one timed iteration so the leg takes ~`seconds`
     per = o.loop_colour(1, cores)
     iters = int(max(1, min(2000, seconds / max(per, 1e-6))))
@@ -294,9 +319,10 @@ def cpu_baseline(mesh, seconds):
     serial.loop_serial(2)
     serial_per = serial.last_loop_seconds / 2
     return {"value": mesh["num_edges"] * iters / secs, "unit": "edges/s", "cores": cores,
-            "kind": "port", "cpu_model": cpu_model(),
+            "kind": "port", "host": host, "build": "gcc -O3 -fopenmp -ffp-contract=off",
             "sample": f"{iters} iterations ({secs:.1f} s) of the colour-parallel OpenMP oracle "
-                      f"loop on the same {mesh['nx']}x{mesh['ny']} mesh, iteration loop only",
+                      f"loop (problem.hpp:382-437 restated) on {cores} threads, on the same "
+                      f"{mesh['nx']}x{mesh['ny']} mesh, iteration loop only",
             "serial_edges_per_s": mesh["num_edges"] / serial_per}
 
 
@@ -307,8 +333,8 @@ def cpu_baseline_mine(p, f, seconds):
     per-iteration copy excluded, as the reference times them)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    cores = min(len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
-                else (os.cpu_count() or 1), 16)
+    host = host_cpu()
+    cores = host["threads"]
     ne = p.maps[0].shape[0]
     o = O.OracleProblem(O.MINE2, p.maps, p.num_points, p.point_data, p.cell_data)
     per = o.loop_colour(1, cores)
@@ -321,7 +347,7 @@ def cpu_baseline_mine(p, f, seconds):
     s.loop_serial(it_s)
     secs_s = s.last_loop_seconds
     return {"value": ne * iters / secs, "unit": "edges/s", "cores": cores, "kind": "port",
-            "cpu_model": cpu_model(),
+            "host": host, "build": "gcc -O3 -fopenmp -ffp-contract=off",
             "sample": f"{iters} iterations ({secs:.1f} s) of the colour-parallel OpenMP oracle "
                       f"loop and {it_s} ({secs_s:.1f} s) of the serial one on the same grid, "
                       f"iteration loop only",

@@ -258,17 +258,105 @@ void cuthill_mckee(const uint32_t *ci, const uint32_t *ce, uint32_t n,
 
 } // namespace
 
-static fot_status fot_rcm_order_impl(const uint32_t *ci, const uint32_t *ce,
-                                    uint32_t n, uint32_t *perm) {
+// Reverse Cuthill-McKee as scipy.sparse.csgraph.reverse_cuthill_mckee
+// computes it (scipy 1.15, the routine scripts/rcm.py:20 calls), restated
+// from its published algorithm:
+//  * the pattern is that of A + A^T (rows ascending, no duplicates);
+//  * a vertex's degree is its row length, plus one when the row holds the
+//    vertex itself;
+//  * components are seeded in a degree order (`seed_order`: scipy takes
+//    numpy's argsort of the degrees, whose order among equal degrees is the
+//    host numpy's; NULL here means a stable sort, ties to the lower id);
+//  * breadth first from each seed: each vertex of the current level appends
+//    its unvisited neighbours in row order, then those are insertion-sorted
+//    by degree (stable);
+//  * the visiting order, reversed, is the new numbering.
+static fot_status fot_rcm_order_impl(const uint32_t *ci, const uint32_t *ce, uint32_t n,
+                                    const uint32_t *seed_order, uint32_t *perm) {
   if (!ci || !perm || (n && !ce && ci[n])) return fail(FOT_ERR_INVALID_ARGUMENT, "null");
+  // the symmetric pattern, rows sorted and unique
+  std::vector<uint32_t> off(size_t(n) + 1, 0);
+  for (uint32_t i = 0; i < n; ++i)
+    for (uint32_t r = ci[i]; r < ci[i + 1]; ++r) {
+      if (ce[r] >= n) return fail(FOT_ERR_INVALID_ARGUMENT, "endpoint %u >= %u", ce[r], n);
+      ++off[i + 1];
+      ++off[ce[r] + 1];
+    }
+  for (uint32_t i = 0; i < n; ++i) off[i + 1] += off[i];
+  std::vector<uint32_t> adj(off[n]), fill(off.begin(), off.end() - 1);
+  for (uint32_t i = 0; i < n; ++i)
+    for (uint32_t r = ci[i]; r < ci[i + 1]; ++r) {
+      adj[fill[i]++] = ce[r];
+      adj[fill[ce[r]]++] = i;
+    }
+  std::vector<uint32_t> row(size_t(n) + 1, 0), deg(n);
+  {
+    uint32_t w = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      auto b = adj.begin() + off[i], e = adj.begin() + off[i + 1];
+      std::sort(b, e);
+      e = std::unique(b, e);
+      row[i] = w;
+      const bool diag = std::binary_search(b, e, i);
+      for (auto it = b; it != e; ++it) adj[w++] = *it;
+      deg[i] = uint32_t(e - b) + (diag ? 1u : 0u);
+    }
+    row[n] = w;
+  }
+  std::vector<uint32_t> seeds(n);
+  if (seed_order) {
+    std::vector<uint8_t> hit(n, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+      if (seed_order[i] >= n || hit[seed_order[i]])
+        return fail(FOT_ERR_INVALID_ARGUMENT, "seed_order is not a permutation");
+      hit[seed_order[i]] = 1;
+      seeds[i] = seed_order[i];
+    }
+  } else {
+    for (uint32_t i = 0; i < n; ++i) seeds[i] = i;
+    std::stable_sort(seeds.begin(), seeds.end(),
+                     [&](uint32_t a, uint32_t b) { return deg[a] < deg[b]; });
+  }
   std::vector<uint32_t> order;
-  cuthill_mckee(ci, ce, n, order);
+  order.reserve(n);
+  std::vector<uint8_t> seen(n, 0);
+  for (uint32_t z = 0; z < n && order.size() < n; ++z) {
+    const uint32_t seed = seeds[z];
+    if (seen[seed]) continue;
+    seen[seed] = 1;
+    order.push_back(seed);
+    size_t level_start = order.size() - 1, level_end = order.size();
+    while (level_start < level_end) {
+      for (size_t ii = level_start; ii < level_end; ++ii) {
+        const uint32_t v = order[ii];
+        const size_t first = order.size();
+        for (uint32_t r = row[v]; r < row[v + 1]; ++r)
+          if (!seen[adj[r]]) {
+            seen[adj[r]] = 1;
+            order.push_back(adj[r]);
+          }
+        // insertion sort of the appended vertices by degree, stable
+        for (size_t k = first + 1; k < order.size(); ++k) {
+          const uint32_t t = order[k];
+          size_t l = k;
+          while (l > first && deg[t] < deg[order[l - 1]]) {
+            order[l] = order[l - 1];
+            --l;
+          }
+          order[l] = t;
+        }
+      }
+      level_start = level_end;
+      level_end = order.size();
+    }
+  }
   for (uint32_t i = 0; i < n; ++i) perm[order[i]] = n - 1 - i; // reversed
   return FOT_OK;
 }
-extern "C" fot_status fot_rcm_order(const uint32_t *ci, const uint32_t *ce,
-                                    uint32_t n, uint32_t *perm) {
-  return fot::guarded("fot_rcm_order", [&] { return fot_rcm_order_impl(ci, ce, n, perm); });
+extern "C" fot_status fot_rcm_order(const uint32_t *ci, const uint32_t *ce, uint32_t n,
+                                    const uint32_t *seed_order, uint32_t *perm) {
+  return fot::guarded("fot_rcm_order",
+                      [&] { return fot_rcm_order_impl(ci, ce, n, seed_order, perm); });
 }
 
 static fot_status fot_gps_order_impl(const uint32_t *ci, const uint32_t *ce,

@@ -11,7 +11,8 @@
 `write_airfoil_dir` / `read_airfoil_dir` are res_calc.cu's writeAllData /
 initProblem + readData (res_calc.cu:10-98); `reorder_tool` is mainReorder
 (res_calc.cu:376-405): a GPS-ordered copy and a GPS-ordered, partitioned and
-renumbered copy of an airfoil directory.
+renumbered copy of an airfoil directory; `rcm_tool` is scripts/rcm.py (an
+edge list reordered by reverse Cuthill-McKee, edges sorted).
 """
 from __future__ import annotations
 
@@ -153,6 +154,42 @@ def reorder_dir(in_dir: str, out_dir: str, partition: bool, block_size: int) -> 
     return p
 
 
+def rcm_edge_list(edges: np.ndarray, num_points: int):
+    """scripts/rcm.py's transform (rcm.py:16-25) on an edge list: reverse
+    Cuthill-McKee of the directed edge CSR (symmetrised), each edge's ids
+    renumbered, the edges sorted lexicographically.  Returns (perm old -> new,
+    new edges, (bandwidth, mean bandwidth) before, (the same) after)."""
+    from . import plan
+    e = np.asarray(edges, dtype=np.int64)
+    order = np.lexsort((e[:, 1], e[:, 0]))
+    ind = np.zeros(num_points + 1, dtype=np.int64)
+    np.add.at(ind, e[:, 0] + 1, 1)
+    ind = np.cumsum(ind)
+    perm = plan.rcm_order(ind, e[order, 1], None)
+    out = perm[e].astype(np.int64)
+    out = out[np.lexsort((out[:, 1], out[:, 0]))]
+    bw = lambda a: (int(np.max(np.abs(a[:, 0] - a[:, 1]))) if len(a) else 0,  # noqa: E731
+                    float(np.mean(np.abs(a[:, 0] - a[:, 1]))) if len(a) else 0.0)
+    return perm, out, bw(e), bw(out)
+
+
+def rcm_tool(argv=None) -> int:
+    """rcm.py's command line: <in_file> <out_file>, the mapping text format
+    with MeshDim 2 in and out; prints the bandwidths as rcm.py does."""
+    argv = sys.argv[1:] if argv is None else argv
+    if len(argv) != 2:
+        print("Usage: python -m cuda_fot_amd.io rcm <in_file> <out_file>", file=sys.stderr)
+        return 1
+    edges, npts = read_mapping(argv[0], 2)
+    _, out, before, after = rcm_edge_list(edges, npts)
+    print("Original bandwidth:", before[0])
+    print("Original mean bandwidth:", before[1])
+    print("RCM'd bandwidth:", after[0])
+    print("RCM'd mean bandwidth:", after[1])
+    write_mapping(argv[1], out, npts)
+    return 0
+
+
 def reorder_tool(argv=None) -> int:
     """mainReorder (res_calc.cu:391-405):
     python -m cuda_fot_amd.io <input_dir> <output_dir_GPS> <output_dir_part> <block_size>"""
@@ -167,4 +204,6 @@ def reorder_tool(argv=None) -> int:
 
 
 if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "rcm":
+        sys.exit(rcm_tool(sys.argv[2:]))
     sys.exit(reorder_tool())

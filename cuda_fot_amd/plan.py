@@ -183,10 +183,15 @@ def kway_partition(ind, end, max_part: int) -> tuple[np.ndarray, int]:
     return part, npart.value
 
 
-def rcm_order(ind, end) -> np.ndarray:
+def rcm_order(ind, end, seed_order=None) -> np.ndarray:
+    """scipy's reverse Cuthill-McKee of the CSR pattern (symmetrised), as a
+    permutation old -> new (scripts/rcm.py:20-21).  seed_order: the degree
+    order that seeds the components (scipy uses numpy's argsort of the
+    degrees); None: ascending degree, ties to the lower id."""
     ind, end = u32(ind), u32(end)
+    seeds = None if seed_order is None else u32(seed_order)
     perm = np.empty(ind.size - 1, dtype=np.uint32)
-    call("fot_rcm_order", ptr(ind), ptr(end), ind.size - 1, ptr(perm))
+    call("fot_rcm_order", ptr(ind), ptr(end), ind.size - 1, ptr(seeds), ptr(perm))
     return perm
 
 

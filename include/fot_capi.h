@@ -368,11 +368,19 @@ fot_status fot_kway_partition(const uint32_t *csr_indices,
                               uint32_t num_vertices, uint32_t max_part,
                               uint32_t *part, uint32_t *num_parts);
 
-/* Bandwidth-reducing point orderings of a CSR graph, permutation old->new.
- * Stand-ins for Scotch strategy "g" (reorder.hpp:135-175, absent here) and
- * scipy's reverse_cuthill_mckee (scripts/rcm.py:16-37). */
+/* scipy.sparse.csgraph.reverse_cuthill_mckee of the CSR matrix's pattern,
+ * the ordering scripts/rcm.py:16-25 applies to an edge list: perm old->new
+ * (rcm.py's np.argsort of scipy's result).  The pattern is symmetrised
+ * (A + A^T); degrees count a diagonal entry twice; components are seeded in
+ * seed_order (scipy: numpy's argsort of the degrees, whose order among equal
+ * degrees depends on the host's numpy; NULL: ascending degree, ties to the
+ * lower id).  Given the same seed order the permutation is scipy's. */
 fot_status fot_rcm_order(const uint32_t *csr_indices, const uint32_t *csr_endpoints,
-                         uint32_t num_vertices, uint32_t *perm);
+                         uint32_t num_vertices, const uint32_t *seed_order /* nullable */,
+                         uint32_t *perm);
+/* Bandwidth-reducing level-structure numbering from a pseudo-peripheral
+ * vertex, permutation old->new: the stand-in for Scotch strategy "g"
+ * (reorder.hpp:135-175, Scotch is absent here). */
 fot_status fot_gps_order(const uint32_t *csr_indices, const uint32_t *csr_endpoints,
                          uint32_t num_vertices, uint32_t *perm);
 

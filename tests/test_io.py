@@ -67,3 +67,25 @@ def test_reorder_tool_keeps_the_loop_result(tmp_path):
         assert_close_scaled(oq.target[perm], o.target, S, what=f"reordered copy {d}")
     assert (tmp_path / "part" / "mesh_part").exists()
     assert not (tmp_path / "gps" / "mesh_part").exists()
+
+
+def test_rcm_tool_follows_rcm_py(tmp_path, capsys):
+    """scripts/rcm.py:16-45 on a shuffled airfoil's edge list: the edges come
+    back renumbered by reverse Cuthill-McKee of the directed edge CSR and
+    sorted lexicographically; the printed bandwidth falls."""
+    from cuda_fot_amd import plan
+    m = meshgen.airfoil(30, 20, seed=2, shuffle=7)
+    e, n = m["pedge"].astype(np.int64), m["num_nodes"]
+    io.write_mapping(tmp_path / "in", e, n)
+    assert io.rcm_tool([str(tmp_path / "in"), str(tmp_path / "out")]) == 0
+    out, n2 = io.read_mapping(tmp_path / "out", 2)
+    assert n2 == n
+    order = np.lexsort((e[:, 1], e[:, 0]))
+    ind = np.concatenate([[0], np.cumsum(np.bincount(e[:, 0], minlength=n))])
+    perm = plan.rcm_order(ind, e[order, 1])
+    want = perm[e].astype(np.int64)
+    want = want[np.lexsort((want[:, 1], want[:, 0]))]
+    assert np.array_equal(out, want)
+    printed = capsys.readouterr().out.split("\n")
+    before, after = int(printed[0].split()[-1]), int(printed[2].split()[-1])
+    assert after < before / 10

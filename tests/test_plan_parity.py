@@ -189,27 +189,59 @@ def _csr_bandwidth(ind, end, perm):
     return int(np.max(np.abs(perm[rows].astype(np.int64) - perm[end].astype(np.int64))))
 
 
-def test_rcm_against_scipy():
-    # scripts/rcm.py orders with scipy's reverse_cuthill_mckee; the native
-    # ordering must reduce bandwidth as well as scipy's does on a shuffled grid
-    p = mine_problem(N=40, M=30, seed=13)
-    shuffled_ids = meshgen.permutation(p.num_points[0], 13)[p.maps[0]]
-    ind, end = plan.graph_csr(shuffled_ids, p.num_points[0])
-    n = ind.size - 1
-    perm = plan.rcm_order(ind, end)
-    assert np.array_equal(np.sort(perm), np.arange(n))
-    g = sp.csr_matrix((np.ones(end.size), end, ind), shape=(n, n))
-    order = reverse_cuthill_mckee(g, symmetric_mode=True)
-    sperm = np.empty(n, dtype=np.int64)
-    sperm[order] = np.arange(n)
-    shuffled = _csr_bandwidth(ind, end, np.arange(n))
-    ours, theirs = _csr_bandwidth(ind, end, perm), _csr_bandwidth(ind, end, sperm)
-    assert ours <= 1.25 * theirs and ours < shuffled / 10
-    # a path graph has one RCM order up to reversal: identical bandwidth 1
+def _scipy_rcm(edges, n):
+    """scripts/rcm.py:16-21: the directed CSR of an edge list, scipy's
+    reverse_cuthill_mckee (which symmetrises it), then argsort: old -> new.
+    Also returns the CSR and the seed order scipy's routine used: numpy's
+    argsort of its int32 degrees (row length of A + A^T, the diagonal
+    counted twice), recomputed here the same way on the same numpy."""
+    A = sp.csr_matrix((np.ones(edges.shape[0]), (edges[:, 0], edges[:, 1])), shape=(n, n))
+    perm = np.argsort(reverse_cuthill_mckee(A))
+    S = (A + A.T).tocsr()
+    deg = np.diff(S.indptr).astype(np.int32)
+    rows = np.repeat(np.arange(n), np.diff(S.indptr))
+    deg[np.unique(rows[S.indices == rows])] += 1
+    return perm, A.indptr.astype(np.uint32), A.indices.astype(np.uint32), np.argsort(deg), deg
+
+
+def _rcm_graphs():
+    m = meshgen.airfoil(40, 24, seed=3, shuffle=7)
+    yield "shuffled airfoil nodes", m["pedge"], m["num_nodes"]
+    g = meshgen.grid_edges(30, 20)
+    yield "shuffled grid", g[meshgen.permutation(g.shape[0], 3)], 600
+    # three components (a shuffled grid, a path, a star), two isolated
+    # points, a self loop, an edge given in both directions and a duplicate
+    h = meshgen.permutation(200, 4)[meshgen.grid_edges(20, 10)]
+    path = np.array([[200 + i, 201 + i] for i in range(9)])
+    star = np.array([[210, 211 + i] for i in range(6)])
+    extra = np.array([[205, 205], [3, 7], [7, 3], [path[0, 0], path[0, 1]]])
+    yield "disconnected", np.concatenate([h, path, star, extra]), 219
+
+
+RCM_GRAPHS = list(_rcm_graphs())
+
+
+@pytest.mark.parametrize("name,edges,n", RCM_GRAPHS, ids=[g[0] for g in RCM_GRAPHS])
+def test_rcm_is_scipys(name, edges, n):
+    """fot_rcm_order is scipy's reverse Cuthill-McKee on rcm.py's input
+    convention: given the seed order scipy's routine used, the permutation
+    is identical; with none it seeds by a stable degree sort."""
+    want, ind, end, seeds, deg = _scipy_rcm(np.asarray(edges, dtype=np.int64), n)
+    assert np.array_equal(plan.rcm_order(ind, end, seeds), want), name
+    stable = plan.rcm_order(ind, end)
+    assert np.array_equal(stable, plan.rcm_order(ind, end, np.argsort(deg, kind="stable")))
+    assert np.array_equal(np.sort(stable), np.arange(n))
+
+
+def test_rcm_and_gps_on_a_path():
+    # a path graph has one RCM order up to reversal: bandwidth 1
     path = np.array([[i, i + 1] for i in range(9)])
     pi, pe = plan.graph_csr(path, 10)
     assert _csr_bandwidth(pi, pe, plan.rcm_order(pi, pe)) == 1
     assert _csr_bandwidth(pi, pe, plan.gps_order(pi, pe)) == 1
+    # the stable seed is the lower-id end (degree 1 ties 0 and 9); reversal
+    # numbers it last
+    assert plan.rcm_order(pi, pe).tolist() == [9 - i for i in range(10)]
 
 
 def test_owner_plan_invariants():
