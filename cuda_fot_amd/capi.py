@@ -254,6 +254,7 @@ _PROTOS = {
     "fot_mlp_get_params": [P, P, U64],
     "fot_mlp_set_params": [P, P, U64],
     "fot_mlp_forward": [P, P, U32, P],
+    "fot_mlp_forward_noalloc": [P, P, U32, P, P],
     "fot_mlp_loss_grad": [P, P, P, P, U32, P, P],
     "fot_mlp_train": [P, P, P, P, U32, U32, F32, F32, P],
     "fot_gemm_f32": [P, P, P, U32, U32, U32, C.c_int, C.c_int, P],

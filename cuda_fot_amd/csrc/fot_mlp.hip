@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "fot_hip.hpp"
@@ -161,6 +162,7 @@ struct SplitMix {
 // parameter layout (flat, row-major): W1 [H][I], b1 [H], W2 [H][H], b2 [H],
 // W3 [O][H], b3 [O]
 struct fot_mlp {
+  int device = 0; // the device holding the parameters
   uint32_t I = 0, H = 0, O = 0;
   size_t nparam = 0;
   size_t off_w1 = 0, off_b1 = 0, off_w2 = 0, off_b2 = 0, off_w3 = 0, off_b3 = 0;
@@ -201,15 +203,18 @@ struct fot_mlp {
     cap = rows;
     return FOT_OK;
   }
-  // forward over `rows` rows of X (device)
-  fot_status forward(uint32_t rows, hipStream_t s) {
+  // forward over `rows` rows of x (device) through the hidden activations
+  // h1, h2 (rows x H each) into y
+  fot_status forward_into(const float *x, float *h1, float *h2, float *y, uint32_t rows,
+                          hipStream_t s) const {
     fot_status st;
-    if ((st = gemm(X, p + off_w1, H1, p + off_b1, rows, H, I, I, 1, 1, I, H, true, false, s)))
+    if ((st = gemm(x, p + off_w1, h1, p + off_b1, rows, H, I, I, 1, 1, I, H, true, false, s)))
       return st;
-    if ((st = gemm(H1, p + off_w2, H2, p + off_b2, rows, H, H, H, 1, 1, H, H, true, false, s)))
+    if ((st = gemm(h1, p + off_w2, h2, p + off_b2, rows, H, H, H, 1, 1, H, H, true, false, s)))
       return st;
-    return gemm(H2, p + off_w3, Y, p + off_b3, rows, O, H, H, 1, 1, H, O, false, false, s);
+    return gemm(h2, p + off_w3, y, p + off_b3, rows, O, H, H, 1, 1, H, O, false, false, s);
   }
+  fot_status forward(uint32_t rows, hipStream_t s) { return forward_into(X, H1, H2, Y, rows, s); }
   // gradients of the masked MSE into g; returns the loss
   fot_status backward(uint32_t rows, bool masked, float count, float *loss, hipStream_t s) {
     const uint32_t n = rows * O, nb = cdiv(n, 256);
@@ -270,6 +275,7 @@ static fot_status fot_mlp_create_impl(uint32_t in, uint32_t hidden, uint32_t out
     return fail(FOT_ERR_INVALID_ARGUMENT, "bad MLP shape");
   *mlp = nullptr;
   std::unique_ptr<fot_mlp> M(new fot_mlp);
+  FOT_HIP(hipGetDevice(&M->device));
   M->I = in, M->H = hidden, M->O = out;
   M->off_w1 = 0;
   M->off_b1 = M->off_w1 + size_t(hidden) * in;
@@ -354,6 +360,70 @@ static fot_status fot_mlp_forward_impl(fot_mlp *mlp, const float *x, uint32_t ro
 }
 extern "C" fot_status fot_mlp_forward(fot_mlp *mlp, const float *x, uint32_t rows, float *y) {
   return fot::guarded("fot_mlp_forward", [&] { return fot_mlp_forward_impl(mlp, x, rows, y); });
+}
+
+// ---- inference that allocates nothing ------------------------------------
+// The two hidden activations of one fot_mlp_forward_noalloc call live in a
+// fixed device array instead of buffers the library allocates: each device
+// holds its own copy from first use until the process ends.  Host code never
+// uses the array's name as an address (on the host it is a stand-in, not
+// device memory); it uses the address hipGetSymbolAddress returns for the
+// current device, looked up once per device on the entry point's first call
+// there.
+constexpr size_t kNoAllocScratchFloats = size_t(1) << 20; // 4 MiB per device
+static __device__ __align__(256) float scratch[kNoAllocScratchFloats];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static float *g_scratch_ptr[kMaxScratchDevices] = {};
+
+static fot_status scratch_of_current_device(float **out) {
+  int dev = 0;
+  FOT_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxScratchDevices)
+    return fail(FOT_ERR_UNSUPPORTED, "device %d beyond the %d scratch slots", dev,
+                kMaxScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *q = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !q)
+      return fail(FOT_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<float *>(q);
+  }
+  *out = g_scratch_ptr[dev];
+  return FOT_OK;
+}
+
+static fot_status fot_mlp_forward_noalloc_impl(const fot_mlp *mlp, const float *x,
+                                              uint32_t rows, float *y, void *stream) {
+  if (!mlp) return fail(FOT_ERR_INVALID_ARGUMENT, "null MLP");
+  if (rows == 0) return FOT_OK;
+  if (!x || !y) return fail(FOT_ERR_INVALID_ARGUMENT, "null device buffer");
+  if (2 * uint64_t(rows) * mlp->H > kNoAllocScratchFloats)
+    return fail(FOT_ERR_INVALID_ARGUMENT,
+                "rows %u x hidden %u needs %llu activation floats; the scratch holds %zu "
+                "(at most %llu rows)",
+                rows, mlp->H, (unsigned long long)(2 * uint64_t(rows) * mlp->H),
+                kNoAllocScratchFloats,
+                (unsigned long long)(kNoAllocScratchFloats / (2 * uint64_t(mlp->H))));
+  int dev = 0;
+  FOT_HIP(hipGetDevice(&dev));
+  if (dev != mlp->device)
+    return fail(FOT_ERR_INVALID_ARGUMENT, "current device %d, the MLP lives on device %d", dev,
+                mlp->device);
+  float *h = nullptr;
+  if (fot_status st = scratch_of_current_device(&h)) return st;
+  // every element of both activations is written before it is read: no
+  // clearing needed
+  return mlp->forward_into(x, h, h + size_t(rows) * mlp->H, y, rows,
+                           static_cast<hipStream_t>(stream));
+}
+extern "C" fot_status fot_mlp_forward_noalloc(const fot_mlp *mlp, const float *x, uint32_t rows,
+                                              float *y, void *stream) {
+  return fot::guarded("fot_mlp_forward_noalloc", [&] {
+    return fot_mlp_forward_noalloc_impl(mlp, x, rows, y, stream);
+  });
 }
 
 static fot_status fot_mlp_loss_grad_impl(fot_mlp *mlp, const float *x, const float *t,

@@ -670,6 +670,19 @@ fot_status fot_mlp_get_params(const fot_mlp *mlp, float *params, uint64_t n);
 fot_status fot_mlp_set_params(fot_mlp *mlp, const float *params, uint64_t n);
 /* y[rows][out] = MLP(x[rows][in]) */
 fot_status fot_mlp_forward(fot_mlp *mlp, const float *x, uint32_t rows, float *y);
+/* fot_mlp_forward on DEVICE buffers x[rows][in] and y[rows][out], enqueued on
+ * `stream` (returns without waiting), allocating nothing: no hipMalloc, no
+ * hipFree, no allocating library call.  No reference counterpart; it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid it).  The two hidden layers'
+ * activations go to a fixed 4 MiB device array (one per device, held until
+ * the process ends), so 2 * rows * hidden <= 1048576 (rows <= 8192 at
+ * hidden 64); larger calls return FOT_ERR_INVALID_ARGUMENT.  The current
+ * device must be the one the MLP was created on.  Every call on a device
+ * shares that device's array: calls that overlap on one device (different
+ * streams, or host threads) race; calls on one stream are ordered. */
+fot_status fot_mlp_forward_noalloc(const fot_mlp *mlp, const float *x, uint32_t rows, float *y,
+                                   void *stream);
 /* loss = sum(mask (y - t)^2) / sum(mask) (mask NULL: all ones) and its
  * gradient with respect to the flat parameters */
 fot_status fot_mlp_loss_grad(fot_mlp *mlp, const float *x, const float *t, const float *mask,
