@@ -84,10 +84,11 @@ def parse():
     ap.add_argument("--nodes", default="gather", choices=["gather", "stage"],
                     help="owner compute: gather node coordinates per edge, or stage each "
                          "block's nodes in LDS first")
-    ap.add_argument("--order", default="owner", choices=["owner", "dep", "launch"],
+    ap.add_argument("--order", default="owner", choices=["owner", "dep", "launch", "two-phase"],
                     help="hierarchical block scheduling: owner compute in one launch (owner), "
-                         "block colours ordered by completion flags in one launch (dep) or "
-                         "one launch per block colour (launch)")
+                         "block colours ordered by completion flags in one launch (dep), "
+                         "one launch per block colour (launch) or every block in one launch "
+                         "with shared points merged by a second (two-phase)")
     ap.add_argument("--graph", action="store_true", help="replay each step from a hipGraph")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -130,7 +131,8 @@ def make_loop(p, strategy, block_size, stage, graph, order="launch", ept=1, gath
     dev = [DeviceArray.from_host(d) for d in p.point_data]
     out = DeviceArray.from_host(p.point_data[0])
     flags = (capi.LOOP_STAGE_INPUTS if stage else 0) | \
-        {"launch": 0, "dep": capi.LOOP_DEPENDENCY_ORDER, "owner": capi.LOOP_OWNER_COMPUTE}[order]
+        {"launch": 0, "dep": capi.LOOP_DEPENDENCY_ORDER, "owner": capi.LOOP_OWNER_COMPUTE,
+         "two-phase": capi.LOOP_TWO_PHASE}[order]
     if ept == 2:
         flags |= capi.LOOP_TWO_PER_THREAD
     if gather:
@@ -576,7 +578,8 @@ def bench_single(args, workload):
         dev = [DeviceArray.from_host(d) for d in prob.point_data]
         out = DeviceArray.from_host(prob.point_data[0])
         cdev = [DeviceArray.from_host(c) for c in prob.cell_data]
-        part = prob.point_partition if flags & capi.LOOP_OWNER_COMPUTE else None
+        part = prob.point_partition if flags & capi.LOOP_OWNER_COMPUTE else (
+            prob.partition_vector if strategy == capi.STRATEGY_HIERARCHICAL else None)
         if part is not None:
             flags |= capi.LOOP_POINT_PARTITION
         l = Loop(f, prob.maps, prob.num_points, dev, out, cdev, soa=prob.soa, strategy=strategy,
@@ -608,15 +611,31 @@ def bench_single(args, workload):
                 row["hierarchical_error"] = str(e)
             sweep[str(bs)] = row
         del g
-    if args.partition == "gps":
-        # the reference-shaped hierarchical loop (one launch per block
-        # colour) on the GPS-ordered mesh, consecutive blocks of
-        # --block-size cells: the autotuner's pick on these meshes
+    if args.partition in ("gps", "kway"):
+        # the hierarchical loop on the GPS-ordered mesh: consecutive blocks
+        # of --block-size cells (gps), or k-way parts of --block-size cells
+        # with the mesh reordered to them and the points renumbered (kway);
+        # one launch per block colour (--order launch), the two-phase
+        # writeback (two-phase) or the dependency-ordered launch (dep)
         p.reorder("gps")
-        wall, ev, st = run(p, capi.STRATEGY_HIERARCHICAL, 0, args.block_size, with_stats=True)
-        variant = f"gps_b{args.block_size}"
-        cfg = {"strategy": "hierarchical", "partition": "gps", "block_size": args.block_size,
-               "launches_per_step": st["num_launches_per_iter"]}
+        if args.partition == "kway":
+            p.block_size = args.block_size
+            p.partition(1.001)
+            p.reorder_to_partition()
+            p.renumber_points()
+        order = args.order if args.order != "owner" else "launch"
+        flags = {"launch": 0, "dep": capi.LOOP_DEPENDENCY_ORDER,
+                 "two-phase": capi.LOOP_TWO_PHASE}[order]
+        wall, ev, st = run(p, capi.STRATEGY_HIERARCHICAL, flags, args.block_size,
+                           with_stats=True)
+        variant = f"{args.partition}_b{args.block_size}" + \
+            ("" if order == "launch" else "_" + order)
+        cfg = {"strategy": "hierarchical", "partition": args.partition,
+               "block_size": args.block_size, "block_order": order,
+               "launches_per_step": st["num_launches_per_iter"],
+               "block_colours": st["num_colours"], "blocks": st["num_blocks"],
+               "avg_thread_colours": round(st["avg_thread_colours"], 3),
+               "reuse_factor": round(st["reuse_factor"], 4)}
     else:
         p.partition_points(args.points_per_block, args.numbering)
         wall, ev, st = run(p, capi.STRATEGY_HIERARCHICAL, capi.LOOP_OWNER_COMPUTE,

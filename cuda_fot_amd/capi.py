@@ -59,6 +59,7 @@ LOOP_POINT_TERMS = 64
 LOOP_OWNER_GATHER = 128
 LOOP_PERSISTENT = 256
 LOOP_LISTS_IN_HBM = 512
+LOOP_TWO_PHASE = 1024
 
 
 class FotError(RuntimeError):

@@ -41,7 +41,55 @@ struct HierDev {
   const uint8_t *blk_ncol;       // [nb] thread colours per block
   const uint32_t *blk_owned;     // [nb] owned points (owner-compute plans only)
   const uint32_t *packed;        // res_calc owner compute: l0 | l1 << 12 | colour << 24
+  // two-phase writeback: per cached entry its partial slot, or kExclusive
+  // when no other block touches the point; partial sums [slot][TD]
+  const uint32_t *wslot;
+  double *partial;
 };
+
+constexpr uint32_t kExclusive = 0xffffffffu;
+
+// Two-phase writeback of a block's cache entry i (point g, values v): added
+// in place when this block alone touches g, else stored in its partial slot
+// for the merge launch.
+template <uint32_t TD, bool SOA>
+FOT_DEV void writeback_two_phase(const HierDev &h, double *out, uint32_t np, uint32_t entry,
+                                 uint32_t g, const double *v) {
+  const uint32_t s = h.wslot[entry];
+  if (s == kExclusive) {
+#pragma unroll
+    for (uint32_t d = 0; d < TD; ++d) {
+      double *p = out + pidx<SOA>(np, g, TD, d);
+      *p = *p + v[d];
+    }
+  } else {
+#pragma unroll
+    for (uint32_t d = 0; d < TD; ++d) h.partial[size_t(s) * TD + d] = v[d];
+  }
+}
+
+// Merge launch of the two-phase writeback: point pt[j] gets its partials
+// slots[off[j]] .. slots[off[j+1]-1], which are in block-colour order, added
+// one by one, as the per-colour launches add the blocks' caches.
+template <uint32_t TD, bool SOA>
+__global__ __launch_bounds__(256) void merge_partials_kernel(
+    double *__restrict__ out, uint32_t np, const uint32_t *__restrict__ pt,
+    const uint32_t *__restrict__ off, const uint32_t *__restrict__ slots,
+    const double *__restrict__ partial, uint32_t n) {
+  const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t g = pt[j], s0 = off[j], s1 = off[j + 1];
+  double v[TD];
+#pragma unroll
+  for (uint32_t d = 0; d < TD; ++d) v[d] = out[pidx<SOA>(np, g, TD, d)];
+  for (uint32_t s = s0; s < s1; ++s) {
+    const double *q = partial + size_t(slots[s]) * TD;
+#pragma unroll
+    for (uint32_t d = 0; d < TD; ++d) v[d] = v[d] + q[d];
+  }
+#pragma unroll
+  for (uint32_t d = 0; d < TD; ++d) out[pidx<SOA>(np, g, TD, d)] = v[d];
+}
 
 template <uint32_t MD>
 FOT_DEV void load_loc(const uint16_t *__restrict__ p, uint32_t *loc) {
@@ -204,9 +252,10 @@ FOT_DEV void publish(const DepDev &dd, uint32_t b, uint32_t gen) {
   }
 }
 
-// MODE: 0 one launch per block colour, 1 dependency order.  LB: the most
-// threads a launch may use (a lower bound leaves the compiler more registers
-// per thread).
+// MODE: 0 one launch per block colour, 1 dependency order, 2 two-phase
+// writeback (every block in one launch; merge_partials_kernel after).  LB:
+// the most threads a launch may use (a lower bound leaves the compiler more
+// registers per thread).
 template <class F, bool SOA, int MODE, int LB = 1024>
 __global__ __launch_bounds__(LB) void eng_hier_kernel(EngineArgs a, HierDev h, DepDev dd,
                                                         uint32_t blk_begin, uint32_t nbc) {
@@ -239,10 +288,17 @@ __global__ __launch_bounds__(LB) void eng_hier_kernel(EngineArgs a, HierDev h, D
   const uint32_t np = a.num_points[0];
   for (uint32_t i = tid; i < m; i += blockDim.x) {
     const uint32_t g = h.cached[c0 + i];
+    if constexpr (MODE == 2) {
+      double v[F::TD];
 #pragma unroll
-    for (uint32_t d = 0; d < F::TD; ++d) {
-      double *p = a.out + pidx<SOA>(np, g, F::TD, d);
-      *p = *p + cache[d * m + i];
+      for (uint32_t d = 0; d < F::TD; ++d) v[d] = cache[d * m + i];
+      writeback_two_phase<F::TD, SOA>(h, a.out, np, c0 + i, g, v);
+    } else {
+#pragma unroll
+      for (uint32_t d = 0; d < F::TD; ++d) {
+        double *p = a.out + pidx<SOA>(np, g, F::TD, d);
+        *p = *p + cache[d * m + i];
+      }
     }
   }
   publish<DEP>(dd, b, gen);
@@ -329,6 +385,16 @@ __global__ __launch_bounds__(1024) void res_calc_hier_staged_kernel(EngineArgs a
     __syncthreads();
   }
   wait_predecessors<DEP>(dd, b, gen);
+  if constexpr (MODE == 2) {
+    for (uint32_t i = tid; i < m; i += bd) {
+      const uint32_t g = i == tid ? g0 : h.cached[c0 + i];
+      double v[4];
+#pragma unroll
+      for (uint32_t d = 0; d < 4; ++d) v[d] = res_s[d * m + i];
+      writeback_two_phase<4, SOA>(h, a.out, np, c0 + i, g, v);
+    }
+    return;
+  }
   for (uint32_t i = tid; i < m; i += bd) {
     const uint32_t g = i == tid ? g0 : h.cached[c0 + i];
     double r[4];
@@ -1073,6 +1139,11 @@ struct fot_loop {
   bool staged = false;
   bool dep_order = false;
   bool owner = false;
+  // two-phase writeback: the merge launch's shared points, their partial
+  // slot lists (CSR) and the slots
+  bool two_phase = false;
+  uint32_t merge_n = 0;
+  const uint32_t *merge_pt = nullptr, *merge_off = nullptr, *merge_slots = nullptr;
   const OwnerMeta *owner_meta = nullptr;
   const uint32_t *owner_nodes = nullptr, *owner_nloc = nullptr;
   bool stage_x = false;
@@ -1290,6 +1361,11 @@ fot_status fot_loop::build_hierarchical() {
   owner = (cfg.flags & FOT_LOOP_OWNER_COMPUTE) != 0;
   if (owner && (cfg.flags & FOT_LOOP_DEPENDENCY_ORDER))
     return fail(FOT_ERR_INVALID_ARGUMENT, "owner compute needs no dependency order");
+  two_phase = (cfg.flags & FOT_LOOP_TWO_PHASE) != 0;
+  if (two_phase && (owner || (cfg.flags & FOT_LOOP_DEPENDENCY_ORDER)))
+    return fail(FOT_ERR_INVALID_ARGUMENT,
+                "the two-phase writeback is a block-colour plan's (no owner compute, no "
+                "dependency order)");
   if (f.kernel == FOT_KERNEL_RES_CALC &&
       (owner || (cfg.flags & FOT_LOOP_STAGE_INPUTS))) {
     // these kernels stage q and adt through mapping-0 ids; res_calc reads its
@@ -1532,6 +1608,47 @@ fot_status fot_loop::build_hierarchical() {
     plan_args.maps[m] = d32;
   }
   if ((st = gather_cell_data(plan_cell.data(), uint32_t(plan_cell.size()), plan_args)) != FOT_OK) return st;
+  if (two_phase) {
+    // Blocks in plan (colour-major) order; a point's partial slots are
+    // handed out in that order, so ascending slots are ascending block
+    // colours.  Slots of one block are consecutive (its shared points in
+    // cache order): its partial stores are contiguous.
+    std::vector<uint32_t> touch(desc.num_points[0], 0);
+    for (uint32_t i = 0; i < blk_cache_off[nb_total]; ++i) ++touch[cached[i]];
+    std::vector<uint32_t> rank(desc.num_points[0], 0xffffffffu), pt, off{0};
+    for (uint32_t g = 0; g < desc.num_points[0]; ++g)
+      if (touch[g] > 1) {
+        rank[g] = uint32_t(pt.size());
+        pt.push_back(g);
+        off.push_back(off.back() + touch[g]);
+      }
+    std::vector<uint32_t> wslot(blk_cache_off[nb_total]), slots(off.back()), fill(pt.size(), 0);
+    uint32_t next = 0;
+    for (uint32_t i = 0; i < blk_cache_off[nb_total]; ++i) {
+      const uint32_t r = rank[cached[i]];
+      if (r == 0xffffffffu) {
+        wslot[i] = kExclusive;
+      } else {
+        wslot[i] = next;
+        slots[off[r] + fill[r]++] = next++;
+      }
+    }
+    merge_n = uint32_t(pt.size());
+    if ((st = upload(wslot.data(), wslot.size(), &d32)) != FOT_OK) return st;
+    hier.wslot = d32;
+    if (merge_n) {
+      if ((st = upload(pt.data(), pt.size(), &d32)) != FOT_OK) return st;
+      merge_pt = d32;
+      if ((st = upload(off.data(), off.size(), &d32)) != FOT_OK) return st;
+      merge_off = d32;
+      if ((st = upload(slots.data(), slots.size(), &d32)) != FOT_OK) return st;
+      merge_slots = d32;
+      void *pp = nullptr;
+      if ((st = alloc(sizeof(double) * size_t(next) * desc.point_dim[0], &pp)) != FOT_OK)
+        return st;
+      hier.partial = static_cast<double *>(pp);
+    }
+  }
   if (cfg.flags & FOT_LOOP_DEPENDENCY_ORDER) {
     // immediate lower-colour neighbour of each block through each point: in
     // colour-major plan order the last block seen on a point is it
@@ -1626,7 +1743,7 @@ fot_status fot_loop::build_hierarchical() {
     for (uint32_t nw : blk_owned) most_owned = std::max(most_owned, nw);
     owned_first_round = most_owned <= threads;
   }
-  launch_grid = (dep_order || owner) ? nb_total : 0;
+  launch_grid = (dep_order || owner || two_phase) ? nb_total : 0;
   if (owner && f.kernel == FOT_KERNEL_RES_CALC && owner_gather) {
     const bool persistent = (cfg.flags & FOT_LOOP_PERSISTENT) != 0;
     const uint32_t lw = lists_in_lds ? 0u : list_words;
@@ -1649,7 +1766,8 @@ fot_status fot_loop::build_hierarchical() {
   }
   stats.num_colours = num_plan_colours;
   stats.num_blocks = nb_total;
-  stats.num_launches_per_iter = (dep_order || owner) ? 1u : stats.num_colours;
+  stats.num_launches_per_iter =
+      (dep_order || owner) ? 1u : two_phase ? (merge_n ? 2u : 1u) : stats.num_colours;
   stats.max_cached = max_cached;
   stats.lds_bytes = lds_bytes;
   stats.threads_per_block = threads;
@@ -1756,6 +1874,31 @@ fot_status fot_loop::enqueue(hipStream_t s) {
       if (!done)
         hipLaunchKernelGGL((eng_hier_kernel<F, SOA, 1>), dim3(nb), dim3(threads), lds_bytes, s,
                            plan_args, hier, dep, 0u, nb);
+    } else if (two_phase) {
+      // every block in one launch, then the merge of the shared points
+      const uint32_t nb = colour_blk_off.back();
+      bool done = false;
+      if constexpr (std::is_same<F, ResCalcF>::value) {
+        if (staged) {
+          hipLaunchKernelGGL((res_calc_hier_staged_kernel<SOA, 2>), dim3(nb), dim3(threads),
+                             lds_bytes, s, plan_args, hier, dep, 0u, nb);
+          done = true;
+        }
+      }
+      if constexpr (kSmallBound<F> < 1024) {
+        if (!done && threads <= kSmallBound<F>) {
+          hipLaunchKernelGGL((eng_hier_kernel<F, SOA, 2, kSmallBound<F>>), dim3(nb),
+                             dim3(threads), lds_bytes, s, plan_args, hier, dep, 0u, nb);
+          done = true;
+        }
+      }
+      if (!done)
+        hipLaunchKernelGGL((eng_hier_kernel<F, SOA, 2>), dim3(nb), dim3(threads), lds_bytes, s,
+                           plan_args, hier, dep, 0u, nb);
+      if (merge_n)
+        hipLaunchKernelGGL((merge_partials_kernel<F::TD, SOA>), dim3((merge_n + 255) / 256),
+                           dim3(256), 0, s, plan_args.out, plan_args.num_points[0], merge_pt,
+                           merge_off, merge_slots, hier.partial, merge_n);
     } else {
       for (size_t c = 0; c + 1 < colour_blk_off.size(); ++c) {
         const uint32_t b0 = colour_blk_off[c], nbc = colour_blk_off[c + 1] - b0;

@@ -510,6 +510,14 @@ typedef struct {
  * positions (the default when every cell has at most one edge per thread
  * colour); the same bits */
 #define FOT_LOOP_LISTS_IN_HBM 512u
+/* hierarchical (block colours, not owner compute): every block in ONE
+ * launch, none waiting for another.  A point only one block touches gets
+ * that block's sum added in place; a point several blocks touch gets each
+ * block's sum stored in a partial slot, and a second launch adds a point's
+ * partials in block-colour order.  The additions of the per-colour launches
+ * in their order: the same bits.  No reference counterpart (the reference
+ * launches once per block colour, problem.hpp:276-303). */
+#define FOT_LOOP_TWO_PHASE 1024u
 
 typedef struct fot_loop fot_loop;
 /* Builds the plan (host), uploads it, permutes cell data. */

@@ -60,11 +60,15 @@ def test_getacc_gpu_strategies(gpu, soa):
     colour.loop_colour(iters, 1)
     owner = oracle_of(getacc_problem(60, 45, seed=2, soa=soa), O.GETACC)
     O.loop_owner_replay(owner, plan.owner_plan(base.maps[0], base.num_points[0], bs), iters)
+    hier = oracle_of(getacc_problem(60, 45, seed=2, soa=soa), O.GETACC)
+    hier.loop_hier_replay(O.OraclePlan(base.maps, base.num_points[0], bs), iters)
     for strategy, flags, expect in [(capi.STRATEGY_ATOMIC, 0, None),
                                     (capi.STRATEGY_GLOBAL_COLOUR, 0, colour.target),
                                     (capi.STRATEGY_HIERARCHICAL, capi.LOOP_OWNER_COMPUTE,
                                      owner.target),
-                                    (capi.STRATEGY_HIERARCHICAL, 0, None)]:
+                                    (capi.STRATEGY_HIERARCHICAL, 0, hier.target),
+                                    (capi.STRATEGY_HIERARCHICAL, capi.LOOP_TWO_PHASE,
+                                     hier.target)]:
         p = getacc_problem(60, 45, seed=2, block_size=bs, soa=soa)
         p.gpu_loop(GETACC, strategy, iters, flags=flags)
         assert_close_scaled(p.point_data[0], ref.target, S, what=f"getacc {strategy}/{flags}")

@@ -67,6 +67,7 @@ TERMS = capi.LOOP_POINT_TERMS
 OG = capi.LOOP_OWNER_GATHER
 HBM = capi.LOOP_LISTS_IN_HBM
 PERS = capi.LOOP_PERSISTENT
+TP = capi.LOOP_TWO_PHASE
 
 
 def _owner_replay(p, kernel, block_size, partition=None, iters=ITERS, point_partition=None):
@@ -159,7 +160,9 @@ def test_res_calc_owner_engine_partition(gpu):
 @pytest.mark.parametrize("soa,bs,flags", [(False, 128, 0), (True, 128, 0),
                                           (False, 256, STAGE), (True, 64, STAGE),
                                           (False, 1024, STAGE), (False, 128, DEP),
-                                          (True, 256, DEP | STAGE), (False, 512, DEP | STAGE)])
+                                          (True, 256, DEP | STAGE), (False, 512, DEP | STAGE),
+                                          (False, 128, TP), (True, 64, TP),
+                                          (True, 256, TP | STAGE), (False, 1024, TP | STAGE)])
 def test_res_calc_hierarchical_bitwise(gpu, soa, bs, flags):
     p = airfoil_problem(soa=soa, res0_seed=5, block_size=bs)
     ref = _replay(p, O.RES_CALC, bs)
@@ -170,7 +173,8 @@ def test_res_calc_hierarchical_bitwise(gpu, soa, bs, flags):
 
 
 @pytest.mark.parametrize("method,flags", [("kway", STAGE), ("ours", STAGE),
-                                          ("kway", STAGE | DEP), ("ours", DEP)])
+                                          ("kway", STAGE | DEP), ("ours", DEP),
+                                          ("kway", STAGE | TP), ("ours", TP)])
 def test_res_calc_partitioned_renumbered(gpu, method, flags):
     p = airfoil_problem(nx=64, ny=32, res0_seed=6, block_size=256)
     orig = airfoil_problem(nx=64, ny=32, res0_seed=6, block_size=256)
@@ -189,7 +193,7 @@ def test_res_calc_partitioned_renumbered(gpu, method, flags):
     assert_close_scaled(p.target_in_original_order(), serial, S, what="reordered vs original")
 
 
-@pytest.mark.parametrize("flags", [STAGE, STAGE | DEP])
+@pytest.mark.parametrize("flags", [STAGE, STAGE | DEP, STAGE | TP, TP])
 def test_res_calc_graph_replay_matches_eager(gpu, flags):
     # several replays of one captured iteration, host work in between
     p1 = airfoil_problem(res0_seed=7)
@@ -220,7 +224,7 @@ def test_mine_strategies(gpu, P, C, soa):
     g = mine_problem(P=P, C=C, soa=soa)
     g.gpu_loop(f, capi.STRATEGY_GLOBAL_COLOUR, ITERS)
     assert_bitwise(g.point_data[0], ref_colour, "mine global colour")
-    for flags in (0, DEP):
+    for flags in (0, DEP, TP):
         h = mine_problem(P=P, C=C, soa=soa)
         h.gpu_loop(f, capi.STRATEGY_HIERARCHICAL, ITERS, flags=flags)
         assert_bitwise(h.point_data[0], ref_hier, f"mine hierarchical flags {flags}")
@@ -243,6 +247,7 @@ def test_hex_scatter_strategies(gpu, soa):
     for strategy, ref, exact, flags in [(capi.STRATEGY_HIERARCHICAL, ref_hier, True, 0),
                                         (capi.STRATEGY_HIERARCHICAL, ref_own, True, OWN),
                                         (capi.STRATEGY_HIERARCHICAL, ref_hier, True, DEP),
+                                        (capi.STRATEGY_HIERARCHICAL, ref_hier, True, TP),
                                         (capi.STRATEGY_GLOBAL_COLOUR, ref_colour, True, 0),
                                         (capi.STRATEGY_ATOMIC, ref_serial, False, 0)]:
         p = hex_problem(soa=soa)
@@ -363,6 +368,7 @@ def test_empty_mesh_is_a_no_op(gpu):
     base = airfoil_problem(nx=8, ny=6, res0_seed=14)
     for strategy, flags in [(capi.STRATEGY_ATOMIC, 0), (capi.STRATEGY_GLOBAL_COLOUR, 0),
                             (capi.STRATEGY_HIERARCHICAL, 0), (capi.STRATEGY_HIERARCHICAL, DEP),
+                            (capi.STRATEGY_HIERARCHICAL, TP),
                             (capi.STRATEGY_HIERARCHICAL, OWN | TWO | OG)]:
         for graph in (False, True):
             p = airfoil_problem(nx=8, ny=6, res0_seed=14)

@@ -121,6 +121,8 @@ def test_lulesh_gpu_strategies(gpu, soa):
                                     (capi.STRATEGY_HIERARCHICAL, 0, hier.target),
                                     (capi.STRATEGY_HIERARCHICAL, capi.LOOP_DEPENDENCY_ORDER,
                                      hier.target),
+                                    (capi.STRATEGY_HIERARCHICAL, capi.LOOP_TWO_PHASE,
+                                     hier.target),
                                     (capi.STRATEGY_HIERARCHICAL, capi.LOOP_OWNER_COMPUTE,
                                      owner.target)]:
         p = lulesh_problem(14, 12, 11, seed=4, block_size=bs, soa=soa)

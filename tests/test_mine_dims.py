@@ -16,6 +16,7 @@ pytestmark = pytest.mark.gpu
 ITERS = 3
 OWN = capi.LOOP_OWNER_COMPUTE
 DEP = capi.LOOP_DEPENDENCY_ORDER
+TP = capi.LOOP_TWO_PHASE
 ORACLE_KERNEL = {2: O.MINE2, 4: O.MINE4, 8: O.MINE8}
 
 
@@ -60,7 +61,7 @@ def test_mine_dims_all_strategies(gpu, md, P, C, soa):
     g = _problem(md, P, C, soa)
     g.gpu_loop(f, capi.STRATEGY_GLOBAL_COLOUR, ITERS)
     assert_bitwise(g.point_data[0], oc.target, "global colour vs oracle colour loop")
-    for flags in (0, DEP):
+    for flags in (0, DEP, TP):
         # (fot_loop_run reports a dependency wait that ran out as an error)
         h = _problem(md, P, C, soa)
         h.gpu_loop(f, capi.STRATEGY_HIERARCHICAL, ITERS, flags=flags)
@@ -77,13 +78,13 @@ def test_mine_dims_all_strategies(gpu, md, P, C, soa):
 
 STRATEGIES = [(capi.STRATEGY_ATOMIC, 0), (capi.STRATEGY_GLOBAL_COLOUR, 0),
               (capi.STRATEGY_HIERARCHICAL, 0), (capi.STRATEGY_HIERARCHICAL, DEP),
-              (capi.STRATEGY_HIERARCHICAL, OWN)]
+              (capi.STRATEGY_HIERARCHICAL, TP), (capi.STRATEGY_HIERARCHICAL, OWN)]
 
 
 @pytest.mark.parametrize("kat", [kat_mine2, kat_mine4, kat_mine8],
                          ids=["mine2", "mine4", "mine8"])
 @pytest.mark.parametrize("strategy,flags", STRATEGIES,
-                         ids=["atomic", "global", "hier", "hier-dep", "owner"])
+                         ids=["atomic", "global", "hier", "hier-dep", "hier-2phase", "owner"])
 def test_mine_known_answers_on_the_gpu(gpu, kat, strategy, flags):
     md_of = {O.MINE2: 2, O.MINE4: 4, O.MINE8: 8}
 

@@ -116,11 +116,11 @@ def test_mini_aero_gpu_strategies(gpu, soa):
 @pytest.mark.gpu
 @pytest.mark.parametrize("soa", [False, True])
 @pytest.mark.parametrize("bs", [128, 256, 512])
-@pytest.mark.parametrize("flags", [0, capi.LOOP_DEPENDENCY_ORDER],
-                         ids=["per-colour", "dependency-order"])
+@pytest.mark.parametrize("flags", [0, capi.LOOP_DEPENDENCY_ORDER, capi.LOOP_TWO_PHASE],
+                         ids=["per-colour", "dependency-order", "two-phase"])
 def test_mini_aero_hierarchical_bitwise(gpu, soa, bs, flags):
-    """The reference-shaped hierarchical loop, one launch per block colour or
-    one dependency-ordered launch, at block sizes that run the 256-thread
+    """The reference-shaped hierarchical loop, one launch per block colour,
+    one dependency-ordered launch or the two-phase writeback, at block sizes that run the 256-thread
     instantiation (128, 256) and the 1024-thread one (512): bit for bit
     against the oracle's replay of the same plan.  fot_loop_run reports a
     dependency wait that ran out as an error, so a pass also shows that every
