@@ -57,7 +57,7 @@ def parse():
                     choices=["atomic", "global", "hierarchical"])
     ap.add_argument("--block-size", type=int, default=512)
     ap.add_argument("--partition", default="points",
-                    choices=["none", "kway", "ours", "gps", "points"],
+                    choices=["none", "kway", "bisect", "ours", "gps", "points"],
                     help="points: owner partition of mapping-0 points (owner compute)")
     ap.add_argument("--points-per-block", type=int, default=240)
     ap.add_argument("--numbering", default="cm", choices=["id", "cm"],
@@ -611,16 +611,16 @@ def bench_single(args, workload):
                 row["hierarchical_error"] = str(e)
             sweep[str(bs)] = row
         del g
-    if args.partition in ("gps", "kway"):
+    if args.partition in ("gps", "kway", "bisect"):
         # the hierarchical loop on the GPS-ordered mesh: consecutive blocks
         # of --block-size cells (gps), or k-way parts of --block-size cells
         # with the mesh reordered to them and the points renumbered (kway);
         # one launch per block colour (--order launch), the two-phase
         # writeback (two-phase) or the dependency-ordered launch (dep)
         p.reorder("gps")
-        if args.partition == "kway":
+        if args.partition in ("kway", "bisect"):
             p.block_size = args.block_size
-            p.partition(1.001)
+            p.partition(1.001, method=args.partition)
             p.reorder_to_partition()
             p.renumber_points()
         order = args.order if args.order != "owner" else "launch"

@@ -208,6 +208,7 @@ _PROTOS = {
     "fot_cell_to_cell_csr": [P, P, U32, U32, P, P, P, P],
     "fot_heuristic_partition": [P, P, U32, P, U32, P, P],
     "fot_kway_partition": [P, P, U32, U32, P, P],
+    "fot_bisect_partition": [P, P, U32, U32, P, P],
     "fot_rcm_order": [P, P, U32, P, P],
     "fot_gps_order": [P, P, U32, P],
     "fot_gen_grid_edges": [U32, U32, P],

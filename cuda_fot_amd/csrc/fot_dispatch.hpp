@@ -60,12 +60,17 @@ fot_status dispatch(const fot_functor *f, int soa, Body &&body) {
 
 // Launch bound of the instantiations that run a functor's small blocks.
 // Under a 1024-thread bound a kernel may use 128 VGPRs, where mini_aero's
-// face flux spills about 200 B per lane.  Hierarchical blocks of at most this
-// many threads run a kernel bounded at it, which may use more registers, and
-// direct launches (whose results do not depend on the block size) always do.
-// (lulesh's element forces, at the cap too, measured the same at 512.)
+// face flux spills about 200 B per lane; under 512 it may use 256, and the
+// face flux needs about 170-195.  Hierarchical and owner blocks of at most
+// this many threads run a kernel bounded at it, and direct launches (whose
+// results do not depend on the block size) always do.  (lulesh's element
+// forces, at the cap too, measured the same at 512.)
 template <class F>
-constexpr int kSmallBound = std::is_same_v<F, MiniAeroF> ? 256 : 1024;
+constexpr int kSmallBound = std::is_same_v<F, MiniAeroF> ? 512 : 1024;
+// Waves per SIMD the small-bound hierarchical instantiation is compiled for
+// (at most 512 / waves VGPRs each).
+template <class F>
+constexpr int kSmallBoundWaves = std::is_same_v<F, MiniAeroF> ? 3 : 1;
 
 } // namespace fot
 

@@ -192,7 +192,9 @@ struct HexScatterF {
   static constexpr uint32_t MD = 8, TD = 3;
   static constexpr bool SLOT_OUTER = true;
   // sequential cache adds: batching them costs this light kernel occupancy
-  // (69 -> 80..110 VGPRs, 7 -> 6..4 waves per SIMD), measured slower
+  // (69 -> 80..110 VGPRs, 7 -> 6..4 waves per SIMD), measured slower on
+  // GPS-ordered blocks and within 2 % (chunks of 4 and 8) on compact
+  // bisection blocks with 9 thread colours
   static constexpr uint32_t ADD_CHUNK = 1;
   FOT_DEV static void user(const double *F, uint32_t cs, double *inc) {
 #pragma unroll
@@ -536,6 +538,43 @@ struct GetaccF {
   FOT_DEV static void compute(const A &a, uint32_t ind, const uint32_t *, double *inc) {
     user(a.cd(0) + ind, a.cd(1) + ind, a.cd(2) + ind, a.cd(3) + ind, a.cd(4) + ind, a.cs(), inc);
   }
+};
+
+// The point data a functor reads through its cells' mapping-0 ids (`tgt` in
+// compute), which a hierarchical block can stage in LDS with its cache:
+// slot (-1: none) and values per point.
+template <class F> struct StagedInput {
+  static constexpr int slot = -1;
+  static constexpr uint32_t dim = 0;
+};
+template <> struct StagedInput<MiniAeroF> { // the 28 cell values
+  static constexpr int slot = 1;
+  static constexpr uint32_t dim = 28;
+};
+template <> struct StagedInput<LuleshF> { // node coordinates
+  static constexpr int slot = 1;
+  static constexpr uint32_t dim = 3;
+};
+template <> struct StagedInput<VolnaF> { // cell volumes
+  static constexpr int slot = 1;
+  static constexpr uint32_t dim = 1;
+};
+template <uint32_t MDv, uint32_t P, uint32_t C> struct StagedInput<MineF<MDv, P, C>> {
+  static constexpr int slot = 0; // the iterate
+  static constexpr uint32_t dim = P;
+};
+
+// The engine arguments seen by a functor whose staged input is in LDS: point
+// data `SLOT` is the block's staged copy (AOS rows of the staged dim, or SOA
+// with stride m), indexed by block-local ids; everything else as in `a`.
+template <int SLOT> struct LdsArgs {
+  const EngineArgs &a;
+  const double *staged;
+  uint32_t m;
+  FOT_DEV const double *pd(uint32_t k) const { return k == uint32_t(SLOT) ? staged : a.pd(k); }
+  FOT_DEV uint32_t ps(uint32_t k) const { return k == uint32_t(SLOT) ? m : a.ps(k); }
+  FOT_DEV const double *cd(uint32_t k) const { return a.cd(k); }
+  FOT_DEV uint32_t cs() const { return a.cs(); }
 };
 
 // Mapping-0 ids of cell `ind` (AOS row).

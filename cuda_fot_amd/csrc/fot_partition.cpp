@@ -432,3 +432,88 @@ extern "C" fot_status fot_kway_partition(const uint32_t *ci, const uint32_t *ce,
     return fot_kway_partition_impl(ci, ce, n, max_part, part, num_parts);
   });
 }
+
+namespace {
+
+// Recursive bisection state: verts[lo, hi) holds the vertices of one piece;
+// mark[v] == stamp marks the piece being split.
+struct Bisector {
+  const uint32_t *ci, *ce;
+  std::vector<uint32_t> verts, mark, order, tmp;
+  std::vector<uint8_t> seen;
+  uint32_t stamp = 0;
+  uint32_t *part;
+
+  // Breadth-first order of the piece from `root`, restricted to it; pieces
+  // in several components continue from their lowest unvisited position.
+  // Returns the last vertex reached from `root`'s component.
+  uint32_t bfs(uint32_t lo, uint32_t hi, uint32_t root) {
+    order.clear();
+    uint32_t last = root;
+    auto visit = [&](uint32_t s) {
+      size_t head = order.size();
+      order.push_back(s);
+      seen[s] = 1;
+      for (; head < order.size(); ++head) {
+        const uint32_t v = order[head];
+        for (uint32_t r = ci[v]; r < ci[v + 1]; ++r) {
+          const uint32_t w = ce[r];
+          if (mark[w] == stamp && !seen[w]) {
+            seen[w] = 1;
+            order.push_back(w);
+          }
+        }
+      }
+    };
+    visit(root);
+    last = order.back();
+    for (uint32_t i = lo; i < hi; ++i)
+      if (!seen[verts[i]]) visit(verts[i]);
+    for (uint32_t v : order) seen[v] = 0;
+    return last;
+  }
+
+  void split(uint32_t lo, uint32_t hi, uint32_t p, uint32_t first) {
+    if (p <= 1 || hi - lo <= 1) {
+      for (uint32_t i = lo; i < hi; ++i) part[verts[i]] = first;
+      return;
+    }
+    ++stamp;
+    for (uint32_t i = lo; i < hi; ++i) mark[verts[i]] = stamp;
+    // two sweeps: the far end of a sweep from the piece's first vertex, then
+    // the level order from that end (a pseudo-peripheral start)
+    const uint32_t far = bfs(lo, hi, verts[lo]);
+    bfs(lo, hi, far);
+    std::copy(order.begin(), order.end(), verts.begin() + lo);
+    const uint32_t p1 = p / 2;
+    const uint32_t n1 = uint32_t((uint64_t(hi - lo) * p1 + p / 2) / p);
+    split(lo, lo + n1, p1, first);
+    split(lo + n1, hi, p - p1, first + p1);
+  }
+};
+
+} // namespace
+
+static fot_status fot_bisect_partition_impl(const uint32_t *ci, const uint32_t *ce, uint32_t n,
+                                           uint32_t max_part, uint32_t *part,
+                                           uint32_t *num_parts) {
+  if (!ci || !part || !num_parts || max_part == 0 || (n && !ce && ci[n]))
+    return fail(FOT_ERR_INVALID_ARGUMENT, "fot_bisect_partition: bad arguments");
+  const uint32_t nparts = (n + max_part - 1) / max_part;
+  Bisector b;
+  b.ci = ci, b.ce = ce, b.part = part;
+  b.verts.resize(n);
+  for (uint32_t i = 0; i < n; ++i) b.verts[i] = i;
+  b.mark.assign(n, 0);
+  b.seen.assign(n, 0);
+  b.split(0, n, nparts, 0);
+  *num_parts = nparts;
+  return FOT_OK;
+}
+extern "C" fot_status fot_bisect_partition(const uint32_t *ci, const uint32_t *ce, uint32_t n,
+                                           uint32_t max_part, uint32_t *part,
+                                           uint32_t *num_parts) {
+  return fot::guarded("fot_bisect_partition", [&] {
+    return fot_bisect_partition_impl(ci, ce, n, max_part, part, num_parts);
+  });
+}

@@ -183,6 +183,19 @@ def kway_partition(ind, end, max_part: int) -> tuple[np.ndarray, int]:
     return part, npart.value
 
 
+def bisect_partition(ind, end, max_part: int) -> tuple[np.ndarray, int]:
+    """Recursive-bisection parts of at most max_part vertices (equal sizes)."""
+    ind, end = u32(ind), u32(end)
+    n = ind.size - 1
+    part = np.empty(n, dtype=np.uint32)
+    npart = C.c_uint32()
+    call("fot_bisect_partition", ptr(ind), ptr(end), n, max_part, ptr(part), C.byref(npart))
+    return part, npart.value
+
+
+PARTITIONERS = {"kway": kway_partition, "bisect": bisect_partition}
+
+
 def rcm_order(ind, end, seed_order=None) -> np.ndarray:
     """scipy's reverse Cuthill-McKee of the CSR pattern (symmetrised), as a
     permutation old -> new (scripts/rcm.py:20-21).  seed_order: the degree
@@ -202,20 +215,23 @@ def gps_order(ind, end) -> np.ndarray:
     return perm
 
 
-def partition_metis_enh(maps, num_points, block_size: int, tolerance: float = 1.001):
+def partition_metis_enh(maps, num_points, block_size: int, tolerance: float = 1.001,
+                        method: str = "kway"):
     """Block partition of the cells (Problem::partition, problem.hpp:484-499):
     parts of at most floor(block_size / tolerance) cells
-    (partitionMetisEnh, partition.hpp:23-29) from the k-way stand-in."""
+    (partitionMetisEnh, partition.hpp:23-29) from a METIS stand-in: greedy
+    k-way growing ("kway") or recursive bisection ("bisect")."""
     ind, end = cell_to_cell_csr(maps, num_points)
     max_part = int(np.floor(block_size / tolerance))
-    return kway_partition(ind, end, max_part)[0]
+    return PARTITIONERS[method](ind, end, max_part)[0]
 
 
-def partition_points(maps0, num_points: int, points_per_part: int) -> np.ndarray:
+def partition_points(maps0, num_points: int, points_per_part: int,
+                     method: str = "kway") -> np.ndarray:
     """Parts of at most points_per_part points of mapping 0's point graph
     (the owner partition of the owner-compute plan)."""
     ind, end = graph_csr(maps0, num_points)
-    return kway_partition(ind, end, points_per_part)[0]
+    return PARTITIONERS[method](ind, end, points_per_part)[0]
 
 
 def partition_ours(maps, num_points, block_size: int, num_blocks_in_coarse: int,

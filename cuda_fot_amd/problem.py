@@ -77,18 +77,20 @@ class Problem:
         self.cell_data = [reorder_data_inverse_cells(c, inv) for c in self.cell_data]
         self._compose(perm)
 
-    def partition(self, tolerance: float = 1.001) -> None:
-        """Problem::partition (problem.hpp:484-499) with the native k-way
-        partitioner in place of METIS."""
+    def partition(self, tolerance: float = 1.001, method: str = "kway") -> None:
+        """Problem::partition (problem.hpp:484-499) with a native partitioner
+        in place of METIS: greedy k-way growing ("kway") or recursive
+        bisection into equal compact parts ("bisect")."""
         self.partition_vector = plan.partition_metis_enh(self.maps, self.num_points,
-                                                         self.block_size, tolerance)
+                                                         self.block_size, tolerance, method)
 
     def partition_ours(self, tolerance: float = 1.001, num_blocks_in_coarse: int = 8) -> None:
         """Problem::partitionOurs (problem.hpp:501-515)."""
         self.partition_vector = plan.partition_ours(self.maps, self.num_points, self.block_size,
                                                     num_blocks_in_coarse, tolerance)
 
-    def partition_points(self, points_per_block: int, numbering: str = "cm") -> None:
+    def partition_points(self, points_per_block: int, numbering: str = "cm",
+                         method: str = "kway") -> None:
         """Owner partition for the owner-compute loop: parts of at most
         points_per_block mapping-0 points, then the mesh renumbered so every
         part's points are contiguous (mappings identical to mapping 0 follow),
@@ -99,7 +101,7 @@ class Problem:
         if numbering not in ("id", "cm"):
             raise ValueError(f"numbering {numbering!r}: 'id' or 'cm'")
         ind, end = plan.graph_csr(self.maps[0], self.num_points[0])
-        part = plan.kway_partition(ind, end, points_per_block)[0]
+        part = plan.PARTITIONERS[method](ind, end, points_per_block)[0]
         within = plan.gps_order(ind, end) if numbering == "cm" else np.arange(part.size)
         order = np.lexsort((within, part)).astype(np.uint32)
         perm = np.empty_like(order)

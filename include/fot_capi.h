@@ -368,6 +368,16 @@ fot_status fot_kway_partition(const uint32_t *csr_indices,
                               uint32_t num_vertices, uint32_t max_part,
                               uint32_t *part, uint32_t *num_parts);
 
+/* Recursive bisection of a CSR graph into ceil(n / max_part) parts of equal
+ * size (within one vertex): each piece is ordered breadth first from the far
+ * end of a sweep from its first vertex and cut at the share of its parts, so
+ * parts are compact and parts with nearby ids are neighbours.  A second
+ * stand-in for METIS (partition.cu:11-31, METIS's own recursive-bisection
+ * mode); the interface is fot_kway_partition's. */
+fot_status fot_bisect_partition(const uint32_t *csr_indices, const uint32_t *csr_endpoints,
+                                uint32_t num_vertices, uint32_t max_part, uint32_t *part,
+                                uint32_t *num_parts);
+
 /* scipy.sparse.csgraph.reverse_cuthill_mckee of the CSR matrix's pattern,
  * the ordering scripts/rcm.py:16-25 applies to an edge list: perm old->new
  * (rcm.py's np.argsort of scipy's result).  The pattern is symmetrised
@@ -476,7 +486,11 @@ typedef struct {
   uint32_t flags;         /* FOT_LOOP_* */
 } fot_loop_config;
 
-/* hierarchical: stage mapping-0 input point data in LDS as well */
+/* hierarchical: stage mapping-0 input point data in LDS as well (res_calc:
+ * q and adt; mini_aero: the cell values; lulesh: the node coordinates;
+ * volna: the cell volumes; mine: the iterate).  Functors other than res_calc
+ * gather per cell as without the flag when the staged values and the cache
+ * do not fit in LDS together. */
 #define FOT_LOOP_STAGE_INPUTS 1u
 /* hierarchical: single launch, blocks ordered by per-block dependency
  * flags instead of one launch per block colour */

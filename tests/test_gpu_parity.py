@@ -224,7 +224,7 @@ def test_mine_strategies(gpu, P, C, soa):
     g = mine_problem(P=P, C=C, soa=soa)
     g.gpu_loop(f, capi.STRATEGY_GLOBAL_COLOUR, ITERS)
     assert_bitwise(g.point_data[0], ref_colour, "mine global colour")
-    for flags in (0, DEP, TP):
+    for flags in (0, DEP, TP, STAGE, STAGE | TP, STAGE | DEP):
         h = mine_problem(P=P, C=C, soa=soa)
         h.gpu_loop(f, capi.STRATEGY_HIERARCHICAL, ITERS, flags=flags)
         assert_bitwise(h.point_data[0], ref_hier, f"mine hierarchical flags {flags}")

@@ -123,6 +123,8 @@ def test_lulesh_gpu_strategies(gpu, soa):
                                      hier.target),
                                     (capi.STRATEGY_HIERARCHICAL, capi.LOOP_TWO_PHASE,
                                      hier.target),
+                                    (capi.STRATEGY_HIERARCHICAL,
+                                     capi.LOOP_TWO_PHASE | capi.LOOP_STAGE_INPUTS, hier.target),
                                     (capi.STRATEGY_HIERARCHICAL, capi.LOOP_OWNER_COMPUTE,
                                      owner.target)]:
         p = lulesh_problem(14, 12, 11, seed=4, block_size=bs, soa=soa)

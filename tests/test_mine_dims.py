@@ -61,7 +61,7 @@ def test_mine_dims_all_strategies(gpu, md, P, C, soa):
     g = _problem(md, P, C, soa)
     g.gpu_loop(f, capi.STRATEGY_GLOBAL_COLOUR, ITERS)
     assert_bitwise(g.point_data[0], oc.target, "global colour vs oracle colour loop")
-    for flags in (0, DEP, TP):
+    for flags in (0, DEP, TP, capi.LOOP_STAGE_INPUTS | TP):
         # (fot_loop_run reports a dependency wait that ran out as an error)
         h = _problem(md, P, C, soa)
         h.gpu_loop(f, capi.STRATEGY_HIERARCHICAL, ITERS, flags=flags)

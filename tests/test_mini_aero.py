@@ -115,14 +115,18 @@ def test_mini_aero_gpu_strategies(gpu, soa):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("soa", [False, True])
-@pytest.mark.parametrize("bs", [128, 256, 512])
-@pytest.mark.parametrize("flags", [0, capi.LOOP_DEPENDENCY_ORDER, capi.LOOP_TWO_PHASE],
-                         ids=["per-colour", "dependency-order", "two-phase"])
+@pytest.mark.parametrize("bs", [128, 512, 1024])
+@pytest.mark.parametrize("flags", [0, capi.LOOP_DEPENDENCY_ORDER, capi.LOOP_TWO_PHASE,
+                                   capi.LOOP_STAGE_INPUTS,
+                                   capi.LOOP_STAGE_INPUTS | capi.LOOP_TWO_PHASE],
+                         ids=["per-colour", "dependency-order", "two-phase", "staged",
+                              "staged-two-phase"])
 def test_mini_aero_hierarchical_bitwise(gpu, soa, bs, flags):
     """The reference-shaped hierarchical loop, one launch per block colour,
-    one dependency-ordered launch or the two-phase writeback, at block sizes that run the 256-thread
-    instantiation (128, 256) and the 1024-thread one (512): bit for bit
-    against the oracle's replay of the same plan.  fot_loop_run reports a
+    one dependency-ordered launch or the two-phase writeback, the cell values
+    gathered per face or staged once per block in LDS, at block sizes that run
+    the 512-thread instantiation (128, 512) and the 1024-thread one (1024):
+    bit for bit against the oracle's replay of the same plan.  fot_loop_run reports a
     dependency wait that ran out as an error, so a pass also shows that every
     wait ended."""
     m = meshgen.hex_faces(12, 10, 9, seed=3)

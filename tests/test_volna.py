@@ -114,6 +114,10 @@ def test_volna_gpu_strategies(gpu, soa):
     p.gpu_loop(VOLNA, capi.STRATEGY_HIERARCHICAL, iters, flags=capi.LOOP_TWO_PHASE)
     assert_bitwise(p.point_data[0], replay.target, "volna hierarchical two-phase")
     p = volna_problem(m, block_size=bs, soa=soa)
+    p.gpu_loop(VOLNA, capi.STRATEGY_HIERARCHICAL, iters,
+               flags=capi.LOOP_TWO_PHASE | capi.LOOP_STAGE_INPUTS)
+    assert_bitwise(p.point_data[0], replay.target, "volna hierarchical staged two-phase")
+    p = volna_problem(m, block_size=bs, soa=soa)
     p.gpu_loop(VOLNA, capi.STRATEGY_HIERARCHICAL, iters, flags=capi.LOOP_OWNER_COMPUTE)
     assert_bitwise(p.point_data[0], owner.target, "volna owner compute")
     assert_close_scaled(p.point_data[0], serial, S, what="volna owner vs serial")
