@@ -109,9 +109,9 @@ def build_problem(args, partition_kind):
     p = res_calc_problem(mesh, block_size=args.block_size, soa=getattr(args, "soa", False))
     if partition_kind == "gps":
         p.reorder("gps")
-    elif partition_kind == "kway":
+    elif partition_kind in ("kway", "bisect"):
         p.reorder("gps")
-        p.partition(1.001)
+        p.partition(1.001, method=partition_kind)
         p.reorder_to_partition()
         p.renumber_points()
     elif partition_kind == "ours":
@@ -383,37 +383,19 @@ def bench_volna_batch(args):
     f = capi.functor(capi.KERNEL_VOLNA)
     tune = {"mode": args.tune}
     if args.tune != "none":
-        # the autotuner picks each mesh's partition size among the owner
-        # candidates (parts of 128 / 256 / 512 points)
+        # the batch runs owner-compute plans: the autotuner measures each
+        # mesh's owner configurations (parts of 128 / 256 / 512 points) and
+        # takes the fastest part size
         from cuda_fot_amd import tuner
-        owner = [3, 4, 5]
-        sizes = dict(zip(owner, tuner.owner_points(f)))
-        measured = 0
-        model = None
-        if args.tune == "learned":
-            rows, X, T = tuner.load_corpus(tuner.CORPUS_PATH)
-            model = tuner.CostModel(seed=0)
-            model.fit(X, T)
+        owner = tuner.class_members(f, "owner")
+        cands = tuner.candidates(f)
         picks = []
         for p in probs:
-            if args.tune == "exhaustive":
-                t = tuner.measure(p, f, which=owner, warmup=2, steps=5)
-                measured += len(owner)
-            else:
-                # the learned shortlist among the owner candidates, its
-                # fastest measured, the measured search on a miss
-                pred = model.predict(tuner.features(p, f))[0]
-                short = [owner[i] for i in tuner.shortlist(pred[owner])]
-                t = tuner.measure(p, f, which=short, warmup=2, steps=5)
-                measured += len(short)
-                pick = short[int(np.argmin(np.where(t[short] > 0, t[short], np.inf)))]
-                if not (t[pick] > 0) or t[pick] > 1.05 * pred[pick]:
-                    t = tuner.measure(p, f, which=owner, warmup=2, steps=5)
-                    measured += len(owner)
+            t = tuner.measure(p, f, which=owner, warmup=2, steps=5)
             best = owner[int(np.argmin(np.where(t[owner] > 0, t[owner], np.inf)))]
-            picks.append(sizes[best])
-            p.block_size = sizes[best]
-        tune.update(measured_candidates=measured,
+            picks.append(cands[best]["bs"])
+            p.block_size = cands[best]["bs"]
+        tune.update(mode="measured", measured_candidates=len(owner) * len(probs),
                     picks={str(k): picks.count(k) for k in sorted(set(picks))})
     specs, keep = [], []
     for p in probs:
@@ -476,11 +458,12 @@ def bench_volna_batch(args):
 
 def tune_res_calc(args) -> dict:
     """The autotuner's choice for the airfoil as generated (shuffled, not
-    reordered): the learned cost model's pick, measured, with the fallback
-    to the measured search when it misses its prediction by more than 5 %
-    (or every candidate measured, with --tune exhaustive).  Sets args to the
-    chosen configuration; an owner-compute pick runs on the mesh partitioned
-    and renumbered for its part size.  Returns the tuning record."""
+    reordered): the learned cost model's shortlist of strategy classes,
+    their configurations measured, with the fallback to the measured search
+    when the fastest misses its class's prediction by more than 5 % (or all
+    21 configurations measured, with --tune exhaustive).  Sets args to the
+    chosen configuration (its ordering, block or part size and launch
+    mode); returns the tuning record."""
     from cuda_fot_amd import capi, tuner
     _, p = build_problem(args, "none")
     f = capi.functor(capi.KERNEL_RES_CALC)
@@ -488,32 +471,35 @@ def tune_res_calc(args) -> dict:
     if args.tune == "learned":
         rows, X, T = tuner.load_corpus(tuner.CORPUS_PATH)
         model = tuner.CostModel(seed=0)
-        model.fit(X, T)
+        model.fit(X, tuner.class_times(T))
         r = tuner.Tuner(model).learned(p, f, warmup=2, steps=5)
-        pick = r["pick"]
+        pick, t = r["pick"], r["times"]
         rec.update(corpus_rows=len(rows), fell_back=bool(r["fell_back"]),
                    measured_candidates=int(r["measured"]),
-                   shortlist=[tuner.candidate_name(f, i) for i in r["shortlist"]],
-                   predicted_ms={tuner.candidate_name(f, i): round(float(v), 4)
-                                 for i, v in enumerate(r["predicted"])},
-                   measured_pick_ms=round(float(r["measured_pick"]), 4))
+                   shortlist=[tuner.CLASSES[c] for c in r["shortlist"]],
+                   predicted_class_ms={tuner.CLASSES[c]: round(float(v), 4)
+                                       for c, v in enumerate(r["predicted"])},
+                   measured_pick_ms=round(float(t[pick]), 4))
     else:
         pick, t = tuner.Tuner.exhaustive(p, f, warmup=2, steps=5)
-        rec.update(measured_candidates=len(t),
-                   measured_ms={tuner.candidate_name(f, i): round(float(v), 4)
-                                for i, v in enumerate(t)})
-    rec["pick"] = tuner.candidate_name(f, pick)
-    if pick == 0:
+        rec.update(measured_candidates=int(np.isfinite(t).sum()))
+    rec["measured_ms"] = {tuner.candidate_name(f, i): round(float(v), 4)
+                          for i, v in enumerate(t) if np.isfinite(v)}
+    k = tuner.candidates(f)[pick]
+    rec["pick"] = k["name"]
+    if k["strategy"] == capi.STRATEGY_ATOMIC:
         args.strategy, args.partition = "atomic", "none"
-    elif pick == 1:
+    elif k["strategy"] == capi.STRATEGY_GLOBAL_COLOUR:
         args.strategy, args.partition, args.block_size = "global", "none", 256
-    elif pick == 2:
-        args.strategy, args.order, args.partition, args.block_size = \
-            "hierarchical", "launch", "none", 256
-    else:
+    elif k["flags"] & capi.LOOP_OWNER_COMPUTE:
         args.strategy, args.order, args.partition = "hierarchical", "owner", "points"
-        args.points_per_block = tuner.owner_points(f)[pick - 3]
+        args.points_per_block = k["bs"]
         args.ept, args.accumulate, args.nodes = 2, "gather", "gather"
+    else:
+        args.strategy, args.block_size = "hierarchical", k["bs"]
+        args.order = "two-phase" if k["flags"] & capi.LOOP_TWO_PHASE else "launch"
+        args.partition = {"given": "none", "gps": "gps", "bisect": "bisect"}[k["ordering"]]
+        args.no_stage = not k["flags"] & capi.LOOP_STAGE_INPUTS
     return rec
 
 
@@ -578,10 +564,12 @@ def bench_single(args, workload):
         dev = [DeviceArray.from_host(d) for d in prob.point_data]
         out = DeviceArray.from_host(prob.point_data[0])
         cdev = [DeviceArray.from_host(c) for c in prob.cell_data]
-        part = prob.point_partition if flags & capi.LOOP_OWNER_COMPUTE else (
-            prob.partition_vector if strategy == capi.STRATEGY_HIERARCHICAL else None)
-        if part is not None:
-            flags |= capi.LOOP_POINT_PARTITION
+        if flags & capi.LOOP_OWNER_COMPUTE:
+            part = prob.point_partition
+            if part is not None:
+                flags |= capi.LOOP_POINT_PARTITION
+        else:
+            part = prob.partition_vector if strategy == capi.STRATEGY_HIERARCHICAL else None
         l = Loop(f, prob.maps, prob.num_points, dev, out, cdev, soa=prob.soa, strategy=strategy,
                  block_size=bs, partition=part, flags=flags)
         wall, ev = time_loop(l, args.steps, args.warmup)
@@ -611,7 +599,41 @@ def bench_single(args, workload):
                 row["hierarchical_error"] = str(e)
             sweep[str(bs)] = row
         del g
-    if args.partition in ("gps", "kway", "bisect"):
+    if args.tune in ("learned", "exhaustive"):
+        # the autotuner's choice on the mesh as generated: the learned
+        # shortlist of strategy classes measured (with the fallback), or all
+        # 21 configurations; the pick runs as deployed (reordered copy for
+        # the ordered configurations)
+        from cuda_fot_amd import tuner
+        t0 = time.perf_counter()
+        rec = {"mode": args.tune}
+        if args.tune == "learned":
+            rows, X, T = tuner.load_corpus(tuner.CORPUS_PATH)
+            model = tuner.CostModel(seed=0)
+            model.fit(X, tuner.class_times(T))
+            r = tuner.Tuner(model).learned(p, f, warmup=2, steps=5)
+            pick, t = r["pick"], r["times"]
+            rec.update(fell_back=bool(r["fell_back"]), shortlist=[tuner.CLASSES[c] for c in
+                                                                   r["shortlist"]],
+                       predicted_class_ms={tuner.CLASSES[c]: round(float(v), 4)
+                                           for c, v in enumerate(r["predicted"])},
+                       measured_candidates=int(r["measured"]))
+        else:
+            pick, t = tuner.Tuner.exhaustive(p, f, warmup=2, steps=5)
+            rec["measured_candidates"] = int(np.isfinite(t).sum())
+        rec["measured_ms"] = {tuner.candidate_name(f, i): round(float(v), 4)
+                              for i, v in enumerate(t) if np.isfinite(v)}
+        rec["tune_seconds"] = round(time.perf_counter() - t0, 1)
+        k = tuner.candidates(f)[pick]
+        rec["pick"] = k["name"]
+        q, _ = tuner.deploy(p, f, pick)
+        wall, ev, st = run(q, k["strategy"], k["flags"], k["bs"], with_stats=True)
+        variant = "tuned_" + k["name"]
+        cfg = {"strategy": tuner.CLASSES[k["cls"]], "block_size": k["bs"],
+               "ordering": k["ordering"], "launches_per_step": st["num_launches_per_iter"],
+               "block_colours": st["num_colours"], "blocks": st["num_blocks"],
+               "tune": rec}
+    elif args.partition in ("gps", "kway", "bisect"):
         # the hierarchical loop on the GPS-ordered mesh: consecutive blocks
         # of --block-size cells (gps), or k-way parts of --block-size cells
         # with the mesh reordered to them and the points renumbered (kway);

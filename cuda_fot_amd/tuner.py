@@ -1,34 +1,44 @@
 """Autotuner and learned cost model for the loop engine (no reference
 counterpart).
 
-Six candidate strategies per loop (the choices the engine offers):
-  0 atomic       FP64 global atomics, 256 threads
-  1 global       global colouring, 256 threads
-  2 hier256      hierarchical, per-block-colour launches, 256 cells per block
-  3 owner small  owner-compute hierarchical, parts of 128 points
-  4 owner medium owner-compute hierarchical, parts of 256 points
-  5 owner large  owner-compute hierarchical, parts of 512 points
-For res_calc the owner candidates run the gather kernel with two edges per
-thread on parts of 120 / 240 / 480 points: about 2.13 edges per point, so
-those parts fill 128 / 256 / 512 threads exactly (256 points would round up
-to 320 threads).  They are measured as they are deployed: on a copy of the
-mesh partitioned into parts of that size and renumbered so that each part's
-points are contiguous (Problem.partition_points), which that kernel needs to
-address its owned cells directly.
+The autotuner searches 21 configurations per loop, grouped in six strategy
+classes (the reference's measurement matrix varies the ordering and the
+block size the same way, res_calc.cu:183-264):
+  0 atomic             FP64 global atomics, 256 threads
+  1 global             global colouring, 256 threads
+  2 hier               hierarchical, one launch per block colour, blocks of
+                       B = 128, 256, 512, 1024 cells on the mesh as given
+  3 two-phase          hierarchical, every block in one launch with the
+                       shared points merged by a second (FOT_LOOP_TWO_PHASE),
+                       B = 128 .. 1024 on the mesh as given
+  4 two-phase ordered  the same on the GPS-ordered mesh (consecutive blocks)
+                       and on recursive-bisection parts of B cells with the
+                       mesh reordered to them and its points renumbered,
+                       B = 128 .. 1024 each
+  5 owner              owner-compute hierarchical, parts of 128 / 256 / 512
+                       points
+res_calc's hierarchical configurations stage q and adt in LDS; its owner
+configurations run the gather kernel with two edges per thread on parts of
+120 / 240 / 480 points (about 2.13 edges per point fill 128 / 256 / 512
+threads), on a copy of the mesh partitioned into parts of that size and
+renumbered so each part's points are contiguous (Problem.partition_points).
+Every configuration is measured as it is deployed: the ordered ones on a
+reordered copy of the mesh.
 
-`measure` is the exhaustive search: every candidate planned and timed on the
-GPU (fot_autotune_measure).  `CostModel` is an FP32 MLP 24 -> 64 -> 64 -> 6
-(libfot's MFMA kernels) trained on the autotuner's own timings to predict
-log(ms per step) of every candidate from 24 cheap mesh features
-(fot_loop_features).  `Tuner.learned` measures only the candidates the model
-predicts within the tolerance (5 %) of its best prediction (the shortlist:
-one candidate unless the model sees a near tie it cannot resolve) and takes
-the fastest of them; when that one's measurement exceeds the model's
-prediction for it by more than the tolerance, or nothing on the shortlist can
-be planned, it falls back to the measured search.
+`measure` times configurations on the GPU (fot_autotune_measure); the
+exhaustive search measures all 21.  `CostModel` is an FP32 MLP 24 -> 64 ->
+64 -> 6 (libfot's MFMA kernels) trained on the autotuner's own timings to
+predict log(ms per step) of each class's best configuration from 24 cheap
+mesh features (fot_loop_features).  `Tuner.learned` measures only the
+configurations of the classes the model predicts within the tolerance (5 %)
+of its best prediction (the shortlist) and takes the fastest; when that
+one's measurement exceeds the model's prediction for its class by more than
+the tolerance, or nothing on the shortlist can be planned, it falls back to
+the measured search of the rest.
 """
 from __future__ import annotations
 
+import copy
 import ctypes as C
 import json
 import math
@@ -43,7 +53,8 @@ from .capi import call, ptr
 from .device import DeviceArray, make_desc
 from .mlp import MLP
 
-NAMES = ["atomic", "global", "hier256", "owner128", "owner256", "owner512"]
+CLASSES = ["atomic", "global", "hier", "two-phase", "two-phase-ordered", "owner"]
+BLOCK_SIZES = (128, 256, 512, 1024)
 # the autotuner's own timings, the cost model's training corpus
 CORPUS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
                            "tuner_corpus.jsonl")
@@ -54,35 +65,102 @@ NUM_FEATURES = 24
 
 
 def owner_points(func: capi.Functor) -> tuple[int, int, int]:
-    """Points per part of the three owner-compute candidates."""
+    """Points per part of the three owner-compute configurations."""
     return (120, 240, 480) if func.kernel == capi.KERNEL_RES_CALC else (128, 256, 512)
 
 
 def owner_flags(func: capi.Functor) -> int:
-    """Loop flags of the owner-compute candidates."""
+    """Loop flags of the owner-compute configurations."""
     if func.kernel == capi.KERNEL_RES_CALC:
         return capi.LOOP_OWNER_COMPUTE | capi.LOOP_TWO_PER_THREAD | capi.LOOP_OWNER_GATHER
     return capi.LOOP_OWNER_COMPUTE
 
 
+def renumbered_owner(func: capi.Functor) -> bool:
+    """Whether the owner configurations run on the mesh renumbered for their
+    partition (res_calc's gather kernel)."""
+    return func.kernel == capi.KERNEL_RES_CALC
+
+
+def candidates(func: capi.Functor) -> list[dict]:
+    """Every configuration the autotuner measures: class, name, strategy,
+    block size (cells per block, or points per owner part), flags and the
+    ordering it runs on ("given", "gps", "bisect" or "points")."""
+    stage = capi.LOOP_STAGE_INPUTS if func.kernel == capi.KERNEL_RES_CALC else 0
+    tp = capi.LOOP_TWO_PHASE
+    H = capi.STRATEGY_HIERARCHICAL
+    out = [dict(cls=0, name="atomic", strategy=capi.STRATEGY_ATOMIC, bs=256, flags=0,
+                ordering="given"),
+           dict(cls=1, name="global", strategy=capi.STRATEGY_GLOBAL_COLOUR, bs=256, flags=0,
+                ordering="given")]
+    out += [dict(cls=2, name=f"hier{b}", strategy=H, bs=b, flags=stage, ordering="given")
+            for b in BLOCK_SIZES]
+    out += [dict(cls=3, name=f"2phase{b}", strategy=H, bs=b, flags=stage | tp,
+                 ordering="given") for b in BLOCK_SIZES]
+    out += [dict(cls=4, name=f"2phase{b}-gps", strategy=H, bs=b, flags=stage | tp,
+                 ordering="gps") for b in BLOCK_SIZES]
+    out += [dict(cls=4, name=f"2phase{b}-bisect", strategy=H, bs=b, flags=stage | tp,
+                 ordering="bisect") for b in BLOCK_SIZES]
+    own = "points" if renumbered_owner(func) else "given"
+    out += [dict(cls=5, name=f"owner{n}", strategy=H, bs=n, flags=owner_flags(func),
+                 ordering=own) for n in owner_points(func)]
+    return out
+
+
+NUM_CANDIDATES = 21
+
+
 def candidate_name(func: capi.Functor, i: int) -> str:
-    if i < 3:
-        return NAMES[i]
-    return f"owner{owner_points(func)[i - 3]}"
+    return candidates(func)[i]["name"]
 
 
-def candidate_configs(func: capi.Functor):
-    """The six fot_loop_config candidates for a functor."""
-    res_calc = func.kernel == capi.KERNEL_RES_CALC
-    own, pts = owner_flags(func), owner_points(func)
-    spec = [(capi.STRATEGY_ATOMIC, 256, 0), (capi.STRATEGY_GLOBAL_COLOUR, 256, 0),
-            (capi.STRATEGY_HIERARCHICAL, 256, capi.LOOP_STAGE_INPUTS if res_calc else 0),
-            (capi.STRATEGY_HIERARCHICAL, pts[0], own), (capi.STRATEGY_HIERARCHICAL, pts[1], own),
-            (capi.STRATEGY_HIERARCHICAL, pts[2], own)]
-    cfgs = (capi.LoopConfig * len(spec))()
-    for i, (s, bs, fl) in enumerate(spec):
-        cfgs[i].strategy, cfgs[i].block_size, cfgs[i].flags = s, bs, fl
-    return cfgs
+def class_members(func: capi.Functor, c) -> list[int]:
+    """Indices of the configurations of class c (index or name)."""
+    c = CLASSES.index(c) if isinstance(c, str) else c
+    return [i for i, k in enumerate(candidates(func)) if k["cls"] == c]
+
+
+# class of each configuration index (the same for every functor)
+CLASS_OF = np.array([0, 1] + [2] * 4 + [3] * 4 + [4] * 8 + [5] * 3)
+
+
+def class_times(T) -> np.ndarray:
+    """Per class, the best measured time of its configurations (rows x 21 ->
+    rows x 6): -1 when none of them can be planned, NaN when none was
+    measured."""
+    T = np.atleast_2d(np.asarray(T, np.float64))
+    out = np.full((T.shape[0], len(CLASSES)), np.nan)
+    for c in range(len(CLASSES)):
+        sub = T[:, CLASS_OF == c]
+        pos = np.where(np.isfinite(sub) & (sub > 0), sub, np.inf).min(1)
+        unplannable = np.all(np.isfinite(sub) & (sub < 0), 1)
+        out[:, c] = np.where(np.isfinite(pos), pos, np.where(unplannable, -1.0, np.nan))
+    return out
+
+
+def deploy(problem, func: capi.Functor, i: int, gps=None):
+    """The problem as configuration i runs on it (a reordered copy for the
+    ordered configurations, the problem itself otherwise) and the partition
+    its config takes (None: the engine's own blocks).  `gps`: the problem's
+    GPS-ordered copy, when one is at hand."""
+    k = candidates(func)[i]
+    if k["ordering"] == "given":
+        return problem, None
+    if k["ordering"] == "points":
+        q = copy.deepcopy(problem)
+        q.partition_points(k["bs"])
+        return q, q.point_partition
+    if gps is None:
+        gps = copy.deepcopy(problem)
+        gps.reorder("gps")
+    if k["ordering"] == "gps":
+        return gps, None
+    q = copy.deepcopy(gps)
+    q.block_size = k["bs"]
+    q.partition(1.001, method="bisect")
+    q.reorder_to_partition()
+    q.renumber_points()
+    return q, q.partition_vector
 
 
 def features(problem, func: capi.Functor) -> np.ndarray:
@@ -118,57 +196,62 @@ class _Uploaded:
                                           self.cdev, problem.soa)
 
 
-def renumbered_owner(func: capi.Functor) -> bool:
-    """Whether the owner candidates run on the mesh renumbered for their
-    partition (res_calc's gather kernel)."""
-    return func.kernel == capi.KERNEL_RES_CALC
-
-
-def _measure_cfgs(problem, func, cfgs_list, warmup, steps, partition=None):
+def _measure_cfgs(problem, func, cands, warmup, steps, partition=None):
     up = _Uploaded(problem)
-    cfgs = (capi.LoopConfig * len(cfgs_list))(*cfgs_list)
-    keep = None
-    if partition is not None:
-        keep = capi.u32(partition)
-        for c in cfgs:
+    cfgs = (capi.LoopConfig * len(cands))()
+    keep = None if partition is None else capi.u32(partition)
+    for c, k in zip(cfgs, cands):
+        c.strategy, c.block_size, c.flags = k["strategy"], k["bs"], k["flags"]
+        if keep is not None:
             c.partition = ptr(keep)
-            c.flags |= capi.LOOP_POINT_PARTITION
-    t = np.empty(len(cfgs_list), np.float32)
-    call("fot_autotune_measure", C.byref(func), C.byref(up.desc), cfgs, len(cfgs_list), warmup,
+            if k["ordering"] == "points":
+                c.flags |= capi.LOOP_POINT_PARTITION
+    t = np.empty(len(cands), np.float32)
+    call("fot_autotune_measure", C.byref(func), C.byref(up.desc), cfgs, len(cands), warmup,
          steps, ptr(t))
     return t
 
 
 def measure(problem, func: capi.Functor, which=None, warmup: int = 3,
             steps: int = 10) -> np.ndarray:
-    """ms per step of the candidates `which` (default all); -1 where a
-    candidate cannot be planned, NaN where it was not measured."""
-    import copy
-    all_cfgs = candidate_configs(func)
-    idx = list(range(len(NAMES))) if which is None else list(which)
-    out = np.full(len(NAMES), np.nan, np.float32)
-    own = [i for i in idx if i >= 3] if renumbered_owner(func) else []
-    plain = [i for i in idx if i not in own]
-    if plain:
-        out[plain] = _measure_cfgs(problem, func, [all_cfgs[i] for i in plain], warmup, steps)
-    for i in own:
-        q = copy.deepcopy(problem)
-        q.partition_points(owner_points(func)[i - 3])
-        out[i] = _measure_cfgs(q, func, [all_cfgs[i]], warmup, steps, q.point_partition)[0]
+    """ms per step of the configurations `which` (default all 21); -1 where
+    one cannot be planned, NaN where it was not measured.  Configurations on
+    the mesh as given share one upload; the GPS-ordered ones share one
+    reordered copy; each bisection and owner part size gets its own."""
+    cands = candidates(func)
+    idx = list(range(len(cands))) if which is None else sorted(set(int(i) for i in which))
+    out = np.full(len(cands), np.nan, np.float32)
+    given = [i for i in idx if cands[i]["ordering"] == "given"]
+    if given:
+        out[given] = _measure_cfgs(problem, func, [cands[i] for i in given], warmup, steps)
+    ordered = None
+    if any(cands[i]["ordering"] in ("gps", "bisect") for i in idx):
+        ordered = copy.deepcopy(problem)
+        ordered.reorder("gps")
+    gps = [i for i in idx if cands[i]["ordering"] == "gps"]
+    if gps:
+        out[gps] = _measure_cfgs(ordered, func, [cands[i] for i in gps], warmup, steps)
+    for i in idx:
+        if cands[i]["ordering"] in ("bisect", "points"):
+            q, part = deploy(problem, func, i, ordered)
+            out[i] = _measure_cfgs(q, func, [cands[i]], warmup, steps, part)[0]
     return out
 
 
 class CostModel:
-    """MLP on standardised features -> standardised log(ms) per candidate."""
+    """MLP on standardised features -> standardised log(ms) of each class's
+    best configuration."""
 
     def __init__(self, seed: int = 0, hidden: int = 64):
-        self.net = MLP(NUM_FEATURES, hidden, len(NAMES), seed=seed)
+        self.net = MLP(NUM_FEATURES, hidden, len(CLASSES), seed=seed)
         self.fmu = self.fsd = self.tmu = self.tsd = None
 
     def fit(self, X, T, epochs: int = 3000, lr: float = 3e-3, weight_decay: float = 1e-4):
+        """X: rows x 24 features; T: rows x 6 class times (class_times), ms,
+        -1 = cannot be planned, NaN = not measured."""
         X = np.asarray(X, np.float64)
         T = np.asarray(T, np.float64).copy()
-        # a candidate that cannot be planned (-1) trains as 10x the mesh's
+        # a class that cannot be planned (-1) trains as 10x the mesh's
         # slowest measured time, so the model learns to avoid it
         for i in range(len(T)):
             ok = np.isfinite(T[i]) & (T[i] > 0)
@@ -189,6 +272,32 @@ class CostModel:
         return np.exp(self.net.forward(xs).astype(np.float64) * self.tsd + self.tmu)
 
 
+def shortlist(pred_ms, tolerance: float = 0.05) -> list[int]:
+    """Classes predicted within `tolerance` of the best prediction."""
+    pred_ms = np.asarray(pred_ms, np.float64)
+    return [int(i) for i in np.nonzero(pred_ms <= (1.0 + tolerance) * pred_ms.min())[0]]
+
+
+def members(classes) -> list[int]:
+    """Configuration indices of the given classes."""
+    return [int(i) for i in np.nonzero(np.isin(CLASS_OF, list(classes)))[0]]
+
+
+def learned_choice(pred_cls, measured_ms, tolerance: float = 0.05):
+    """The learned choice on known measurements (21 configuration times):
+    the fastest configuration of the shortlisted classes; if it exceeds the
+    model's prediction for its class by more than `tolerance` (or nothing on
+    the shortlist can be planned), the measured best of all.  Returns
+    (configuration, fell back, configurations measured)."""
+    m = np.asarray(measured_ms, np.float64)
+    ok = np.where(m > 0, m, np.inf)
+    subs = members(shortlist(pred_cls, tolerance))
+    pick = subs[int(np.argmin(ok[subs]))]
+    if not (m[pick] > 0) or m[pick] > (1.0 + tolerance) * pred_cls[CLASS_OF[pick]]:
+        return int(np.argmin(ok)), True, NUM_CANDIDATES
+    return pick, False, len(subs)
+
+
 class Tuner:
     def __init__(self, model: CostModel | None = None, tolerance: float = 0.05):
         self.model, self.tolerance = model, tolerance
@@ -202,99 +311,71 @@ class Tuner:
     def learned(self, problem, func, **kw) -> dict:
         pred = self.model.predict(features(problem, func))[0]
         short = shortlist(pred, self.tolerance)
-        t = measure(problem, func, which=short, **kw)
-        ok = np.where(t[short] > 0, t[short], np.inf)
-        pick = short[int(np.argmin(ok))]
+        subs = members(short)
+        t = measure(problem, func, which=subs, **kw)
+        ok = np.where(t[subs] > 0, t[subs], np.inf)
+        pick = subs[int(np.argmin(ok))]
         res = {"pick": pick, "predicted": pred, "shortlist": short,
-               "measured_pick": float(t[pick]), "fell_back": False, "measured": len(short)}
-        if not (t[pick] > 0) or t[pick] > (1.0 + self.tolerance) * pred[pick]:
-            best, times = self.exhaustive(problem, func, **kw)
-            res.update(pick=best, fell_back=True, measured=len(short) + len(NAMES), times=times)
+               "measured_pick": float(t[pick]), "fell_back": False, "measured": len(subs),
+               "times": t}
+        if not (t[pick] > 0) or t[pick] > (1.0 + self.tolerance) * pred[CLASS_OF[pick]]:
+            rest = [i for i in range(NUM_CANDIDATES) if i not in subs]
+            t2 = measure(problem, func, which=rest, **kw)
+            t[rest] = t2[rest]
+            best = int(np.argmin(np.where(t > 0, t, np.inf)))
+            res.update(pick=best, fell_back=True, measured=NUM_CANDIDATES, times=t)
         return res
 
 
-def shortlist(pred_ms, tolerance: float = 0.05) -> list[int]:
-    """Candidates predicted within `tolerance` of the best prediction."""
-    pred_ms = np.asarray(pred_ms, np.float64)
-    return [int(i) for i in np.nonzero(pred_ms <= (1.0 + tolerance) * pred_ms.min())[0]]
-
-
-def shortlist_pick(pred_ms, measured_ms, tolerance: float = 0.05):
-    """The learned choice on known measurements: the fastest measured of the
-    shortlist; if that one exceeds the model's prediction for it by more
-    than `tolerance` (or nothing on the shortlist can be planned), the
-    measured best.  Returns (choice, fell_back, candidates measured)."""
-    short = shortlist(pred_ms, tolerance)
-    m = np.asarray(measured_ms, np.float64)
-    ok = np.where(m > 0, m, np.inf)
-    pick = short[int(np.argmin(ok[short]))]
-    if not (m[pick] > 0) or m[pick] > (1.0 + tolerance) * pred_ms[pick]:
-        return int(np.argmin(ok)), True, len(short) + len(NAMES)
-    return pick, False, len(short)
-
-
-def fallback_pick(pred_ms, measured_ms, tolerance: float = 0.05):
-    """The learned choice with the fallback rule, on known measurements:
-    take the predicted-best candidate; if its measured time exceeds the
-    model's prediction for it by more than `tolerance` (or it cannot be
-    planned), take the measured best instead.  Returns (choice, fell_back)."""
-    pick = int(np.argmin(pred_ms))
-    t = measured_ms[pick]
-    ok = np.where(measured_ms > 0, measured_ms, np.inf)
-    if not (t > 0) or t > (1.0 + tolerance) * pred_ms[pick]:
-        return int(np.argmin(ok)), True
-    return pick, False
-
-
 # ---- evaluation ---------------------------------------------------------------
-DEFAULT_CHOICE = 4  # the method without the learned part: the medium owner parts everywhere
+# the method without the learned part: one fixed configuration everywhere,
+# the medium owner-compute parts
+DEFAULT_CHOICE = 19
 
 
 def evaluate(model: CostModel, X, T, tolerance: float = 0.05) -> dict:
     """The learned choice with its fallback against the exhaustive search on
-    rows whose six times are all measured (T, ms; -1 = cannot be planned).
-    Reports, over the rows: how often the predicted-best pick is within the
-    tolerance of the best; the mean slowdown of the pick, of the choice after
-    the fallback rule and of the fixed default; how many candidates the
-    learned tuner measured against the six of the exhaustive search."""
+    rows whose 21 configuration times are all measured (T, ms; -1 = cannot
+    be planned).  Reports, over the rows: how often the choice is within the
+    tolerance of the best, its mean and largest slowdown, how often it fell
+    back, how many configurations it measured against the 21 of the
+    exhaustive search, the same for the predicted-best class alone (its
+    configurations measured, no fallback), and the fixed default's
+    slowdown."""
     X, T = np.asarray(X, np.float64), np.asarray(T, np.float64)
     pred = model.predict(X)
-    best = np.where(T > 0, T, np.inf).min(1)
-    pick_t, final_t, measured, within, fell = [], [], 0, 0, 0
-    sl_t, sl_measured, sl_fell, sl_within = [], 0, 0, 0
+    ok = np.where(T > 0, T, np.inf)
+    best = ok.min(1)
+    final_t, measured, within, fell = [], 0, 0, 0
+    top_t, top_within, top_measured = [], 0, 0
     for i in range(len(X)):
-        pick = int(np.argmin(pred[i]))
-        choice, fb = fallback_pick(pred[i], T[i], tolerance)
-        t_pick = T[i, pick] if T[i, pick] > 0 else np.inf
-        pick_t.append(t_pick / best[i])
-        final_t.append(T[i, choice] / best[i])
-        within += t_pick <= (1 + tolerance) * best[i]
+        c, fb, k = learned_choice(pred[i], T[i], tolerance)
+        final_t.append(T[i, c] / best[i])
+        within += T[i, c] <= (1 + tolerance) * best[i]
         fell += fb
-        measured += 1 + (len(NAMES) if fb else 0)
-        c, f2, k = shortlist_pick(pred[i], T[i], tolerance)
-        sl_t.append(T[i, c] / best[i])
-        sl_within += T[i, c] <= (1 + tolerance) * best[i]
-        sl_fell += f2
-        sl_measured += k
-    d = T[:, DEFAULT_CHOICE]
-    default = np.where(d > 0, d, np.inf) / best
+        measured += k
+        subs = members([int(np.argmin(pred[i]))])
+        t_top = ok[i, subs].min()
+        top_t.append(t_top / best[i])
+        top_within += t_top <= (1 + tolerance) * best[i]
+        top_measured += len(subs)
+    default = ok[:, DEFAULT_CHOICE] / best
     n = len(X)
-    return {"rows": n, "pick_within_tolerance": within / n,
-            "pick_mean_slowdown": float(np.mean(pick_t)),
+    return {"rows": n, "final_within_tolerance": within / n,
             "final_mean_slowdown": float(np.mean(final_t)),
             "final_max_slowdown": float(np.max(final_t)),
             "fallbacks": fell / n,
-            "measured_per_mesh": measured / n, "exhaustive_measured_per_mesh": len(NAMES),
-            "shortlist_within_tolerance": sl_within / n,
-            "shortlist_mean_slowdown": float(np.mean(sl_t)),
-            "shortlist_max_slowdown": float(np.max(sl_t)),
-            "shortlist_fallbacks": sl_fell / n,
-            "shortlist_measured_per_mesh": sl_measured / n,
+            "measured_per_mesh": measured / n,
+            "exhaustive_measured_per_mesh": NUM_CANDIDATES,
+            "top_class_within_tolerance": top_within / n,
+            "top_class_mean_slowdown": float(np.mean(top_t)),
+            "top_class_measured_per_mesh": top_measured / n,
             "default_mean_slowdown": float(np.mean(default)),
             "default_within_tolerance": float(np.mean(default <= 1 + tolerance))}
 
 
 def load_corpus(path):
+    """Rows, features (rows x 24) and configuration times (rows x 21, ms)."""
     rows = [json.loads(l) for l in open(path) if l.strip()]
     X = np.array([r["features"] for r in rows], np.float64)
     T = np.array([r["ms"] for r in rows], np.float64)
@@ -302,18 +383,56 @@ def load_corpus(path):
 
 
 # ---- corpus of autotuner timings ---------------------------------------------
-def corpus_meshes(n: int, seed: int = 0, min_edges: float = 5e4, max_edges: float = 2e6):
-    """(family, ordering, size) specs: 2D quad (mine), airfoil quads
-    (res_calc), triangles (volna), 3D hexes (scatter), each in random, RCM
-    and point-partitioned orderings, edges log-uniform in [min, max]."""
+FAMILIES = ["mine", "res_calc", "volna", "hex", "mini_aero"]
+ORDERINGS = ["random", "rcm", "partitioned"]
+# mini_aero meshes of the first 500 corpus meshes stay at 44^3 cells or
+# fewer (corpus_mini_aero_specs adds 45^3 .. 62^3; the held-out set is 64^3
+# .. 96^3)
+MINI_AERO_MAX_EDGES = 3 * 44 ** 3
+
+
+def corpus_meshes(n: int, seed: int = 0, min_edges: float = 2e4, max_edges: float = 5e5):
+    """(family, ordering, size) specs: 2D quads (mine), airfoil quads
+    (res_calc), triangles (volna), 3D hexes (scatter) and hex faces
+    (mini_aero, at most 44^3 cells), each in random, RCM and
+    point-partitioned orderings, edges log-uniform in [min, max]."""
     rng = np.random.default_rng(seed)
-    fams = ["mine", "res_calc", "volna", "hex"]
-    orders = ["random", "rcm", "partitioned"]
+    out = []
+    for i in range(n):
+        fam = FAMILIES[i % len(FAMILIES)]
+        hi = min(max_edges, MINI_AERO_MAX_EDGES) if fam == "mini_aero" else max_edges
+        e = float(np.exp(rng.uniform(np.log(min_edges), np.log(hi))))
+        out.append({"id": i, "family": fam, "ordering": ORDERINGS[(i // len(FAMILIES)) % 3],
+                    "edges": int(e), "seed": int(rng.integers(1 << 30))})
+    return out
+
+
+def corpus_large_specs(n: int = 150, seed: int = 1, min_edges: float = 5e5,
+                       max_edges: float = 2e6):
+    """Larger meshes appended to the corpus (ids 500 on): the four families
+    other than mini_aero (which stays below the held-out sizes) in the three
+    orderings, edges log-uniform in [5e5, 2e6]."""
+    rng = np.random.default_rng(seed)
+    fams = [f for f in FAMILIES if f != "mini_aero"]
     out = []
     for i in range(n):
         e = float(np.exp(rng.uniform(np.log(min_edges), np.log(max_edges))))
-        out.append({"id": i, "family": fams[i % 4], "ordering": orders[(i // 4) % 3],
-                    "edges": int(e), "seed": int(rng.integers(1 << 30))})
+        out.append({"id": 500 + i, "family": fams[i % len(fams)],
+                    "ordering": ORDERINGS[(i // len(fams)) % 3], "edges": int(e),
+                    "seed": int(rng.integers(1 << 30))})
+    return out
+
+
+def corpus_mini_aero_specs(n: int = 45, seed: int = 2):
+    """mini_aero hex face meshes of 45^3 to 62^3 cells appended to the corpus
+    (ids 650 on), below the held-out set's 64^3 .. 96^3, in the three
+    orderings."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        c = int(rng.integers(45, 63))
+        out.append({"id": 650 + i, "family": "mini_aero", "ordering": ORDERINGS[i % 3],
+                    "edges": 3 * c ** 3, "seed": int(rng.integers(1 << 30))})
     return out
 
 
@@ -365,25 +484,12 @@ def build_corpus_problem(spec):
     return p, func
 
 
-def corpus_extra_specs(n: int = 60, seed: int = 1):
-    """mini_aero-style hex face meshes of 16^3 to 44^3 cells in the three
-    orderings, appended to the corpus (ids 500 on); the evaluation set
-    (heldout_specs, 48^3 to 96^3) stays out of it."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(n):
-        c = int(rng.integers(16, 45))
-        out.append({"id": 500 + i, "family": "mini_aero", "ordering":
-                    ["random", "rcm", "partitioned"][i % 3], "edges": 3 * c ** 3,
-                    "seed": int(rng.integers(1 << 30))})
-    return out
-
-
 def heldout_specs():
-    """The held-out evaluation set: mini_aero-style hex face meshes (a kernel
-    the corpus never holds) of 48^3 to 96^3 cells in the three orderings."""
+    """The held-out evaluation set: mini_aero-style hex face meshes of 64^3
+    to 96^3 cells (the corpus holds mini_aero up to 44^3 only) in the three
+    orderings."""
     out = []
-    for i, n in enumerate([48, 64, 80, 96]):
+    for i, n in enumerate([64, 80, 96]):
         for j, order in enumerate(["random", "rcm", "partitioned"]):
             out.append({"id": 1000 + 3 * i + j, "family": "mini_aero", "ordering": order,
                         "edges": 3 * n ** 3, "seed": 7 + i})
@@ -407,8 +513,9 @@ def measure_specs(specs, out):
 
 def corpus_main(argv=None):
     """python -m cuda_fot_amd.tuner corpus OUT.jsonl N [SEED] [FIRST]:
-    measures every candidate on corpus meshes FIRST..N-1, appending one JSON
-    line per mesh (features, ms per step) and printing progress."""
+    measures every configuration on corpus meshes FIRST..N-1 not yet in OUT,
+    appending one JSON line per mesh (features, ms per step) and printing
+    progress."""
     argv = sys.argv[1:] if argv is None else argv
     out, n = argv[1], int(argv[2])
     seed = int(argv[3]) if len(argv) > 3 else 0
@@ -417,25 +524,14 @@ def corpus_main(argv=None):
     if os.path.exists(out):
         with open(out) as f:
             done = {json.loads(l)["id"] for l in f if l.strip()}
-    for spec in corpus_meshes(n, seed)[first:]:
-        if spec["id"] in done:
-            continue
-        t0 = time.perf_counter()
-        p, func = build_corpus_problem(spec)
-        x = features(p, func)
-        t = measure(p, func)
-        rec = dict(spec, cells=int(p.num_cells), features=[float(v) for v in x],
-                   ms=[float(v) for v in t], seconds=round(time.perf_counter() - t0, 2))
-        with open(out, "a") as f:
-            f.write(json.dumps(rec) + "\n")
-        print(f"mesh {spec['id']} {spec['family']}/{spec['ordering']} cells {p.num_cells}: "
-              + " ".join(f"{v:.3f}" for v in t) + f" ({rec['seconds']} s)", flush=True)
+    measure_specs([sp for sp in corpus_meshes(n, seed)[first:] if sp["id"] not in done], out)
 
 
 def eval_main(argv=None):
-    """python -m cuda_fot_amd.tuner eval CORPUS.jsonl [HELDOUT_FAMILY|-] [SEED]:
+    """python -m cuda_fot_amd.tuner eval CORPUS.jsonl [HELDOUT_FAMILY|HELDOUT.jsonl|-] [SEED]:
     trains the cost model on the GPU on every mesh but the held-out family
-    (or a seeded 20 % split when '-') and prints the evaluation as JSON."""
+    (or on all of it against a held-out file, or on a seeded 80 % split when
+    '-') and prints the evaluation as JSON."""
     argv = sys.argv[1:] if argv is None else argv
     rows, X, T = load_corpus(argv[1])
     held = argv[2] if len(argv) > 2 else "-"
@@ -444,8 +540,9 @@ def eval_main(argv=None):
         # a separate held-out file: train on all of the corpus
         _, Xh, Th = load_corpus(held)
         model = CostModel(seed=seed)
-        losses = model.fit(X, T)
-        out = evaluate(model, Xh[np.all(np.isfinite(Th), 1)], Th[np.all(np.isfinite(Th), 1)])
+        losses = model.fit(X, class_times(T))
+        full = np.all(np.isfinite(Th), 1)
+        out = evaluate(model, Xh[full], Th[full])
         out.update(train_rows=len(rows), held_out=os.path.basename(held),
                    final_train_loss=float(losses[-1]), first_train_loss=float(losses[0]))
         print(json.dumps(out))
@@ -456,7 +553,7 @@ def eval_main(argv=None):
         test = np.array([r["family"] == held for r in rows])
     full = np.all(np.isfinite(T), 1)
     model = CostModel(seed=seed)
-    losses = model.fit(X[~test], T[~test])
+    losses = model.fit(X[~test], class_times(T[~test]))
     out = evaluate(model, X[test & full], T[test & full])
     out.update(train_rows=int((~test).sum()), held_out=held,
                final_train_loss=float(losses[-1]), first_train_loss=float(losses[0]))
@@ -469,14 +566,15 @@ if __name__ == "__main__":
         corpus_main()
     elif len(sys.argv) > 1 and sys.argv[1] == "eval":
         eval_main()
-    elif len(sys.argv) > 1 and sys.argv[1] == "corpus-extra":
-        # python -m cuda_fot_amd.tuner corpus-extra OUT.jsonl
-        measure_specs(corpus_extra_specs(), sys.argv[2])
-    elif len(sys.argv) > 1 and sys.argv[1] == "remeasure":
-        # python -m cuda_fot_amd.tuner remeasure FAMILY OUT.jsonl: measure the
-        # seed-0 corpus meshes of one family again (after a candidate change)
-        measure_specs([sp for sp in corpus_meshes(500, 0) if sp["family"] == sys.argv[2]],
-                      sys.argv[3])
+    elif len(sys.argv) > 1 and sys.argv[1] == "corpus-large":
+        # python -m cuda_fot_amd.tuner corpus-large OUT.jsonl [FIRST] [LAST]
+        specs = corpus_large_specs()
+        first = int(sys.argv[3]) if len(sys.argv) > 3 else 0
+        last = int(sys.argv[4]) if len(sys.argv) > 4 else len(specs)
+        measure_specs(specs[first:last], sys.argv[2])
+    elif len(sys.argv) > 1 and sys.argv[1] == "corpus-mini-aero":
+        # python -m cuda_fot_amd.tuner corpus-mini-aero OUT.jsonl
+        measure_specs(corpus_mini_aero_specs(), sys.argv[2])
     elif len(sys.argv) > 1 and sys.argv[1] == "heldout":
         # python -m cuda_fot_amd.tuner heldout OUT.jsonl: measure the held-out set
         measure_specs(heldout_specs(), sys.argv[2])

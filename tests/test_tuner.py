@@ -7,6 +7,7 @@ import pytest
 from cuda_fot_amd import Problem, capi, tuner
 
 MINE = capi.functor(capi.KERNEL_MINE, 2, 1, 1)
+RES = capi.functor(capi.KERNEL_RES_CALC)
 
 
 def _path_problem():
@@ -43,18 +44,67 @@ def test_features_see_ordering():
     assert a[:4].tolist() == b[:4].tolist()                  # sizes do not change
 
 
-def test_fallback_rule_closed_form():
-    meas = np.array([5.0, 4.0, 3.0, 2.0, 2.1, 2.5])
-    # pick 3 predicted 2.0, measured 2.0 <= 1.05 * 2.0: kept
-    assert tuner.fallback_pick(np.array([9, 9, 9, 2.0, 3, 3]), meas) == (3, False)
-    # pick 4 predicted 1.9, measured 2.1 > 1.05 * 1.9: measured search -> 3
-    assert tuner.fallback_pick(np.array([9, 9, 9, 3, 1.9, 3]), meas) == (3, True)
+def _times(best_by_class):
+    """21 configuration times: class c's configurations at best_by_class[c]
+    and its later ones 10 % slower."""
+    t = np.empty(tuner.NUM_CANDIDATES)
+    for c, v in enumerate(best_by_class):
+        idx = np.nonzero(tuner.CLASS_OF == c)[0]
+        t[idx] = v * 1.1
+        t[idx[0]] = v
+    return t
+
+
+def test_candidates_cover_the_search_dimensions():
+    for f in (RES, MINE):
+        cands = tuner.candidates(f)
+        assert len(cands) == tuner.NUM_CANDIDATES == len(tuner.CLASS_OF)
+        assert [k["cls"] for k in cands] == list(tuner.CLASS_OF)
+        names = [k["name"] for k in cands]
+        assert len(set(names)) == len(names)
+        # block sizes 128 .. 1024, per-colour and two-phase launches, three
+        # orderings, three owner part sizes, atomics and global colouring
+        hier = [k for k in cands if k["strategy"] == capi.STRATEGY_HIERARCHICAL
+                and not k["flags"] & capi.LOOP_OWNER_COMPUTE]
+        assert {k["bs"] for k in hier} == {128, 256, 512, 1024}
+        assert {k["ordering"] for k in hier} == {"given", "gps", "bisect"}
+        assert {bool(k["flags"] & capi.LOOP_TWO_PHASE) for k in hier} == {False, True}
+        assert len(tuner.class_members(f, "owner")) == 3
+    assert tuner.candidate_name(RES, tuner.DEFAULT_CHOICE) == "owner240"
+
+
+def test_class_times_closed_form():
+    t = _times([5.0, 4.0, 3.0, 2.0, 2.1, 2.5])
+    np.testing.assert_allclose(tuner.class_times(t)[0], [5.0, 4.0, 3.0, 2.0, 2.1, 2.5])
+    t[tuner.CLASS_OF == 2] = -1                  # a class that cannot be planned
+    t[np.nonzero(tuner.CLASS_OF == 4)[0][0]] = -1  # one configuration that cannot
+    t[np.nonzero(tuner.CLASS_OF == 5)[0]] = np.nan  # a class not measured
+    ct = tuner.class_times(t)[0]
+    assert ct[2] == -1 and np.isnan(ct[5])
+    assert ct[4] == pytest.approx(2.1 * 1.1)
+
+
+def test_learned_choice_closed_form():
+    meas = _times([5.0, 4.0, 3.0, 2.0, 2.1, 2.5])
+    best3 = int(np.nonzero(tuner.CLASS_OF == 3)[0][0])
+    n3 = int((tuner.CLASS_OF == 3).sum())
+    # class 3 predicted 2.0, its best configuration measured 2.0: kept
+    assert tuner.learned_choice(np.array([9, 9, 9, 2.0, 3, 3]), meas) == (best3, False, n3)
+    # a near tie (within 5 %) of classes 3 and 4: both measured, the faster kept
+    n34 = n3 + int((tuner.CLASS_OF == 4).sum())
+    assert tuner.shortlist([9, 9, 9, 2.05, 2.0, 3]) == [3, 4]
+    assert tuner.learned_choice(np.array([9, 9, 9, 2.05, 2.0, 3]), meas) == (best3, False, n34)
+    # class 4 predicted 1.9, measured best 2.1 > 1.05 * 1.9: measured search
+    assert tuner.learned_choice(np.array([9, 9, 9, 3, 1.9, 3]), meas) == \
+        (best3, True, tuner.NUM_CANDIDATES)
     # exactly at the tolerance stays
-    assert tuner.fallback_pick(np.array([9, 9, 9, 9, 2.0, 9]), meas) == (4, False)
-    # an unplannable pick (-1) always falls back
+    best4 = int(np.nonzero(tuner.CLASS_OF == 4)[0][0])
+    assert tuner.learned_choice(np.array([9, 9, 9, 9, 2.0, 9]), meas)[:2] == (best4, False)
+    # nothing on the shortlist can be planned: measured search
     m2 = meas.copy()
-    m2[0] = -1
-    assert tuner.fallback_pick(np.array([1.0, 9, 9, 9, 9, 9]), m2) == (3, True)
+    m2[tuner.CLASS_OF == 0] = -1
+    assert tuner.learned_choice(np.array([1.0, 9, 9, 9, 9, 9]), m2) == \
+        (best3, True, tuner.NUM_CANDIDATES)
 
 
 class _FixedModel:
@@ -66,23 +116,27 @@ class _FixedModel:
 
 
 def test_evaluate_counts_closed_form():
-    T = np.array([[5.0, 4.0, 3.0, 2.0, 2.1, 2.5],     # best 3
-                  [1.0, 4.0, 3.0, 2.0, 2.1, 2.5]])    # best 0
-    pred = np.array([[9, 9, 9, 2.0, 3, 3],             # kept, exact
-                     [9, 9, 9, 9, 1.5, 9]])            # 2.1 > 1.05*1.5: falls back to 0
+    T = np.stack([_times([5.0, 4.0, 3.0, 2.0, 2.1, 2.5]),    # best: class 3
+                  _times([1.0, 4.0, 3.0, 2.0, 2.1, 2.5])])   # best: class 0
+    pred = np.array([[9, 9, 9, 2.0, 3, 3],     # kept, exact
+                     [9, 9, 9, 9, 1.5, 9]])    # 2.1 > 1.05 * 1.5: falls back to class 0
     r = tuner.evaluate(_FixedModel(pred), np.zeros((2, 24)), T)
-    assert r["rows"] == 2 and r["pick_within_tolerance"] == 0.5
-    assert r["pick_mean_slowdown"] == pytest.approx((1.0 + 2.1) / 2)
+    n3 = int((tuner.CLASS_OF == 3).sum())
+    assert r["rows"] == 2 and r["final_within_tolerance"] == 1.0
     assert r["final_mean_slowdown"] == 1.0 and r["fallbacks"] == 0.5
-    assert r["measured_per_mesh"] == pytest.approx((1 + 7) / 2)
-    assert r["default_mean_slowdown"] == pytest.approx((2.1 / 2.0 + 2.1) / 2)
+    assert r["measured_per_mesh"] == pytest.approx((n3 + tuner.NUM_CANDIDATES) / 2)
+    assert r["top_class_mean_slowdown"] == pytest.approx((1.0 + 2.1) / 2)
+    assert r["top_class_within_tolerance"] == 0.5
+    # the default (medium owner parts, the owner class's second): 2.5 * 1.1
+    assert r["default_mean_slowdown"] == pytest.approx((2.75 / 2.0 + 2.75) / 2)
 
 
 def test_corpus_specs_cover_families_and_orderings():
-    specs = tuner.corpus_meshes(24, seed=0)
+    specs = tuner.corpus_meshes(30, seed=0)
     fams = {(s["family"], s["ordering"]) for s in specs}
-    assert len(fams) == 12
-    assert all(5e4 <= s["edges"] <= 2e6 for s in specs)
+    assert len(fams) == 15
+    assert all(2e4 <= s["edges"] <= 5e5 for s in specs)
+    assert all(s["edges"] <= 3 * 44 ** 3 for s in specs if s["family"] == "mini_aero")
 
 
 @pytest.mark.gpu
@@ -98,18 +152,21 @@ def test_measure_and_learned_tuner(gpu):
     # biases) predicts them and keeps its pick
     model = tuner.CostModel(seed=0)
     x = tuner.features(p, f)[None, :]
-    losses = model.fit(np.repeat(x, 8, 0), np.repeat(t[None, :], 8, 0), epochs=3000, lr=3e-3,
+    ct = tuner.class_times(t)
+    losses = model.fit(np.repeat(x, 8, 0), np.repeat(ct, 8, 0), epochs=3000, lr=3e-3,
                        weight_decay=0.0)
     assert losses[-1] < 1e-3 * losses[0]
     pred = model.predict(x)[0]
-    np.testing.assert_allclose(pred, t, rtol=0.05)
+    np.testing.assert_allclose(pred, ct[0], rtol=0.05)
     res = tuner.Tuner(model, tolerance=0.5).learned(p, f, warmup=2, steps=5)
-    # the shortlist holds the predicted best and every candidate predicted
-    # within the tolerance of it; the pick is its fastest measured one
+    # the shortlist holds the predicted best class and every class predicted
+    # within the tolerance of it; the pick is the fastest measured
+    # configuration of those classes
     assert res["shortlist"] == tuner.shortlist(pred, 0.5)
     assert int(np.argmin(pred)) in res["shortlist"]
-    assert res["fell_back"] or res["pick"] in res["shortlist"]
-    assert res["measured"] == len(res["shortlist"]) + (6 if res["fell_back"] else 0)
+    assert res["fell_back"] or tuner.CLASS_OF[res["pick"]] in res["shortlist"]
+    assert res["measured"] == (tuner.NUM_CANDIDATES if res["fell_back"]
+                               else len(tuner.members(res["shortlist"])))
 
 
 CORPUS = tuner.CORPUS_PATH
@@ -121,7 +178,8 @@ def test_learned_tuner_gains_over_the_method_without_it(gpu):
     seeded 20 % split), the cost model's choice with the 5 % fallback stays
     within 5 % of the exhaustive search on the held-out meshes while
     measuring fewer candidates than the exhaustive search, and beats the
-    fixed choice (owner256 everywhere) of the method without it."""
+    fixed choice (the medium owner parts everywhere) of the method without
+    it."""
     import os
     if not os.path.exists(CORPUS):
         pytest.skip("no autotuner corpus committed yet")
@@ -129,7 +187,7 @@ def test_learned_tuner_gains_over_the_method_without_it(gpu):
     test = np.random.default_rng(0).random(len(rows)) < 0.2
     full = np.all(np.isfinite(T), 1)
     model = tuner.CostModel(seed=0)
-    losses = model.fit(X[~test], T[~test])
+    losses = model.fit(X[~test], tuner.class_times(T[~test]))
     assert losses[-1] < 0.2 * losses[0]
     r = tuner.evaluate(model, X[test & full], T[test & full])
     assert r["final_mean_slowdown"] <= 1.05, r
@@ -140,7 +198,7 @@ def test_learned_tuner_gains_over_the_method_without_it(gpu):
 @pytest.mark.gpu
 def test_learned_tuner_on_held_out_mini_aero(gpu):
     """Trained on the whole corpus (mini_aero up to 44^3 only), evaluated on
-    mini_aero 48^3 .. 96^3 in three orderings: with the fallback the choice
+    mini_aero 64^3 .. 96^3 in three orderings: with the fallback the choice
     is within 5 % of the exhaustive search, with fewer measurements than it,
     and better than the fixed choice of the method without the model."""
     import os
@@ -148,26 +206,11 @@ def test_learned_tuner_on_held_out_mini_aero(gpu):
         pytest.skip("no autotuner corpus committed yet")
     _, X, T = tuner.load_corpus(CORPUS)
     _, Xh, Th = tuner.load_corpus(tuner.HELDOUT_PATH)
-    assert all(r["family"] == "mini_aero" and r["cells"] >= 3 * 47 * 48 * 48
+    assert all(r["family"] == "mini_aero" and r["cells"] >= 3 * 63 * 64 * 64
                for r in tuner.load_corpus(tuner.HELDOUT_PATH)[0])
     model = tuner.CostModel(seed=0)
-    model.fit(X, T)
+    model.fit(X, tuner.class_times(T))
     r = tuner.evaluate(model, Xh, Th)
     assert r["final_mean_slowdown"] <= 1.05, r
     assert r["measured_per_mesh"] < r["exhaustive_measured_per_mesh"], r
     assert r["final_mean_slowdown"] < r["default_mean_slowdown"], r
-
-
-def test_shortlist_rule_closed_form():
-    meas = np.array([5.0, 4.0, 3.0, 2.0, 2.1, 2.5])
-    # a near tie (within 5 %): both measured, the faster kept
-    assert tuner.shortlist([9, 9, 9, 2.05, 2.0, 3]) == [3, 4]
-    assert tuner.shortlist_pick(np.array([9, 9, 9, 2.05, 2.0, 3]), meas) == (3, False, 2)
-    # a clear winner: one measurement, as the single-pick rule
-    assert tuner.shortlist_pick(np.array([9, 9, 9, 2.0, 3, 3]), meas) == (3, False, 1)
-    # the shortlist's best misses its prediction by more than 5 %: measured search
-    assert tuner.shortlist_pick(np.array([9, 9, 9, 1.8, 1.85, 9]), meas) == (3, True, 8)
-    # nothing on the shortlist can be planned
-    m2 = meas.copy()
-    m2[0] = -1
-    assert tuner.shortlist_pick(np.array([1.0, 9, 9, 9, 9, 9]), m2) == (3, True, 7)
