@@ -367,8 +367,11 @@ def _line(metric, value, unit, steps, warmup, ms, workload, **extra):
 
 def bench_volna_batch(args):
     """64 shallow-water triangle meshes, cells log-uniform in [2e4, 4e5], one
-    persistent owner-compute launch per step; fot_batch_run timed with eager
-    launches and with hipGraph replays, beside one owner loop per mesh."""
+    persistent owner-compute launch per step on meshes renumbered by their
+    point partition; the part size is the autotuner's (the batch timed at
+    each size; --tune none takes --points-per-block); fot_batch_run timed
+    with eager launches and with hipGraph replays, beside one owner loop per
+    mesh."""
     from cuda_fot_amd import Batch, DeviceArray, Timer, capi, meshgen, synchronize, volna_problem
     from cuda_fot_amd.device import Loop
     rng = np.random.default_rng(0)
@@ -382,29 +385,56 @@ def bench_volna_batch(args):
                                    block_size=args.points_per_block))
     f = capi.functor(capi.KERNEL_VOLNA)
     tune = {"mode": args.tune}
-    if args.tune != "none":
-        # the batch runs owner-compute plans: the autotuner measures each
-        # mesh's owner configurations (parts of 128 / 256 / 512 points) and
-        # takes the fastest part size
-        from cuda_fot_amd import tuner
-        owner = tuner.class_members(f, "owner")
-        cands = tuner.candidates(f)
-        picks = []
+    # every mesh runs owner compute on its point partition, renumbered so a
+    # part's cells are contiguous (Problem.partition_points): a block's state
+    # and volumes are then contiguous runs instead of scattered lines
+    import copy
+
+    def sized(size):
+        out = []
         for p in probs:
-            t = tuner.measure(p, f, which=owner, warmup=2, steps=5)
-            best = owner[int(np.argmin(np.where(t[owner] > 0, t[owner], np.inf)))]
-            picks.append(cands[best]["bs"])
-            p.block_size = cands[best]["bs"]
-        tune.update(mode="measured", measured_candidates=len(owner) * len(probs),
-                    picks={str(k): picks.count(k) for k in sorted(set(picks))})
-    specs, keep = [], []
-    for p in probs:
-        dev = [DeviceArray.from_host(d) for d in p.point_data]
-        out = DeviceArray(p.point_data[0].shape, np.float64)
-        cdev = [DeviceArray.from_host(c) for c in p.cell_data]
-        keep.append((dev, out, cdev))
-        specs.append(dict(maps=p.maps, num_points=p.num_points, point_data=dev, out=out,
-                          cell_data=cdev, block_size=p.block_size))
+            q = copy.deepcopy(p)
+            q.partition_points(size)
+            q.block_size = size
+            out.append(q)
+        return out
+
+    def upload(ps):
+        specs, keep = [], []
+        for p in ps:
+            dev = [DeviceArray.from_host(d) for d in p.point_data]
+            out = DeviceArray(p.point_data[0].shape, np.float64)
+            cdev = [DeviceArray.from_host(c) for c in p.cell_data]
+            keep.append((dev, out, cdev))
+            specs.append(dict(maps=p.maps, num_points=p.num_points, point_data=dev, out=out,
+                              cell_data=cdev, block_size=p.block_size,
+                              point_partition=p.point_partition))
+        return specs, keep
+
+    ppb = args.points_per_block
+    if args.tune != "none":
+        # the batch's part size: the whole batch timed at each owner part
+        # size (128 / 256 / 512 points; the meshes fill the GPU together, so
+        # a size measured on one mesh alone would mislead), the fastest kept
+        from cuda_fot_amd import tuner
+        times = {}
+        for size in tuner.owner_points(f):
+            sp, kp = upload(sized(size))
+            b = Batch(f, sp)
+            b.run(2)
+            synchronize()
+            t = Timer()
+            t.start()
+            b.run(10)
+            t.stop()
+            synchronize()
+            times[str(size)] = round(t.elapsed_ms() / 10, 4)
+            b.close()
+            del sp, kp
+        ppb = int(min(times, key=times.get))
+        tune.update(mode="measured", measured_ms_per_step=times, pick=ppb)
+    probs = sized(ppb)
+    specs, keep = upload(probs)
     edges = sum(p.num_cells for p in probs)
     cells = sum(p.num_points[0] for p in probs)
     ubytes = 72 * cells + 72 * edges  # state r+w 64 + volume 8 / cell; 64 data + 8 map / edge
@@ -425,7 +455,9 @@ def bench_volna_batch(args):
     # the same meshes one owner loop (one launch) each
     loops = [Loop(f, sp["maps"], p.num_points, sp["point_data"], sp["out"], sp["cell_data"],
                   strategy=capi.STRATEGY_HIERARCHICAL, block_size=p.block_size,
-                  flags=capi.LOOP_OWNER_COMPUTE) for sp, p in zip(specs, probs)]
+                  partition=p.point_partition,
+                  flags=capi.LOOP_OWNER_COMPUTE | capi.LOOP_POINT_PARTITION)
+             for sp, p in zip(specs, probs)]
     for _ in range(args.warmup):
         for l in loops:
             l.step()
@@ -446,14 +478,16 @@ def bench_volna_batch(args):
                  f"volna batch: 64 triangle meshes, {cells} cells, {edges} edges, seed 0",
                  roofline={"bound": "hbm", "achieved": ubytes * args.steps / ev / 1e9,
                            "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                           "frac": ubytes * args.steps / ev / PEAK_HBM, "traffic": None,
+                           "frac": ubytes * args.steps / ev / PEAK_HBM,
+                           "traffic": load_workload_traffic("volna-batch", f"owner_ppb{ppb}"),
                            "algorithmic_bytes_per_step": ubytes},
                  graph_edges_per_s=edges * args.steps / gwall,
                  graph_ms_per_step=gwall / args.steps * 1e3,
                  per_mesh_launch_edges_per_s=edges * args.steps / seq,
                  tuning=tune,
-                 batch={"grid": st["grid"], "work_items": st["num_work_items"],
-                        "threads": st["threads"], "lds_bytes": st["lds_bytes"]})
+                 batch={"points_per_block": ppb, "grid": st["grid"],
+                        "work_items": st["num_work_items"], "threads": st["threads"],
+                        "lds_bytes": st["lds_bytes"]})
 
 
 def tune_res_calc(args) -> dict:
@@ -628,7 +662,14 @@ def bench_single(args, workload):
         rec["pick"] = k["name"]
         q, _ = tuner.deploy(p, f, pick)
         wall, ev, st = run(q, k["strategy"], k["flags"], k["bs"], with_stats=True)
-        variant = "tuned_" + k["name"]
+        # the same name as the explicit options give this configuration
+        if k["flags"] & capi.LOOP_OWNER_COMPUTE:
+            variant = f"owner_ppb{k['bs']}"
+        elif k["strategy"] == capi.STRATEGY_HIERARCHICAL:
+            variant = f"{ {'given': 'none'}.get(k['ordering'], k['ordering'])}_b{k['bs']}" + \
+                ("_two-phase" if k["flags"] & capi.LOOP_TWO_PHASE else "")
+        else:
+            variant = k["name"]
         cfg = {"strategy": tuner.CLASSES[k["cls"]], "block_size": k["bs"],
                "ordering": k["ordering"], "launches_per_step": st["num_launches_per_iter"],
                "block_colours": st["num_colours"], "blocks": st["num_blocks"],
@@ -700,7 +741,7 @@ def main():
         print(json.dumps(bench_copy(args)), flush=True)
         return
     if args.workload == "volna-batch":
-        args.tune = args.tune or "none"
+        args.tune = args.tune or "exhaustive"
         print(json.dumps(bench_volna_batch(args)), flush=True)
         return
     if args.workload in ("mini_aero", "hex", "lulesh", "mine", "getacc"):
