@@ -236,8 +236,11 @@ def partition_points(maps0, num_points: int, points_per_part: int,
 
 def partition_ours(maps, num_points, block_size: int, num_blocks_in_coarse: int,
                    tolerance: float = 1.001):
-    """partitionOurs (partition.cu:102-126): coarse k-way parts of
-    block_size * num_blocks_in_coarse cells, then the heuristic partitioner."""
+    """partitionOurs (partition.cu:42-66): coarse k-way parts of about
+    block_size * num_blocks_in_coarse cells, at most `tolerance` times that
+    (METIS's ubvec there), then the heuristic partitioner."""
     ind, end = cell_to_cell_csr(maps, num_points)
-    coarse, _ = kway_partition(ind, end, int(block_size * num_blocks_in_coarse))
+    coarse, _ = kway_partition(ind, end,
+                               max(1, int(np.floor(block_size * num_blocks_in_coarse *
+                                                   tolerance))))
     return heuristic_partition(ind, end, coarse, block_size)[0]

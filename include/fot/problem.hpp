@@ -145,8 +145,11 @@ template <bool SOA = false> struct Problem {
   void partitionOurs(float tolerance, MY_SIZE num_blocks_in_coarse) {
     std::vector<MY_SIZE> ind, end;
     cellGraph(ind, end);
-    (void)tolerance;  // the coarse parts are bounded by size alone here
-    const MY_SIZE coarse_size = std::max<MY_SIZE>(1, block_size * num_blocks_in_coarse);
+    // METIS's ubvec = tolerance (partition.cu:55-61) lets a coarse part
+    // exceed the average of block_size * num_blocks_in_coarse cells by that
+    // factor: the stand-in's parts are bounded by the same size
+    const MY_SIZE coarse_size = std::max<MY_SIZE>(
+        1, MY_SIZE(std::floor(double(block_size) * num_blocks_in_coarse * tolerance)));
     std::vector<MY_SIZE> coarse(mesh.numCells());
     std::uint32_t nparts = 0;
     fot::check(fot_kway_partition(ind.data(), end.data(), mesh.numCells(), coarse_size,
