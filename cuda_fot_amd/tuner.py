@@ -196,7 +196,10 @@ class _Uploaded:
                                           self.cdev, problem.soa)
 
 
-def _measure_cfgs(problem, func, cands, warmup, steps, partition=None):
+def measure_configs(problem, func, cands, warmup, steps, partition=None):
+    """ms per step of the configurations `cands` (candidates() entries) on
+    `problem` as given, `partition` passed to each (a point partition for the
+    owner ones); -1 where one cannot be planned."""
     up = _Uploaded(problem)
     cfgs = (capi.LoopConfig * len(cands))()
     keep = None if partition is None else capi.u32(partition)
@@ -223,18 +226,18 @@ def measure(problem, func: capi.Functor, which=None, warmup: int = 3,
     out = np.full(len(cands), np.nan, np.float32)
     given = [i for i in idx if cands[i]["ordering"] == "given"]
     if given:
-        out[given] = _measure_cfgs(problem, func, [cands[i] for i in given], warmup, steps)
+        out[given] = measure_configs(problem, func, [cands[i] for i in given], warmup, steps)
     ordered = None
     if any(cands[i]["ordering"] in ("gps", "bisect") for i in idx):
         ordered = copy.deepcopy(problem)
         ordered.reorder("gps")
     gps = [i for i in idx if cands[i]["ordering"] == "gps"]
     if gps:
-        out[gps] = _measure_cfgs(ordered, func, [cands[i] for i in gps], warmup, steps)
+        out[gps] = measure_configs(ordered, func, [cands[i] for i in gps], warmup, steps)
     for i in idx:
         if cands[i]["ordering"] in ("bisect", "points"):
             q, part = deploy(problem, func, i, ordered)
-            out[i] = _measure_cfgs(q, func, [cands[i]], warmup, steps, part)[0]
+            out[i] = measure_configs(q, func, [cands[i]], warmup, steps, part)[0]
     return out
 
 
