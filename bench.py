@@ -79,6 +79,9 @@ def parse():
     ap.add_argument("--hbm-lists", action="store_true",
                     help="owner compute, gather: load the owned cells' edge lists from the "
                          "plan in HBM instead of building them in LDS")
+    ap.add_argument("--output", default="auto", choices=["auto", "nontemporal", "cached"],
+                    help="owner compute, gather: res read and written with nontemporal hints "
+                         "(auto: when the step's data exceed the 256 MB Infinity Cache)")
     ap.add_argument("--point-terms", action="store_true",
                     help="owner compute: 1/rho and pressure once per staged cell")
     ap.add_argument("--nodes", default="gather", choices=["gather", "stage"],
@@ -124,7 +127,8 @@ def build_problem(args, partition_kind):
 
 
 def make_loop(p, strategy, block_size, stage, graph, order="launch", ept=1, gather=False,
-              point_terms=False, owner_gather=False, persistent=False, hbm_lists=False):
+              point_terms=False, owner_gather=False, persistent=False, hbm_lists=False,
+              output="auto"):
     from cuda_fot_amd import DeviceArray, Loop, capi
     strat = {"atomic": capi.STRATEGY_ATOMIC, "global": capi.STRATEGY_GLOBAL_COLOUR,
              "hierarchical": capi.STRATEGY_HIERARCHICAL}[strategy]
@@ -145,6 +149,8 @@ def make_loop(p, strategy, block_size, stage, graph, order="launch", ept=1, gath
         flags |= capi.LOOP_PERSISTENT
     if hbm_lists:
         flags |= capi.LOOP_LISTS_IN_HBM
+    flags |= {"auto": 0, "nontemporal": capi.LOOP_OUTPUT_NONTEMPORAL,
+              "cached": capi.LOOP_OUTPUT_CACHED}[output]
     part = p.partition_vector if strat == capi.STRATEGY_HIERARCHICAL else None
     if strat == capi.STRATEGY_HIERARCHICAL and order == "owner" and p.point_partition is not None:
         part = p.point_partition
@@ -227,7 +233,7 @@ def bench_copy(args):
                       "Infinity Cache)")
 
 
-def load_traffic(args):
+def load_traffic(args, stats):
     """HBM bytes per launch from the committed rocprofv3 counter summary
     (FETCH_SIZE and WRITE_SIZE, scaled by a copy-kernel calibration) when it
     was taken on this configuration; else None."""
@@ -246,6 +252,7 @@ def load_traffic(args):
             args.accumulate == c.get("accumulate", "colours") and
             args.persistent == c.get("persistent", False) and
             ("hbm" if args.hbm_lists else "lds") == c.get("lists", "hbm") and
+            stats["output_nontemporal"] == c.get("output_nontemporal", 0) and
             args.soa == c.get("soa", False))
     return t["traffic_bytes"] if same else None
 
@@ -755,7 +762,8 @@ def main():
     mesh, p = build_problem(args, args.partition)
     loop, dev, out = make_loop(p, args.strategy, args.block_size, not args.no_stage,
                                args.graph, args.order, args.ept, args.nodes == "gather", args.point_terms,
-                               args.accumulate == "gather", args.persistent, args.hbm_lists)
+                               args.accumulate == "gather", args.persistent, args.hbm_lists,
+                               args.output)
     plan_s = time.perf_counter() - t_plan
     stats = loop.stats()
     wall, ev = time_loop(loop, args.steps, args.warmup)
@@ -767,7 +775,7 @@ def main():
     # launch duration, from HIP events bracketing the K timed launches (the
     # per-launch event pairs are reported too; their own packets add ~1 us)
     achieved = ubytes * args.steps / ev
-    traffic = load_traffic(args)
+    traffic = load_traffic(args, stats)
     result = {
         "metric": "res_calc edges/s, achieved HBM GB/s (% of peak); learned-tuned vs plain atomics",
         "value": ne * args.steps / wall,
@@ -801,6 +809,7 @@ def main():
                    "stage_per_thread": stats["stage_per_thread"],
                    "persistent": args.persistent, "launch_grid": stats["launch_grid"],
                    "lists": "hbm" if args.hbm_lists else "lds",
+                   "output_nontemporal": stats["output_nontemporal"],
                    "plan_seconds": round(plan_s, 2), "tune": tune},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,

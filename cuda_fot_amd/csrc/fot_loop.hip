@@ -22,6 +22,9 @@ using namespace fot;
 namespace {
 
 constexpr uint32_t kNumXcd = 8; // MI355X: 8 XCDs of 32 CUs
+// MI355X's Infinity Cache (MALL): the memory-side cache in front of HBM
+constexpr uint64_t kInfinityCacheBytes = uint64_t(256) << 20;
+typedef double dvec2 __attribute__((ext_vector_type(2)));
 
 // Workgroups are dispatched round-robin over the XCDs; this bijection gives
 // each XCD a contiguous run of plan blocks, so neighbouring blocks (which
@@ -863,7 +866,11 @@ FOT_DEV void owner3_load_ids(Owner3Ids<EPT, KS, KO> &id, const OwnerMeta &mt, co
 // every id is a valid point (point 0 past the block's end): unconditional.
 // res is read for owned slots only; the others re-read the first owned cell
 // (a line the block reads anyway) instead of a halo cell's res
-template <bool SOA, int EPT, int KS, int KO>
+// NT: the owned res values are loaded (and stored by owner3_body) with
+// nontemporal hints, so that on a working set beyond the Infinity Cache the
+// read-once, write-once res does not evict the x and q values other blocks
+// read again
+template <bool SOA, int EPT, int KS, int KO, bool NT = false>
 FOT_DEV void owner3_load_data(Owner3Data<EPT, KS, KO> &d, const Owner3Ids<EPT, KS, KO> &id,
                               const OwnerMeta &mt, const EngineArgs &a, uint32_t tid,
                               uint32_t bd) {
@@ -878,15 +885,25 @@ FOT_DEV void owner3_load_data(Owner3Data<EPT, KS, KO> &d, const Owner3Ids<EPT, K
       d.vq0[k] = make_double2(q[g], q[size_t(nq) + g]);
       d.vq1[k] = make_double2(q[2 * size_t(nq) + g], q[3 * size_t(nq) + g]);
       if (k < KO) {
-        d.vr0[k] = make_double2(a.out[gr], a.out[size_t(np) + gr]);
-        d.vr1[k] = make_double2(a.out[2 * size_t(np) + gr], a.out[3 * size_t(np) + gr]);
+        double r[4];
+#pragma unroll
+        for (uint32_t c = 0; c < 4; ++c)
+          r[c] = NT ? __builtin_nontemporal_load(&a.out[c * size_t(np) + gr])
+                    : a.out[c * size_t(np) + gr];
+        d.vr0[k] = make_double2(r[0], r[1]), d.vr1[k] = make_double2(r[2], r[3]);
       }
     } else {
       const double2 *q2p = reinterpret_cast<const double2 *>(q);
       d.vq0[k] = q2p[2 * size_t(g)], d.vq1[k] = q2p[2 * size_t(g) + 1];
       if (k < KO) {
-        const double2 *o = reinterpret_cast<const double2 *>(a.out) + 2 * size_t(gr);
-        d.vr0[k] = o[0], d.vr1[k] = o[1];
+        if (NT) {
+          const dvec2 *o = reinterpret_cast<const dvec2 *>(a.out) + 2 * size_t(gr);
+          const dvec2 r0 = __builtin_nontemporal_load(o), r1 = __builtin_nontemporal_load(o + 1);
+          d.vr0[k] = make_double2(r0.x, r0.y), d.vr1[k] = make_double2(r1.x, r1.y);
+        } else {
+          const double2 *o = reinterpret_cast<const double2 *>(a.out) + 2 * size_t(gr);
+          d.vr0[k] = o[0], d.vr1[k] = o[1];
+        }
       }
     }
     d.vadt[k] = adt[g];
@@ -910,7 +927,7 @@ FOT_DEV void owner3_load_data(Owner3Data<EPT, KS, KO> &d, const Owner3Ids<EPT, K
 // With PRE, 1/rho and the pressure of each staged cell are computed once at
 // staging (LDS) instead of once per edge side: the same operations on the
 // same values, the same bits.
-template <bool SOA, int EPT, int KS, int LW, int KO, bool PRE>
+template <bool SOA, int EPT, int KS, int LW, int KO, bool PRE, bool NT = false>
 FOT_DEV void owner3_body(double *lds, Owner3Data<EPT, KS, KO> &d, const Owner3Ids<EPT, KS, KO> &id,
                          const OwnerMeta &mt, const EngineArgs &a, uint32_t tid, uint32_t bd) {
   const uint32_t m = mt.ncached, nw = mt.nowned, ne = mt.ncells;
@@ -1013,7 +1030,16 @@ FOT_DEV void owner3_body(double *lds, Owner3Data<EPT, KS, KO> &d, const Owner3Id
       const uint32_t g = id.gq[k];
       if (SOA) {
 #pragma unroll
-        for (uint32_t c = 0; c < 4; ++c) a.out[size_t(c) * np + g] = r[c];
+        for (uint32_t c = 0; c < 4; ++c) {
+          if (NT)
+            __builtin_nontemporal_store(r[c], &a.out[size_t(c) * np + g]);
+          else
+            a.out[size_t(c) * np + g] = r[c];
+        }
+      } else if (NT) {
+        dvec2 *o = reinterpret_cast<dvec2 *>(a.out) + 2 * size_t(g);
+        __builtin_nontemporal_store(dvec2{r[0], r[1]}, o);
+        __builtin_nontemporal_store(dvec2{r[2], r[3]}, o + 1);
       } else {
         double2 *o = reinterpret_cast<double2 *>(a.out) + 2 * size_t(g);
         o[0] = make_double2(r[0], r[1]);
@@ -1039,6 +1065,23 @@ __global__ __launch_bounds__(1024) void res_calc_owner3_kernel(EngineArgs a,
   owner3_load_ids<EPT, KS, LW, KO>(id, mt, a, cached, packed, plist, tid, bd);
   owner3_load_data<SOA, EPT, KS, KO>(d, id, mt, a, tid, bd);
   owner3_body<SOA, EPT, KS, LW, KO, PRE>(lds, d, id, mt, a, tid, bd);
+}
+
+// The same with the owned res loaded and stored nontemporally (edge lists
+// built in LDS, no per-cell terms): the same bits.
+template <bool SOA, int EPT, int KS, int KO>
+__global__ __launch_bounds__(1024) void res_calc_owner3_stream_kernel(
+    EngineArgs a, const OwnerMeta *__restrict__ meta, const uint32_t *__restrict__ cached,
+    const uint32_t *__restrict__ packed, const uint32_t *__restrict__ plist, uint32_t nb) {
+  extern __shared__ double lds[];
+  const uint32_t b = xcd_block(blockIdx.x, nb);
+  const OwnerMeta mt = meta[b];
+  const uint32_t tid = threadIdx.x, bd = blockDim.x;
+  Owner3Ids<EPT, KS, KO> id;
+  Owner3Data<EPT, KS, KO> d;
+  owner3_load_ids<EPT, KS, 0, KO>(id, mt, a, cached, packed, plist, tid, bd);
+  owner3_load_data<SOA, EPT, KS, KO, true>(d, id, mt, a, tid, bd);
+  owner3_body<SOA, EPT, KS, 0, KO, false, true>(lds, d, id, mt, a, tid, bd);
 }
 
 // Persistent form: gridDim.x workgroups (a multiple of 8, at most the
@@ -1092,7 +1135,10 @@ const void *owner3_pick(bool persistent, bool pre) {
              : reinterpret_cast<const void *>(&res_calc_owner3_kernel<SOA, E, K, W, O, false>);
 }
 template <bool SOA, int E, int K>
-const void *owner3_pick_w(bool persistent, bool pre, uint32_t lw, bool ko1) {
+const void *owner3_pick_w(bool persistent, bool pre, uint32_t lw, bool ko1, bool nt) {
+  if (lw == 0 && nt && !persistent && !pre)
+    return ko1 ? reinterpret_cast<const void *>(&res_calc_owner3_stream_kernel<SOA, E, K, 1>)
+               : reinterpret_cast<const void *>(&res_calc_owner3_stream_kernel<SOA, E, K, K>);
   if (lw == 0)
     return ko1 ? owner3_pick<SOA, E, K, 0, 1>(persistent, pre)
                : owner3_pick<SOA, E, K, 0, K>(persistent, pre);
@@ -1103,19 +1149,23 @@ const void *owner3_pick_w(bool persistent, bool pre, uint32_t lw, bool ko1) {
              : owner3_pick<SOA, E, K, 4, K>(persistent, pre);
 }
 template <bool SOA, int E>
-const void *owner3_pick_k(bool persistent, bool pre, uint32_t ks, uint32_t lw, bool ko1) {
+const void *owner3_pick_k(bool persistent, bool pre, uint32_t ks, uint32_t lw, bool ko1,
+                          bool nt) {
   switch (ks) {
-  case 1: return owner3_pick_w<SOA, E, 1>(persistent, pre, lw, ko1);
-  case 2: return owner3_pick_w<SOA, E, 2>(persistent, pre, lw, ko1);
-  case 3: return owner3_pick_w<SOA, E, 3>(persistent, pre, lw, ko1);
-  default: return owner3_pick_w<SOA, E, 4>(persistent, pre, lw, ko1);
+  case 1: return owner3_pick_w<SOA, E, 1>(persistent, pre, lw, ko1, nt);
+  case 2: return owner3_pick_w<SOA, E, 2>(persistent, pre, lw, ko1, nt);
+  case 3: return owner3_pick_w<SOA, E, 3>(persistent, pre, lw, ko1, nt);
+  default: return owner3_pick_w<SOA, E, 4>(persistent, pre, lw, ko1, nt);
   }
 }
+// nt: the streaming-output instantiation, which exists for the default form
+// only (one block per workgroup, lists in LDS, no per-cell terms); the other
+// forms ignore it
 template <bool SOA>
 const void *owner3_pick_kernel(bool persistent, bool pre, uint32_t ept, uint32_t ks, uint32_t lw,
-                               bool ko1) {
-  return ept == 2 ? owner3_pick_k<SOA, 2>(persistent, pre, ks, lw, ko1)
-                  : owner3_pick_k<SOA, 1>(persistent, pre, ks, lw, ko1);
+                               bool ko1, bool nt) {
+  return ept == 2 ? owner3_pick_k<SOA, 2>(persistent, pre, ks, lw, ko1, nt)
+                  : owner3_pick_k<SOA, 1>(persistent, pre, ks, lw, ko1, nt);
 }
 
 // dst[d * n_dst + i] = src[d * n_src + idx[i]] (SOA gather of cell data)
@@ -1233,6 +1283,7 @@ struct fot_loop {
   uint32_t stage_per_thread = 1; // owner res_calc: staged points per thread
   bool owned_first_round = false; // owner res_calc: every block owns <= threads cells
   const void *owner3_fn = nullptr; // owner gather: the kernel instantiation to launch
+  bool output_nt = false;          // owner gather: res loaded and stored nontemporally
   uint32_t launch_grid = 0;        // workgroups of the single-launch strategies
   uint32_t ept = 1;
   DepDev dep{};
@@ -1834,10 +1885,22 @@ fot_status fot_loop::build_hierarchical() {
   if (owner && f.kernel == FOT_KERNEL_RES_CALC && owner_gather) {
     const bool persistent = (cfg.flags & FOT_LOOP_PERSISTENT) != 0;
     const uint32_t lw = lists_in_lds ? 0u : list_words;
+    // nontemporal res when asked, or by default when the step's data (the
+    // point arrays, res and the plan) exceed the Infinity Cache
+    uint64_t step_bytes = stats.plan_device_bytes;
+    for (uint32_t k = 0; k < desc.num_mappings; ++k)
+      if (desc.point_data[k])
+        step_bytes += uint64_t(desc.num_points[k]) * desc.point_dim[k] * sizeof(double);
+    step_bytes += uint64_t(desc.num_points[0]) * desc.point_dim[0] * sizeof(double); // out
+    output_nt = (cfg.flags & FOT_LOOP_OUTPUT_NONTEMPORAL) != 0 ||
+                (!(cfg.flags & FOT_LOOP_OUTPUT_CACHED) && step_bytes > kInfinityCacheBytes);
+    output_nt = output_nt && lw == 0 && !persistent && !point_terms;
     owner3_fn = desc.soa ? owner3_pick_kernel<true>(persistent, point_terms, ept,
-                                                    stage_per_thread, lw, owned_first_round)
+                                                    stage_per_thread, lw, owned_first_round,
+                                                    output_nt)
                          : owner3_pick_kernel<false>(persistent, point_terms, ept,
-                                                     stage_per_thread, lw, owned_first_round);
+                                                     stage_per_thread, lw, owned_first_round,
+                                                     output_nt);
     if (persistent && nb_total) {
       // as many workgroups as are resident at once, a multiple of the 8 XCDs
       int dev = 0, cus = 0, per_cu = 0;
@@ -1860,6 +1923,7 @@ fot_status fot_loop::build_hierarchical() {
   stats.threads_per_block = threads;
   stats.stage_per_thread = stage_per_thread;
   stats.launch_grid = launch_grid;
+  stats.output_nontemporal = output_nt ? 1u : 0u;
   stats.max_thread_colours = max_tc;
   stats.avg_thread_colours = nb_total ? tc_sum / nb_total : 0.0;
   // (cached may carry padding at its end: count the plan's entries only)

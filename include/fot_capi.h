@@ -532,6 +532,17 @@ typedef struct {
  * in their order: the same bits.  No reference counterpart (the reference
  * launches once per block colour, problem.hpp:276-303). */
 #define FOT_LOOP_TWO_PHASE 1024u
+/* owner compute, res_calc gather (one block per workgroup, lists built in
+ * LDS, no per-cell terms): the owned output values are read and written
+ * with nontemporal hints (the same bits).  By default this is on when the
+ * step's data (point arrays, output and plan) exceed the 256 MB Infinity
+ * Cache, where it keeps the read-once output from evicting the point data
+ * other blocks read again, and off below, where it slows the step;
+ * FOT_LOOP_OUTPUT_NONTEMPORAL forces it on and FOT_LOOP_OUTPUT_CACHED off.
+ * fot_loop_stats.output_nontemporal reports the choice.  No reference
+ * counterpart. */
+#define FOT_LOOP_OUTPUT_NONTEMPORAL 2048u
+#define FOT_LOOP_OUTPUT_CACHED 4096u
 
 typedef struct fot_loop fot_loop;
 /* Builds the plan (host), uploads it, permutes cell data. */
@@ -562,6 +573,9 @@ typedef struct {
   uint32_t stage_per_thread;
   /* single-launch strategies: workgroups per launch */
   uint32_t launch_grid;
+  /* owner res_calc gather: 1 when the output is read and written with
+   * nontemporal hints (FOT_LOOP_OUTPUT_NONTEMPORAL) */
+  uint32_t output_nontemporal;
 } fot_loop_stats;
 fot_status fot_loop_stats_get(const fot_loop *loop, fot_loop_stats *stats);
 

@@ -1,7 +1,8 @@
 """res_calc at the benchmark's own sizes: the shuffled 1200x600 airfoil
 (1 438 200 edges, the headline) and 2400x1200 (5 756 400 edges), built as
 bench.py builds them, run by the default plan (owner compute on 240-point
-parts, the gather kernel, two edges per thread).  Two steps, checked bit for
+parts, the gather kernel, two edges per thread; at 2400x1200 its res loads
+and stores nontemporal).  Two steps, checked bit for
 bit against the oracle's replay of the same plan and within 1e-12 * S of the
 serial oracle loop, plus the conservation that holds at any size: every edge
 adds +f to one cell and -f to the other, so the increments sum to zero up to
@@ -29,7 +30,10 @@ def test_res_calc_at_benchmark_size(gpu, nx, ny):
     p = res_calc_problem(mesh)
     p.partition_points(ppb)
     ref = copy.deepcopy(p)
-    p.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, iters, block_size=ppb, flags=FLAGS)
+    stats = p.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, iters, block_size=ppb, flags=FLAGS)
+    # nontemporal res by default only where the step's data exceed the
+    # 256 MB Infinity Cache (2400x1200: about 420 MB; 1200x600: about 105 MB)
+    assert stats["output_nontemporal"] == (1 if nx * ny > 2_000_000 else 0)
     got = p.point_data[0]
     o = oracle_of(ref, O.RES_CALC)
     O.loop_owner_replay(o, plan.owner_plan(ref.maps[0], ref.num_points[0], ppb, None,

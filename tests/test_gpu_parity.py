@@ -68,6 +68,7 @@ OG = capi.LOOP_OWNER_GATHER
 HBM = capi.LOOP_LISTS_IN_HBM
 PERS = capi.LOOP_PERSISTENT
 TP = capi.LOOP_TWO_PHASE
+NT = capi.LOOP_OUTPUT_NONTEMPORAL
 
 
 def _owner_replay(p, kernel, block_size, partition=None, iters=ITERS, point_partition=None):
@@ -112,7 +113,13 @@ def _owner_replay(p, kernel, block_size, partition=None, iters=ITERS, point_part
                                                  (False, 128, OWN | OG | PERS | HBM, "points"),
                                                  (True, 192, OWN | TWO | OG | TERMS | HBM,
                                                   "points"),
-                                                 (True, 512, OWN | OG | HBM, "kway")])
+                                                 (True, 512, OWN | OG | HBM, "kway"),
+                                                 # res loaded and stored nontemporally
+                                                 (False, 240, OWN | TWO | OG | NT, "points"),
+                                                 (True, 192, OWN | TWO | OG | NT, "points"),
+                                                 (False, 128, OWN | OG | NT, "points"),
+                                                 (True, 512, OWN | OG | NT, "kway"),
+                                                 (False, 256, OWN | TWO | OG | NT, None)])
 def test_res_calc_owner_compute_bitwise(gpu, soa, bs, flags, method):
     p = airfoil_problem(nx=64, ny=32, soa=soa, res0_seed=10, block_size=bs)
     orig = airfoil_problem(nx=64, ny=32, soa=soa, res0_seed=10, block_size=bs)
@@ -128,9 +135,13 @@ def test_res_calc_owner_compute_bitwise(gpu, soa, bs, flags, method):
         p.partition_points(bs)
     ref = _owner_replay(p, O.RES_CALC, bs, p.partition_vector, point_partition=p.point_partition)
     serial, S = _serial(orig, O.RES_CALC)
-    p.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, ITERS, flags=flags)
+    stats = p.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, ITERS, flags=flags)
     assert_bitwise(p.point_data[0], ref, "owner compute")
     assert_close_scaled(p.target_in_original_order(), serial, S, what="owner compute vs serial")
+    # the nontemporal form exists for the gather kernel with lists in LDS,
+    # one block per workgroup and no per-cell terms
+    nt = bool(flags & NT) and bool(flags & OG) and not flags & (PERS | HBM | TERMS)
+    assert stats["output_nontemporal"] == int(nt)
 
 
 def test_res_calc_gather_edge_on_one_cell_twice(gpu):
