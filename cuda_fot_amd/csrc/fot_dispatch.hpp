@@ -60,15 +60,18 @@ fot_status dispatch(const fot_functor *f, int soa, Body &&body) {
 
 // Launch bound of the instantiations that run a functor's small blocks.
 // Under a 1024-thread bound a kernel may use 128 VGPRs, where mini_aero's
-// face flux spills about 200 B per lane; under 512 it may use 256, and the
-// face flux needs about 170-195.  Hierarchical and owner blocks of at most
+// face flux spills up to 100 B per lane (none in the AOS two-phase and
+// per-colour forms); under 512 it may use 256, and the face flux needs
+// about 145-165.  Hierarchical and owner blocks of at most
 // this many threads run a kernel bounded at it, and direct launches (whose
 // results do not depend on the block size) always do.  (lulesh's element
 // forces, at the cap too, measured the same at 512.)
 template <class F>
 constexpr int kSmallBound = std::is_same_v<F, MiniAeroF> ? 512 : 1024;
 // Waves per SIMD the small-bound hierarchical instantiation is compiled for
-// (at most 512 / waves VGPRs each).
+// (at most 512 / waves VGPRs each).  mini_aero at 4 (128 VGPRs, spill-free
+// in AOS) measured 5-10 % slower than at 3 on blocks of 128 and 256 faces,
+// faster only on 512.
 template <class F>
 constexpr int kSmallBoundWaves = std::is_same_v<F, MiniAeroF> ? 3 : 1;
 

@@ -360,7 +360,9 @@ FOT_HD void face_flux(const double *L, const double *R, uint32_t s, const double
   }
   double n[3], t[3], b[3];
   for (int d = 0; d < 3; ++d) n[d] = fd[(3 + d) * fs], t[d] = fd[(6 + d) * fs], b[d] = fd[(9 + d) * fs];
-  roe_flux(pl, pr, n, t, b, flux);
+  // the viscous flux (independent of the Roe flux) first, so that the
+  // gradients are dead before the Roe flux: the same operations, fewer
+  // values live at once
   double pf[5], gf[5][3];
   for (int k = 0; k < 5; ++k) {
     pf[k] = 0.5 * (pl[k] + pr[k]);
@@ -368,6 +370,7 @@ FOT_HD void face_flux(const double *L, const double *R, uint32_t s, const double
   }
   double vf[5];
   viscous_flux(gf, pf, n, vf);
+  roe_flux(pl, pr, n, t, b, flux);
   for (int k = 0; k < 5; ++k) flux[k] -= vf[k];
 }
 
