@@ -55,10 +55,12 @@ def parse():
     ap.add_argument("--ny", type=int, default=600)
     ap.add_argument("--strategy", default="hierarchical",
                     choices=["atomic", "global", "hierarchical"])
-    ap.add_argument("--block-size", type=int, default=512)
-    ap.add_argument("--partition", default="points",
+    ap.add_argument("--block-size", type=int, default=None,
+                    help="default 512; hex 512 and mini_aero 128 (their tuned picks)")
+    ap.add_argument("--partition", default=None,
                     choices=["none", "kway", "bisect", "ours", "gps", "points"],
-                    help="points: owner partition of mapping-0 points (owner compute)")
+                    help="points: owner partition of mapping-0 points (owner compute); "
+                         "default points, hex and mini_aero bisect")
     ap.add_argument("--points-per-block", type=int, default=240)
     ap.add_argument("--numbering", default="cm", choices=["id", "cm"],
                     help="owner partition: points inside a part in their previous order "
@@ -87,11 +89,12 @@ def parse():
     ap.add_argument("--nodes", default="gather", choices=["gather", "stage"],
                     help="owner compute: gather node coordinates per edge, or stage each "
                          "block's nodes in LDS first")
-    ap.add_argument("--order", default="owner", choices=["owner", "dep", "launch", "two-phase"],
+    ap.add_argument("--order", default=None, choices=["owner", "dep", "launch", "two-phase"],
                     help="hierarchical block scheduling: owner compute in one launch (owner), "
                          "block colours ordered by completion flags in one launch (dep), "
                          "one launch per block colour (launch) or every block in one launch "
-                         "with shared points merged by a second (two-phase)")
+                         "with shared points merged by a second (two-phase); default owner, "
+                         "hex and mini_aero two-phase")
     ap.add_argument("--graph", action="store_true", help="replay each step from a hipGraph")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -103,7 +106,18 @@ def parse():
                          "block colour) at block sizes 128-1024")
     ap.add_argument("--soa", action="store_true",
                     help="res_calc: point data in SOA layout (d * n + i) instead of AOS")
-    return ap.parse_args()
+    args = ap.parse_args()
+    # hex and mini_aero default to the autotuner's pick on their meshes (the
+    # two-phase write-back on recursive-bisection blocks of 512 and 128
+    # cells); every other line to owner compute on point parts
+    part, bs, order = WORKLOAD_DEFAULTS.get(args.workload, ("points", 512, "owner"))
+    args.partition = part if args.partition is None else args.partition
+    args.block_size = bs if args.block_size is None else args.block_size
+    args.order = order if args.order is None else args.order
+    return args
+
+
+WORKLOAD_DEFAULTS = {"hex": ("bisect", 512, "two-phase"), "mini_aero": ("bisect", 128, "two-phase")}
 
 
 def build_problem(args, partition_kind):
