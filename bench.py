@@ -56,11 +56,11 @@ def parse():
     ap.add_argument("--strategy", default="hierarchical",
                     choices=["atomic", "global", "hierarchical"])
     ap.add_argument("--block-size", type=int, default=None,
-                    help="default 512; hex 512 and mini_aero 128 (their tuned picks)")
+                    help="default 512 (mini_aero 128)")
     ap.add_argument("--partition", default=None,
                     choices=["none", "kway", "bisect", "ours", "gps", "points"],
                     help="points: owner partition of mapping-0 points (owner compute); "
-                         "default points, hex and mini_aero bisect")
+                         "default points (res_calc, volna-batch), else bisect")
     ap.add_argument("--points-per-block", type=int, default=240)
     ap.add_argument("--numbering", default="cm", choices=["id", "cm"],
                     help="owner partition: points inside a part in their previous order "
@@ -93,8 +93,8 @@ def parse():
                     help="hierarchical block scheduling: owner compute in one launch (owner), "
                          "block colours ordered by completion flags in one launch (dep), "
                          "one launch per block colour (launch) or every block in one launch "
-                         "with shared points merged by a second (two-phase); default owner, "
-                         "hex and mini_aero two-phase")
+                         "with shared points merged by a second (two-phase); default owner "
+                         "(res_calc, volna-batch), else two-phase")
     ap.add_argument("--graph", action="store_true", help="replay each step from a hipGraph")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -107,9 +107,10 @@ def parse():
     ap.add_argument("--soa", action="store_true",
                     help="res_calc: point data in SOA layout (d * n + i) instead of AOS")
     args = ap.parse_args()
-    # hex and mini_aero default to the autotuner's pick on their meshes (the
-    # two-phase write-back on recursive-bisection blocks of 512 and 128
-    # cells); every other line to owner compute on point parts
+    # the 3D element loops, getacc and mine default to the two-phase
+    # write-back on recursive-bisection blocks (the autotuner's pick on hex
+    # and mini_aero; measured fastest on lulesh, getacc and mine), res_calc
+    # and the volna batch to owner compute on point parts
     part, bs, order = WORKLOAD_DEFAULTS.get(args.workload, ("points", 512, "owner"))
     args.partition = part if args.partition is None else args.partition
     args.block_size = bs if args.block_size is None else args.block_size
@@ -117,7 +118,9 @@ def parse():
     return args
 
 
-WORKLOAD_DEFAULTS = {"hex": ("bisect", 512, "two-phase"), "mini_aero": ("bisect", 128, "two-phase")}
+WORKLOAD_DEFAULTS = {"hex": ("bisect", 512, "two-phase"), "mini_aero": ("bisect", 128, "two-phase"),
+                     "lulesh": ("bisect", 512, "two-phase"), "getacc": ("bisect", 512, "two-phase"),
+                     "mine": ("bisect", 512, "two-phase")}
 
 
 def build_problem(args, partition_kind):
@@ -614,6 +617,9 @@ def bench_single(args, workload):
         name = f"hex scatter {n}^3 elements, 8 corners x 3 forces, shuffled, seed 0"
     from cuda_fot_amd import DeviceArray
     from cuda_fot_amd.device import Loop
+    import copy
+    # mine's CPU loops run on the grid as generated, before any reordering
+    p_cpu = copy.deepcopy(p) if workload == "mine" and not args.no_cpu else None
 
     def run(prob, strategy, flags, bs, with_stats=False):
         dev = [DeviceArray.from_host(d) for d in prob.point_data]
@@ -638,7 +644,6 @@ def bench_single(args, workload):
         # atomics on the mesh as generated; the hierarchical loop (LDS-cached
         # point data, thread colours, one launch per block colour) on the
         # GPS-ordered mesh with consecutive blocks
-        import copy
         g = copy.deepcopy(p)
         g.reorder("gps")
         sweep = {}
@@ -732,7 +737,7 @@ def bench_single(args, workload):
     if wa is not None:
         extra.update(atomic_per_s=units * args.steps / wa, speedup_vs_atomic=wa / wall)
     if workload == "mine" and not args.no_cpu:
-        extra["cpu_baseline"] = cpu_baseline_mine(p, f, args.cpu_seconds)
+        extra["cpu_baseline"] = cpu_baseline_mine(p_cpu, f, args.cpu_seconds)
     line = _line(f"{workload} {unit}", units * args.steps / wall, unit, args.steps, args.warmup,
                  wall / args.steps * 1e3, name,
                  roofline={"bound": "hbm", "achieved": ubytes * args.steps / ev / 1e9,
