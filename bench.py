@@ -81,9 +81,10 @@ def parse():
     ap.add_argument("--hbm-lists", action="store_true",
                     help="owner compute, gather: load the owned cells' edge lists from the "
                          "plan in HBM instead of building them in LDS")
-    ap.add_argument("--output", default="auto", choices=["auto", "nontemporal", "cached"],
-                    help="owner compute, gather: res read and written with nontemporal hints "
-                         "(auto: when the step's data exceed the 256 MB Infinity Cache)")
+    ap.add_argument("--nontemporal", default="auto", choices=["auto", "on", "off"],
+                    help="nontemporal hints on read-once data (res_calc owner gather: the "
+                         "owned res; hex and mini_aero two-phase: the cell data); auto: when "
+                         "the step's data exceed the 256 MB Infinity Cache")
     ap.add_argument("--point-terms", action="store_true",
                     help="owner compute: 1/rho and pressure once per staged cell")
     ap.add_argument("--nodes", default="gather", choices=["gather", "stage"],
@@ -145,7 +146,7 @@ def build_problem(args, partition_kind):
 
 def make_loop(p, strategy, block_size, stage, graph, order="launch", ept=1, gather=False,
               point_terms=False, owner_gather=False, persistent=False, hbm_lists=False,
-              output="auto"):
+              nontemporal="auto"):
     from cuda_fot_amd import DeviceArray, Loop, capi
     strat = {"atomic": capi.STRATEGY_ATOMIC, "global": capi.STRATEGY_GLOBAL_COLOUR,
              "hierarchical": capi.STRATEGY_HIERARCHICAL}[strategy]
@@ -166,8 +167,7 @@ def make_loop(p, strategy, block_size, stage, graph, order="launch", ept=1, gath
         flags |= capi.LOOP_PERSISTENT
     if hbm_lists:
         flags |= capi.LOOP_LISTS_IN_HBM
-    flags |= {"auto": 0, "nontemporal": capi.LOOP_OUTPUT_NONTEMPORAL,
-              "cached": capi.LOOP_OUTPUT_CACHED}[output]
+    flags |= nt_flags(nontemporal)
     part = p.partition_vector if strat == capi.STRATEGY_HIERARCHICAL else None
     if strat == capi.STRATEGY_HIERARCHICAL and order == "owner" and p.point_partition is not None:
         part = p.point_partition
@@ -269,20 +269,27 @@ def load_traffic(args, stats):
             args.accumulate == c.get("accumulate", "colours") and
             args.persistent == c.get("persistent", False) and
             ("hbm" if args.hbm_lists else "lds") == c.get("lists", "hbm") and
-            stats["output_nontemporal"] == c.get("output_nontemporal", 0) and
+            stats["nontemporal"] == c.get("nontemporal", 0) and
             args.soa == c.get("soa", False))
     return t["traffic_bytes"] if same else None
 
 
-def load_workload_traffic(workload: str, variant: str):
+def nt_flags(mode: str) -> int:
+    """The loop flags of --nontemporal auto / on / off."""
+    from cuda_fot_amd import capi
+    return {"auto": 0, "on": capi.LOOP_NONTEMPORAL, "off": capi.LOOP_CACHED}[mode]
+
+
+def load_workload_traffic(workload: str, variant: str, nontemporal: int = 0):
     """HBM bytes per step of a secondary workload's line from its committed
-    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE summed over the
-    step's launches, scaled by the copy-kernel calibration), or None when no
-    summary was taken for this variant."""
+    rocprofv3 counter summary (the L2-to-fabric request counters summed over
+    the step's launches), or None when no summary was taken for this variant
+    with the same nontemporal setting."""
     path = os.path.join(ROOT, "profiles", f"{workload}_{variant}_traffic.json")
     if not os.path.exists(path):
         return None
-    return json.load(open(path))["traffic_bytes"]
+    t = json.load(open(path))
+    return t["traffic_bytes"] if t.get("nontemporal", 0) == nontemporal else None
 
 
 def cpu_model() -> str:
@@ -622,6 +629,7 @@ def bench_single(args, workload):
     p_cpu = copy.deepcopy(p) if workload == "mine" and not args.no_cpu else None
 
     def run(prob, strategy, flags, bs, with_stats=False):
+        flags |= nt_flags(args.nontemporal)
         dev = [DeviceArray.from_host(d) for d in prob.point_data]
         out = DeviceArray.from_host(prob.point_data[0])
         cdev = [DeviceArray.from_host(c) for c in prob.cell_data]
@@ -743,10 +751,11 @@ def bench_single(args, workload):
                  roofline={"bound": "hbm", "achieved": ubytes * args.steps / ev / 1e9,
                            "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                            "frac": ubytes * args.steps / ev / PEAK_HBM,
-                           "traffic": load_workload_traffic(workload, variant),
+                           "traffic": load_workload_traffic(workload, variant,
+                                                            st["nontemporal"]),
                            "algorithmic_bytes_per_step": ubytes},
                  **extra)
-    line["config"].update(cfg)
+    line["config"].update(cfg, nontemporal=st["nontemporal"])
     if sweep is not None:
         line["block_size_sweep"] = sweep
     return line
@@ -782,7 +791,7 @@ def main():
     loop, dev, out = make_loop(p, args.strategy, args.block_size, not args.no_stage,
                                args.graph, args.order, args.ept, args.nodes == "gather", args.point_terms,
                                args.accumulate == "gather", args.persistent, args.hbm_lists,
-                               args.output)
+                               args.nontemporal)
     plan_s = time.perf_counter() - t_plan
     stats = loop.stats()
     wall, ev = time_loop(loop, args.steps, args.warmup)
@@ -828,7 +837,7 @@ def main():
                    "stage_per_thread": stats["stage_per_thread"],
                    "persistent": args.persistent, "launch_grid": stats["launch_grid"],
                    "lists": "hbm" if args.hbm_lists else "lds",
-                   "output_nontemporal": stats["output_nontemporal"],
+                   "nontemporal": stats["nontemporal"],
                    "plan_seconds": round(plan_s, 2), "tune": tune},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,

@@ -60,8 +60,8 @@ LOOP_OWNER_GATHER = 128
 LOOP_PERSISTENT = 256
 LOOP_LISTS_IN_HBM = 512
 LOOP_TWO_PHASE = 1024
-LOOP_OUTPUT_NONTEMPORAL = 2048
-LOOP_OUTPUT_CACHED = 4096
+LOOP_NONTEMPORAL = 2048
+LOOP_CACHED = 4096
 
 
 class FotError(RuntimeError):
@@ -117,7 +117,7 @@ class LoopStats(C.Structure):
                 ("avg_thread_colours", C.c_double), ("reuse_factor", C.c_double),
                 ("redundancy", C.c_double), ("avg_cache_lines", C.c_double),
                 ("threads_per_block", C.c_uint32), ("stage_per_thread", C.c_uint32),
-                ("launch_grid", C.c_uint32), ("output_nontemporal", C.c_uint32)]
+                ("launch_grid", C.c_uint32), ("nontemporal", C.c_uint32)]
 
 
 class BatchStats(C.Structure):

@@ -75,6 +75,13 @@ constexpr int kSmallBound = std::is_same_v<F, MiniAeroF> ? 512 : 1024;
 template <class F>
 constexpr int kSmallBoundWaves = std::is_same_v<F, MiniAeroF> ? 3 : 1;
 
+// Functors whose cell data is read once per step and large beside the
+// point data (the hex corner forces, 192 B per element; mini_aero's face
+// values, 96 B per face): their two-phase launches have an instantiation
+// that loads it with nontemporal hints (StreamCellArgs).
+template <class F>
+constexpr bool kStreamsCells = std::is_same_v<F, HexScatterF> || std::is_same_v<F, MiniAeroF>;
+
 } // namespace fot
 
 #endif

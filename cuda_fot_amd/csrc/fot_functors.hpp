@@ -109,10 +109,12 @@ struct StepArgs {
   FOT_DEV const uint32_t *map(uint32_t k) const { return c2n[k]; }
   FOT_DEV uint32_t ps(uint32_t k) const { return point_stride[k]; }
   FOT_DEV uint32_t cs() const { return cell_stride; }
+  static constexpr bool kStreamCells = false;
 };
 
 // Arguments of the engine's launches: everything by value in the kernel
-// argument segment (no pointer chasing).
+// argument segment (no pointer chasing).  kStreamCells: the functor loads
+// its read-once cell data with nontemporal hints (StreamCellArgs).
 struct EngineArgs {
   const double *point_data[FOT_MAX_MAPPINGS];
   const double *cell_data[FOT_MAX_CELL_DATA];
@@ -126,7 +128,23 @@ struct EngineArgs {
   FOT_DEV const uint32_t *map(uint32_t k) const { return maps[k]; }
   FOT_DEV uint32_t ps(uint32_t k) const { return num_points[k]; }
   FOT_DEV uint32_t cs() const { return cell_stride; }
+  static constexpr bool kStreamCells = false;
 };
+
+// The same arguments, with the functors that stream their cell data (the
+// hex corner forces, mini_aero's face values: read once per step) loading it
+// with nontemporal hints, so that on data beyond the Infinity Cache it does
+// not evict the point data and outputs other blocks touch again.  The same
+// values: the same bits.
+struct StreamCellArgs : EngineArgs {
+  static constexpr bool kStreamCells = true;
+};
+
+// One value of read-once data: a nontemporal load when NT.
+template <bool NT> __host__ __device__ __forceinline__ double load_once(const double *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
 
 // ---- functor traits --------------------------------------------------
 // MD target slots of TD components; SLOT_OUTER selects the reference's add
@@ -196,15 +214,16 @@ struct HexScatterF {
   // GPS-ordered blocks and within 2 % (chunks of 4 and 8) on compact
   // bisection blocks with 9 thread colours
   static constexpr uint32_t ADD_CHUNK = 1;
+  template <bool NT = false>
   FOT_DEV static void user(const double *F, uint32_t cs, double *inc) {
 #pragma unroll
     for (uint32_t j = 0; j < 8; ++j)
 #pragma unroll
-      for (uint32_t i = 0; i < 3; ++i) inc[j * 3 + i] = F[size_t(i * 8 + j) * cs];
+      for (uint32_t i = 0; i < 3; ++i) inc[j * 3 + i] = load_once<NT>(&F[size_t(i * 8 + j) * cs]);
   }
   template <bool SOA, class A>
   FOT_DEV static void compute(const A &a, uint32_t ind, const uint32_t *, double *inc) {
-    user(a.cd(0) + ind, a.cs(), inc);
+    user<A::kStreamCells>(a.cd(0) + ind, a.cs(), inc);
   }
 };
 
@@ -343,6 +362,8 @@ FOT_HD void viscous_flux(const double g[5][3], const double *prim, const double 
 // flux of face `f`: cell rows L / R of 28 values (5 conservatives, 3
 // coordinates, 15 gradients, 5 limiters) with element stride s; face row
 // `fd` of 12 values (coordinates, normal, tangent, binormal), stride fs.
+// NT: the face values loaded with nontemporal hints (the same values).
+template <bool NT = false>
 FOT_HD void face_flux(const double *L, const double *R, uint32_t s, const double *fd,
                       uint32_t fs, double *flux) {
   double cl[5], cr[5], pl[5], pr[5];
@@ -352,14 +373,16 @@ FOT_HD void face_flux(const double *L, const double *R, uint32_t s, const double
   for (int k = 0; k < 5; ++k) {
     double gl = 0, gr = 0;
     for (int d = 0; d < 3; ++d) {
-      gl += (fd[d * fs] - L[(5 + d) * s]) * L[(8 + k * 3 + d) * s];
-      gr += (fd[d * fs] - R[(5 + d) * s]) * R[(8 + k * 3 + d) * s];
+      gl += (load_once<NT>(&fd[d * fs]) - L[(5 + d) * s]) * L[(8 + k * 3 + d) * s];
+      gr += (load_once<NT>(&fd[d * fs]) - R[(5 + d) * s]) * R[(8 + k * 3 + d) * s];
     }
     pl[k] += gl * L[(23 + k) * s];
     pr[k] += gr * R[(23 + k) * s];
   }
   double n[3], t[3], b[3];
-  for (int d = 0; d < 3; ++d) n[d] = fd[(3 + d) * fs], t[d] = fd[(6 + d) * fs], b[d] = fd[(9 + d) * fs];
+  for (int d = 0; d < 3; ++d)
+    n[d] = load_once<NT>(&fd[(3 + d) * fs]), t[d] = load_once<NT>(&fd[(6 + d) * fs]),
+    b[d] = load_once<NT>(&fd[(9 + d) * fs]);
   // the viscous flux (independent of the Roe flux) first, so that the
   // gradients are dead before the Roe flux: the same operations, fewer
   // values live at once
@@ -390,7 +413,7 @@ struct MiniAeroF {
     const double *L = cv + (SOA ? size_t(tgt[0]) : size_t(28) * tgt[0]);
     const double *R = cv + (SOA ? size_t(tgt[1]) : size_t(28) * tgt[1]);
     double flux[5];
-    ma::face_flux(L, R, s, a.cd(0) + ind, a.cs(), flux);
+    ma::face_flux<A::kStreamCells>(L, R, s, a.cd(0) + ind, a.cs(), flux);
 #pragma unroll
     for (int k = 0; k < 5; ++k) inc[k] = -flux[k], inc[5 + k] = flux[k];
   }
@@ -574,6 +597,7 @@ template <int SLOT> struct LdsArgs {
   const EngineArgs &a;
   const double *staged;
   uint32_t m;
+  static constexpr bool kStreamCells = false;
   FOT_DEV const double *pd(uint32_t k) const { return k == uint32_t(SLOT) ? staged : a.pd(k); }
   FOT_DEV uint32_t ps(uint32_t k) const { return k == uint32_t(SLOT) ? m : a.ps(k); }
   FOT_DEV const double *cd(uint32_t k) const { return a.cd(k); }

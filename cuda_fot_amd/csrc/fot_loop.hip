@@ -263,10 +263,13 @@ FOT_DEV void publish(const DepDev &dd, uint32_t b, uint32_t gen) {
 // the most threads a launch may use (a lower bound leaves the compiler more
 // registers per thread).  STG: the functor's staged input (StagedInput) is
 // loaded once per block point into LDS after the cache, and the functor reads
-// it there by block-local id instead of gathering it per cell.
-template <class F, bool SOA, int MODE, int LB = 1024, bool STG = false>
+// it there by block-local id instead of gathering it per cell.  NTC: the
+// functor's read-once cell data loaded with nontemporal hints
+// (StreamCellArgs; the same values).
+template <class F, bool SOA, int MODE, int LB = 1024, bool STG = false, bool NTC = false>
 __global__ __launch_bounds__(LB, LB < 1024 ? kSmallBoundWaves<F> : 1) void eng_hier_kernel(
-    EngineArgs a, HierDev h, DepDev dd, uint32_t blk_begin, uint32_t nbc) {
+    std::conditional_t<NTC, StreamCellArgs, EngineArgs> a, HierDev h, DepDev dd,
+    uint32_t blk_begin, uint32_t nbc) {
   constexpr bool DEP = MODE == 1;
   extern __shared__ double cache[];
   uint32_t gen;
@@ -1212,9 +1215,21 @@ bool reads_iterate(const fot_functor &f) { return f.kernel == FOT_KERNEL_MINE; }
 template <class F, bool SOA, int MODE>
 void launch_eng_hier(bool stage, uint32_t grid, uint32_t threads, uint32_t lds, hipStream_t s,
                      const EngineArgs &a, const HierDev &h, const DepDev &dd, uint32_t b0,
-                     uint32_t nbc) {
+                     uint32_t nbc, bool stream_cells = false) {
   constexpr bool CAN = StagedInput<F>::slot >= 0;
   constexpr int SB = kSmallBound<F>;
+  if constexpr (MODE == 2 && kStreamsCells<F>) {
+    if (stream_cells && !stage) {
+      const StreamCellArgs sa{a};
+      if (threads <= uint32_t(SB))
+        hipLaunchKernelGGL((eng_hier_kernel<F, SOA, 2, SB, false, true>), dim3(grid),
+                           dim3(threads), lds, s, sa, h, dd, b0, nbc);
+      else
+        hipLaunchKernelGGL((eng_hier_kernel<F, SOA, 2, 1024, false, true>), dim3(grid),
+                           dim3(threads), lds, s, sa, h, dd, b0, nbc);
+      return;
+    }
+  }
   if constexpr (SB < 1024) {
     if (threads <= uint32_t(SB)) {
       if constexpr (CAN) {
@@ -1250,6 +1265,16 @@ uint32_t staged_input_dim(const fot_functor &f) {
   return d;
 }
 
+// Whether the functor's two-phase launch has a streaming-cell-data form.
+bool streams_cells(const fot_functor &f) {
+  bool r = false;
+  (void)dispatch(&f, 0, [&]<class F, bool SOA>() -> fot_status {
+    r = kStreamsCells<F>;
+    return FOT_OK;
+  });
+  return r;
+}
+
 struct fot_loop {
   fot_functor f{};
   fot_problem_desc desc{};
@@ -1283,7 +1308,9 @@ struct fot_loop {
   uint32_t stage_per_thread = 1; // owner res_calc: staged points per thread
   bool owned_first_round = false; // owner res_calc: every block owns <= threads cells
   const void *owner3_fn = nullptr; // owner gather: the kernel instantiation to launch
-  bool output_nt = false;          // owner gather: res loaded and stored nontemporally
+  // nontemporal hints on read-once data: owner res_calc gather, the owned
+  // res; two-phase, the streaming functors' cell data
+  bool nontemporal = false;
   uint32_t launch_grid = 0;        // workgroups of the single-launch strategies
   uint32_t ept = 1;
   DepDev dep{};
@@ -1882,25 +1909,30 @@ fot_status fot_loop::build_hierarchical() {
     owned_first_round = most_owned <= threads;
   }
   launch_grid = (dep_order || owner || two_phase) ? nb_total : 0;
+  // nontemporal hints when asked, or by default when the step's data (point
+  // arrays, output, cell data and the plan) exceed the Infinity Cache:
+  // inside it they slow a step down, since every array is read again the
+  // next step
+  uint64_t step_bytes = stats.plan_device_bytes;
+  for (uint32_t k = 0; k < desc.num_mappings; ++k)
+    if (desc.point_data[k])
+      step_bytes += uint64_t(desc.num_points[k]) * desc.point_dim[k] * sizeof(double);
+  step_bytes += uint64_t(desc.num_points[0]) * desc.point_dim[0] * sizeof(double); // out
+  for (uint32_t k = 0; k < desc.num_cell_data; ++k)
+    step_bytes += uint64_t(desc.num_cells) * desc.cell_dim[k] * sizeof(double);
+  const bool want_nt = (cfg.flags & FOT_LOOP_NONTEMPORAL) != 0 ||
+                       (!(cfg.flags & FOT_LOOP_CACHED) && step_bytes > kInfinityCacheBytes);
+  nontemporal = want_nt && two_phase && !stage_pd && streams_cells(f);
   if (owner && f.kernel == FOT_KERNEL_RES_CALC && owner_gather) {
     const bool persistent = (cfg.flags & FOT_LOOP_PERSISTENT) != 0;
     const uint32_t lw = lists_in_lds ? 0u : list_words;
-    // nontemporal res when asked, or by default when the step's data (the
-    // point arrays, res and the plan) exceed the Infinity Cache
-    uint64_t step_bytes = stats.plan_device_bytes;
-    for (uint32_t k = 0; k < desc.num_mappings; ++k)
-      if (desc.point_data[k])
-        step_bytes += uint64_t(desc.num_points[k]) * desc.point_dim[k] * sizeof(double);
-    step_bytes += uint64_t(desc.num_points[0]) * desc.point_dim[0] * sizeof(double); // out
-    output_nt = (cfg.flags & FOT_LOOP_OUTPUT_NONTEMPORAL) != 0 ||
-                (!(cfg.flags & FOT_LOOP_OUTPUT_CACHED) && step_bytes > kInfinityCacheBytes);
-    output_nt = output_nt && lw == 0 && !persistent && !point_terms;
+    nontemporal = want_nt && lw == 0 && !persistent && !point_terms;
     owner3_fn = desc.soa ? owner3_pick_kernel<true>(persistent, point_terms, ept,
                                                     stage_per_thread, lw, owned_first_round,
-                                                    output_nt)
+                                                    nontemporal)
                          : owner3_pick_kernel<false>(persistent, point_terms, ept,
                                                      stage_per_thread, lw, owned_first_round,
-                                                     output_nt);
+                                                     nontemporal);
     if (persistent && nb_total) {
       // as many workgroups as are resident at once, a multiple of the 8 XCDs
       int dev = 0, cus = 0, per_cu = 0;
@@ -1923,7 +1955,7 @@ fot_status fot_loop::build_hierarchical() {
   stats.threads_per_block = threads;
   stats.stage_per_thread = stage_per_thread;
   stats.launch_grid = launch_grid;
-  stats.output_nontemporal = output_nt ? 1u : 0u;
+  stats.nontemporal = nontemporal ? 1u : 0u;
   stats.max_thread_colours = max_tc;
   stats.avg_thread_colours = nb_total ? tc_sum / nb_total : 0.0;
   // (cached may carry padding at its end: count the plan's entries only)
@@ -2041,7 +2073,7 @@ fot_status fot_loop::enqueue(hipStream_t s) {
       }
       if (!done)
         launch_eng_hier<F, SOA, 2>(stage_pd, nb, threads, lds_bytes, s, plan_args, hier, dep,
-                                   0u, nb);
+                                   0u, nb, nontemporal);
       if (merge_n)
         hipLaunchKernelGGL((merge_partials_kernel<F::TD, SOA>), dim3((merge_n + 255) / 256),
                            dim3(256), 0, s, plan_args.out, plan_args.num_points[0], merge_pt,

@@ -532,17 +532,19 @@ typedef struct {
  * in their order: the same bits.  No reference counterpart (the reference
  * launches once per block colour, problem.hpp:276-303). */
 #define FOT_LOOP_TWO_PHASE 1024u
-/* owner compute, res_calc gather (one block per workgroup, lists built in
- * LDS, no per-cell terms): the owned output values are read and written
- * with nontemporal hints (the same bits).  By default this is on when the
- * step's data (point arrays, output and plan) exceed the 256 MB Infinity
- * Cache, where it keeps the read-once output from evicting the point data
- * other blocks read again, and off below, where it slows the step;
- * FOT_LOOP_OUTPUT_NONTEMPORAL forces it on and FOT_LOOP_OUTPUT_CACHED off.
- * fot_loop_stats.output_nontemporal reports the choice.  No reference
- * counterpart. */
-#define FOT_LOOP_OUTPUT_NONTEMPORAL 2048u
-#define FOT_LOOP_OUTPUT_CACHED 4096u
+/* Nontemporal hints on the data a step reads or writes once: the owned
+ * output values of the owner-compute res_calc gather kernel (one block per
+ * workgroup, lists built in LDS, no per-cell terms), and the cell data of
+ * the two-phase hex corner scatter and mini_aero face flux (not staged).
+ * The same values: the same bits.  By default this is on when the step's
+ * data (point arrays, output, cell data and plan) exceed the 256 MB Infinity
+ * Cache, where it keeps the read-once data from evicting the point data
+ * other blocks touch again, and off below, where it slows the step (every
+ * array is read again the next step); FOT_LOOP_NONTEMPORAL forces it on and
+ * FOT_LOOP_CACHED off where it exists.  fot_loop_stats.nontemporal reports
+ * the choice.  No reference counterpart. */
+#define FOT_LOOP_NONTEMPORAL 2048u
+#define FOT_LOOP_CACHED 4096u
 
 typedef struct fot_loop fot_loop;
 /* Builds the plan (host), uploads it, permutes cell data. */
@@ -573,9 +575,8 @@ typedef struct {
   uint32_t stage_per_thread;
   /* single-launch strategies: workgroups per launch */
   uint32_t launch_grid;
-  /* owner res_calc gather: 1 when the output is read and written with
-   * nontemporal hints (FOT_LOOP_OUTPUT_NONTEMPORAL) */
-  uint32_t output_nontemporal;
+  /* 1 when the launch uses nontemporal hints (FOT_LOOP_NONTEMPORAL) */
+  uint32_t nontemporal;
 } fot_loop_stats;
 fot_status fot_loop_stats_get(const fot_loop *loop, fot_loop_stats *stats);
 

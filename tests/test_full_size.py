@@ -33,7 +33,7 @@ def test_res_calc_at_benchmark_size(gpu, nx, ny):
     stats = p.gpu_loop(RES_CALC, capi.STRATEGY_HIERARCHICAL, iters, block_size=ppb, flags=FLAGS)
     # nontemporal res by default only where the step's data exceed the
     # 256 MB Infinity Cache (2400x1200: about 420 MB; 1200x600: about 105 MB)
-    assert stats["output_nontemporal"] == (1 if nx * ny > 2_000_000 else 0)
+    assert stats["nontemporal"] == (1 if nx * ny > 2_000_000 else 0)
     got = p.point_data[0]
     o = oracle_of(ref, O.RES_CALC)
     O.loop_owner_replay(o, plan.owner_plan(ref.maps[0], ref.num_points[0], ppb, None,
@@ -82,11 +82,11 @@ def _lulesh_128():
 @pytest.mark.parametrize("make", [_mine_1024, _hex_128, _mini_aero_96, _lulesh_128],
                          ids=["mine-1024^2", "hex-128^3", "mini_aero-96^3", "lulesh-128^3"])
 def test_other_workloads_at_benchmark_size(gpu, make):
-    # one step of owner compute (240-point parts, as bench.py runs it) and of
-    # the hierarchical loop on the GPS-ordered mesh (B = 256, the autotuner's
-    # pick for the 3D meshes): each bit for bit against the oracle's replay of
-    # its plan and, mapped back to the original numbering, within 1e-12 * S of
-    # the serial oracle loop
+    # one step of owner compute (240-point parts) and of the hierarchical
+    # loop on the GPS-ordered mesh (B = 256) per block colour and two-phase:
+    # each bit for bit against the oracle's replay of its plan and, mapped
+    # back to the original numbering, within 1e-12 * S of the serial oracle
+    # loop
     p, f, kernel = make()
     print(f"{make.__name__}: {p.num_cells} cells", flush=True)
     s = oracle_of(p, kernel)
@@ -104,12 +104,20 @@ def test_other_workloads_at_benchmark_size(gpu, make):
     g = copy.deepcopy(p)
     g.reorder("gps")
     g0 = copy.deepcopy(g)
+    t = copy.deepcopy(g)
     g.gpu_loop(f, capi.STRATEGY_HIERARCHICAL, 1, block_size=256, partition=False)
     h = oracle_of(g0, kernel)
     h.loop_hier_replay(O.OraclePlan(g0.maps, g0.num_points[0], 256, None), 1)
     assert_bitwise(g.point_data[0], h.target, f"{make.__name__} GPS hierarchical vs replay")
     assert_close_scaled(g.target_in_original_order(), s.target, S,
                         what=f"{make.__name__} GPS hierarchical vs serial")
+    # the two-phase write-back of the same plan: the same bits; at these
+    # sizes the hex forces and mini_aero's face values (beyond the Infinity
+    # Cache) are loaded with nontemporal hints by default
+    st = t.gpu_loop(f, capi.STRATEGY_HIERARCHICAL, 1, block_size=256, partition=False,
+                    flags=capi.LOOP_TWO_PHASE)
+    assert_bitwise(t.point_data[0], h.target, f"{make.__name__} GPS two-phase vs replay")
+    assert st["nontemporal"] == (1 if make in (_hex_128, _mini_aero_96) else 0)
 
 
 def test_volna_batch_at_benchmark_size(gpu):

@@ -68,7 +68,7 @@ OG = capi.LOOP_OWNER_GATHER
 HBM = capi.LOOP_LISTS_IN_HBM
 PERS = capi.LOOP_PERSISTENT
 TP = capi.LOOP_TWO_PHASE
-NT = capi.LOOP_OUTPUT_NONTEMPORAL
+NT = capi.LOOP_NONTEMPORAL
 
 
 def _owner_replay(p, kernel, block_size, partition=None, iters=ITERS, point_partition=None):
@@ -141,7 +141,7 @@ def test_res_calc_owner_compute_bitwise(gpu, soa, bs, flags, method):
     # the nontemporal form exists for the gather kernel with lists in LDS,
     # one block per workgroup and no per-cell terms
     nt = bool(flags & NT) and bool(flags & OG) and not flags & (PERS | HBM | TERMS)
-    assert stats["output_nontemporal"] == int(nt)
+    assert stats["nontemporal"] == int(nt)
 
 
 def test_res_calc_gather_edge_on_one_cell_twice(gpu):
@@ -259,10 +259,13 @@ def test_hex_scatter_strategies(gpu, soa):
                                         (capi.STRATEGY_HIERARCHICAL, ref_own, True, OWN),
                                         (capi.STRATEGY_HIERARCHICAL, ref_hier, True, DEP),
                                         (capi.STRATEGY_HIERARCHICAL, ref_hier, True, TP),
+                                        (capi.STRATEGY_HIERARCHICAL, ref_hier, True, TP | NT),
                                         (capi.STRATEGY_GLOBAL_COLOUR, ref_colour, True, 0),
                                         (capi.STRATEGY_ATOMIC, ref_serial, False, 0)]:
         p = hex_problem(soa=soa)
-        p.gpu_loop(f, strategy, ITERS, flags=flags)
+        st = p.gpu_loop(f, strategy, ITERS, flags=flags)
+        # the nontemporal force loads exist for the two-phase launch
+        assert st["nontemporal"] == int(flags == TP | NT)
         if exact:
             assert_bitwise(p.point_data[0], ref, f"hex strategy {strategy}")
         else:

@@ -118,13 +118,18 @@ def test_mini_aero_gpu_strategies(gpu, soa):
 @pytest.mark.parametrize("bs", [128, 512, 1024])
 @pytest.mark.parametrize("flags", [0, capi.LOOP_DEPENDENCY_ORDER, capi.LOOP_TWO_PHASE,
                                    capi.LOOP_STAGE_INPUTS,
-                                   capi.LOOP_STAGE_INPUTS | capi.LOOP_TWO_PHASE],
+                                   capi.LOOP_STAGE_INPUTS | capi.LOOP_TWO_PHASE,
+                                   capi.LOOP_TWO_PHASE | capi.LOOP_NONTEMPORAL,
+                                   capi.LOOP_STAGE_INPUTS | capi.LOOP_TWO_PHASE |
+                                   capi.LOOP_NONTEMPORAL],
                          ids=["per-colour", "dependency-order", "two-phase", "staged",
-                              "staged-two-phase"])
+                              "staged-two-phase", "two-phase-nontemporal",
+                              "staged-two-phase-nontemporal"])
 def test_mini_aero_hierarchical_bitwise(gpu, soa, bs, flags):
     """The reference-shaped hierarchical loop, one launch per block colour,
     one dependency-ordered launch or the two-phase writeback, the cell values
-    gathered per face or staged once per block in LDS, at block sizes that run
+    gathered per face or staged once per block in LDS, the face values
+    loaded plainly or with nontemporal hints, at block sizes that run
     the 512-thread instantiation (128, 512) and the 1024-thread one (1024):
     bit for bit against the oracle's replay of the same plan.  fot_loop_run reports a
     dependency wait that ran out as an error, so a pass also shows that every
@@ -135,5 +140,11 @@ def test_mini_aero_hierarchical_bitwise(gpu, soa, bs, flags):
     ref = oracle_of(base, O.MINI_AERO)
     ref.loop_hier_replay(O.OraclePlan(base.maps, base.num_points[0], bs), iters)
     p = mini_aero_problem(m, block_size=bs, soa=soa)
-    p.gpu_loop(MINI_AERO, capi.STRATEGY_HIERARCHICAL, iters, flags=flags)
+    st = p.gpu_loop(MINI_AERO, capi.STRATEGY_HIERARCHICAL, iters, flags=flags)
     assert_bitwise(p.point_data[0], ref.target, f"mini_aero hierarchical b{bs} flags {flags}")
+    # the nontemporal face-value loads: two-phase with the values gathered
+    # per face (when staging fits, the staged form has none)
+    if flags & capi.LOOP_NONTEMPORAL and not flags & capi.LOOP_STAGE_INPUTS:
+        assert st["nontemporal"] == 1
+    if not flags & capi.LOOP_NONTEMPORAL:
+        assert st["nontemporal"] == 0
