@@ -83,7 +83,7 @@ def parse():
                          "plan in HBM instead of building them in LDS")
     ap.add_argument("--nontemporal", default="auto", choices=["auto", "on", "off"],
                     help="nontemporal hints on read-once data (res_calc owner gather: the "
-                         "owned res; hex and mini_aero two-phase: the cell data); auto: when "
+                         "owned res; hex, mini_aero and lulesh two-phase: the cell data); auto: when "
                          "the step's data exceed the 256 MB Infinity Cache")
     ap.add_argument("--point-terms", action="store_true",
                     help="owner compute: 1/rho and pressure once per staged cell")

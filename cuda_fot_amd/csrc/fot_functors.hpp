@@ -132,7 +132,8 @@ struct EngineArgs {
 };
 
 // The same arguments, with the functors that stream their cell data (the
-// hex corner forces, mini_aero's face values: read once per step) loading it
+// hex corner forces, mini_aero's face values, lulesh's stresses: read once
+// per step) loading it
 // with nontemporal hints, so that on data beyond the Infinity Cache it does
 // not evict the point data and outputs other blocks touch again.  The same
 // values: the same bits.
@@ -505,7 +506,9 @@ struct LuleshF {
     const double *sig = a.cd(0) + ind;
     const uint32_t cs = a.cs();
     double f[24], det;
-    lu::element(x, y, z, sig[0], sig[cs], sig[2 * size_t(cs)], f, &det);
+    constexpr bool NT = A::kStreamCells;
+    lu::element(x, y, z, load_once<NT>(sig), load_once<NT>(&sig[cs]),
+                load_once<NT>(&sig[2 * size_t(cs)]), f, &det);
     const_cast<double *>(a.cd(1))[ind] = det;
 #pragma unroll
     for (uint32_t j = 0; j < 8; ++j)

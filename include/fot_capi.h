@@ -535,7 +535,8 @@ typedef struct {
 /* Nontemporal hints on the data a step reads or writes once: the owned
  * output values of the owner-compute res_calc gather kernel (one block per
  * workgroup, lists built in LDS, no per-cell terms), and the cell data of
- * the two-phase hex corner scatter and mini_aero face flux (not staged).
+ * the two-phase hex corner scatter, mini_aero face flux and lulesh element
+ * forces (not staged).
  * The same values: the same bits.  By default this is on when the step's
  * data (point arrays, output, cell data and plan) exceed the 256 MB Infinity
  * Cache, where it keeps the read-once data from evicting the point data

@@ -112,12 +112,12 @@ def test_other_workloads_at_benchmark_size(gpu, make):
     assert_close_scaled(g.target_in_original_order(), s.target, S,
                         what=f"{make.__name__} GPS hierarchical vs serial")
     # the two-phase write-back of the same plan: the same bits; at these
-    # sizes the hex forces and mini_aero's face values (beyond the Infinity
-    # Cache) are loaded with nontemporal hints by default
+    # sizes the hex forces, mini_aero's face values and lulesh's stresses
+    # (beyond the Infinity Cache) are loaded with nontemporal hints by default
     st = t.gpu_loop(f, capi.STRATEGY_HIERARCHICAL, 1, block_size=256, partition=False,
                     flags=capi.LOOP_TWO_PHASE)
     assert_bitwise(t.point_data[0], h.target, f"{make.__name__} GPS two-phase vs replay")
-    assert st["nontemporal"] == (1 if make in (_hex_128, _mini_aero_96) else 0)
+    assert st["nontemporal"] == (1 if make in (_hex_128, _mini_aero_96, _lulesh_128) else 0)
 
 
 def test_volna_batch_at_benchmark_size(gpu):

@@ -125,11 +125,15 @@ def test_lulesh_gpu_strategies(gpu, soa):
                                      hier.target),
                                     (capi.STRATEGY_HIERARCHICAL,
                                      capi.LOOP_TWO_PHASE | capi.LOOP_STAGE_INPUTS, hier.target),
+                                    # the stresses loaded with nontemporal hints
+                                    (capi.STRATEGY_HIERARCHICAL,
+                                     capi.LOOP_TWO_PHASE | capi.LOOP_NONTEMPORAL, hier.target),
                                     (capi.STRATEGY_HIERARCHICAL, capi.LOOP_OWNER_COMPUTE,
                                      owner.target)]:
         p = lulesh_problem(14, 12, 11, seed=4, block_size=bs, soa=soa)
-        p.gpu_loop(LULESH, strategy, iters, flags=flags, partition=False)
+        st = p.gpu_loop(LULESH, strategy, iters, flags=flags, partition=False)
         what = f"lulesh {strategy}/{flags}"
+        assert st["nontemporal"] == int(bool(flags & capi.LOOP_NONTEMPORAL))
         assert_close_scaled(p.point_data[0], ser.target, S, what=what)
         if expect is not None:
             assert_bitwise(p.point_data[0], expect, what + " bits")
