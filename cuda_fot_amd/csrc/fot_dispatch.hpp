@@ -79,8 +79,8 @@ constexpr int kSmallBoundWaves = std::is_same_v<F, MiniAeroF> ? 3 : 1;
 // read-once cell data with nontemporal hints (StreamCellArgs): the hex
 // corner forces (192 B per element), mini_aero's face values (96 B per
 // face) and lulesh's stresses.  (getacc's corner data and mine's edge
-// weights measured 21 % and 13 % slower that way at 1024^2, so they have
-// none.)
+// weights measured 21 % and 13 % slower that way at 1024^2, and volna's edge
+// values in the 64-mesh batch 1.5 % slower, so they have none.)
 template <class F>
 constexpr bool kStreamsCells = std::is_same_v<F, HexScatterF> || std::is_same_v<F, MiniAeroF> ||
                                std::is_same_v<F, LuleshF>;
