@@ -41,7 +41,6 @@ __global__ __launch_bounds__(64 * kGemmWaves) void gemm_mfma_kernel(
   const uint32_t r = lane & 15u, q = lane >> 4;
   const uint32_t am = m0 + r, bn = n0 + r;
   v4f acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
   for (uint32_t k0 = 4u * wave; k0 < K; k0 += 4u * kGemmWaves) {
     const uint32_t k = k0 + q;
     const float a = (am < M && k < K) ? A[size_t(am) * sam + size_t(k) * sak] : 0.f;
