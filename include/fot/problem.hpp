@@ -30,8 +30,10 @@ template <bool SOA = false> struct Problem {
   std::vector<MY_SIZE> applied_permutation;
   // loopGPUHierarchical's plan: FOT_LOOP_* flags (default: owner compute, two
   // cells per thread, and for res_calc the gather accumulation); 0 gives the
-  // reference's per-block-colour launches, the faster choice for 3D element
-  // loops on a reorder()ed mesh
+  // reference's per-block-colour launches, and FOT_LOOP_TWO_PHASE the same
+  // plan and bits in two launches (every block at once, shared points merged
+  // in block-colour order), the faster choice for 3D element loops on a mesh
+  // reorder()ed and partition(tolerance, true)ed into compact blocks
   std::uint32_t hierarchical_flags =
       FOT_LOOP_OWNER_COMPUTE | FOT_LOOP_TWO_PER_THREAD | FOT_LOOP_OWNER_GATHER;
 
@@ -129,16 +131,23 @@ template <bool SOA = false> struct Problem {
   }
   // partition: k-way partition of the cell-to-cell graph into parts of at
   // most floor(block_size / tolerance) cells (partitionMetisEnh,
-  // partition.hpp:23-29, with libfot's partitioner in place of METIS)
-  void partition(float tolerance) {
+  // partition.hpp:23-29, with libfot's partitioner in place of METIS).
+  // bisect (no reference counterpart): recursive bisection into equal,
+  // compact parts instead of greedy growing
+  void partition(float tolerance, bool bisect = false) {
     std::vector<MY_SIZE> ind, end;
     cellGraph(ind, end);
     const MY_SIZE max_part = std::max<MY_SIZE>(1, MY_SIZE(std::floor(block_size / tolerance)));
     partition_vector.assign(mesh.numCells(), 0);
     std::uint32_t nparts = 0;
-    fot::check(fot_kway_partition(ind.data(), end.data(), mesh.numCells(), max_part,
-                                  partition_vector.data(), &nparts),
-               "fot_kway_partition");
+    if (bisect)
+      fot::check(fot_bisect_partition(ind.data(), end.data(), mesh.numCells(), max_part,
+                                      partition_vector.data(), &nparts),
+                 "fot_bisect_partition");
+    else
+      fot::check(fot_kway_partition(ind.data(), end.data(), mesh.numCells(), max_part,
+                                    partition_vector.data(), &nparts),
+                 "fot_kway_partition");
   }
   // partitionOurs (partition.cu:42-215): coarse parts of block_size *
   // num_blocks_in_coarse cells, then the heuristic partition inside each

@@ -96,13 +96,13 @@ template <bool SOA> Problem<SOA> mine_grid() {
 }
 
 // hex corner-force scatter on a 5 x 4 x 3 point grid, forces N(0,1)
-template <bool SOA> Problem<SOA> hex_grid() {
+template <bool SOA> Problem<SOA> hex_grid(MY_SIZE block_size = 128) {
   const std::uint32_t n1 = 5, n2 = 4, n3 = 3, np = n1 * n2 * n3,
                       nc = (n1 - 1) * (n2 - 1) * (n3 - 1);
   std::vector<std::uint32_t> c(8 * nc);
   fot::check(fot_gen_hex_grid(n1, n2, n3, c.data()), "fot_gen_hex_grid");
   Problem<SOA> p(Mesh({np}, nc, {c.data()}, {8}), {{3, sizeof(double)}}, {{24, sizeof(double)}},
-                 128);
+                 block_size);
   std::vector<double> F(24 * nc);
   fot::check(fot_gen_normal(F.size(), 9, 0.0, 1.0, F.data()), "fot_gen_normal");
   std::copy(F.begin(), F.end(), p.cell_weights[0].template begin<double>());
@@ -334,6 +334,18 @@ template <bool SOA> void gpu_cases(const char *tag) {
     auto p = hex_grid<SOA>();
     p.template loopGPUCellCentred<hex_scatter::StepGPUGlobal>(2);
     dump(p, "hex_global_" + t);
+  }
+  // the reference's per-block-colour plan on a reordered, bisected mesh, and
+  // the same plan run by the two-phase write-back: the same bits
+  for (std::uint32_t flags : {0u, std::uint32_t(FOT_LOOP_TWO_PHASE)}) {
+    auto p = hex_grid<SOA>(7);  // 24 elements in 4 blocks: most points shared
+    p.hierarchical_flags = flags;
+    p.reorder();
+    p.partition(1.001f, true);
+    p.reorderToPartition();
+    p.renumberPoints();
+    p.template loopGPUHierarchical<hex_scatter::StepGPUHierarchical>(2);
+    dump(p, (flags ? "hex_2phase_" : "hex_hier_") + t);
   }
   {
     auto p = volna_mesh<SOA>();

@@ -154,6 +154,10 @@ def test_cpp_dropin_gpu_loops(gpu, tmp_path):
         assert_close_scaled(_load(tmp_path, "mine_owner_" + tag, 4), ms, mS, what="mine " + tag)
         assert_close_scaled(_load(tmp_path, "hex_atomic_" + tag, 3), hs, hS, what="hex " + tag)
         assert_bitwise(_load(tmp_path, "hex_global_" + tag, 3), hc, "hex global " + tag)
+        # per-colour launches and the two-phase write-back of one bisected plan
+        hh = _load(tmp_path, "hex_hier_" + tag, 3)
+        assert_close_scaled(hh, hs, hS, what="hex hierarchical " + tag)
+        assert_bitwise(_load(tmp_path, "hex_2phase_" + tag, 3), hh, "hex two-phase " + tag)
         assert_close_scaled(_load(tmp_path, "volna_owner_" + tag, 4), vs, vS,
                             what="volna owner " + tag)
         assert_close_scaled(_load(tmp_path, "ma_owner_" + tag, 5), mas, maS,
