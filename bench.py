@@ -524,9 +524,10 @@ def bench_volna_batch(args):
 def tune_res_calc(args) -> dict:
     """The autotuner's choice for the airfoil as generated (shuffled, not
     reordered): the learned cost model's shortlist of strategy classes,
-    their configurations measured, with the fallback to the measured search
-    when the fastest misses its class's prediction by more than 5 % (or all
-    21 configurations measured, with --tune exhaustive).  Sets args to the
+    their configurations measured; when the fastest misses its class's
+    prediction by more than 5 %, further classes measured one at a time, or
+    all 21 past a miss of 30 % (tuner.choose); --tune exhaustive measures
+    all 21.  Sets args to the
     chosen configuration (its ordering, block or part size and launch
     mode); returns the tuning record."""
     from cuda_fot_amd import capi, tuner

@@ -33,8 +33,10 @@ mesh features (fot_loop_features).  `Tuner.learned` measures only the
 configurations of the classes the model predicts within the tolerance (5 %)
 of its best prediction (the shortlist) and takes the fastest; when that
 one's measurement exceeds the model's prediction for its class by more than
-the tolerance, or nothing on the shortlist can be planned, it falls back to
-the measured search of the rest.
+the tolerance, it measures further classes one at a time (those predicted
+below the best time measured so far), or, when the miss exceeds
+WIDEN_LIMIT or nothing on the shortlist can be planned, all the rest
+(`choose`).
 """
 from __future__ import annotations
 
@@ -286,19 +288,65 @@ def members(classes) -> list[int]:
     return [int(i) for i in np.nonzero(np.isin(CLASS_OF, list(classes)))[0]]
 
 
+# A miss of the picked class's prediction by more than this factor means the
+# model does not know the mesh: the measured search over all 21.  A smaller
+# miss widens the shortlist one class at a time instead.
+WIDEN_LIMIT = 1.3
+
+
+def choose(pred_cls, measure, tolerance: float = 0.05):
+    """The learned choice, measuring through `measure(classes)`, which
+    returns the 21 configuration times with those classes' configurations
+    filled in (ms; -1 = cannot be planned).
+
+    The shortlisted classes are measured.  The fastest of them is kept when
+    it is within `tolerance` of its class's prediction.  When it misses by
+    more than WIDEN_LIMIT (or nothing on the shortlist can be planned), every
+    configuration is measured and the fastest kept.  Otherwise the class
+    predicted fastest among those not yet measured and predicted below the
+    best time measured so far (within the tolerance) is measured, and the
+    test repeats, until the pick is within its prediction or no such class
+    is left.  Returns (configuration, measured beyond the shortlist,
+    configurations measured, times)."""
+    pred = np.asarray(pred_cls, np.float64)
+    short = shortlist(pred, tolerance)
+    done = list(short)
+    t = np.asarray(measure(short), np.float64)
+    count = len(members(short))
+    while True:
+        subs = members(done)
+        ok = np.where(t[subs] > 0, t[subs], np.inf)
+        pick = subs[int(np.argmin(ok))]
+        best = float(ok.min())
+        limit = pred[CLASS_OF[pick]]
+        if np.isfinite(best) and best <= (1.0 + tolerance) * limit:
+            return pick, len(done) > len(short), count, t
+        if len(done) == len(short) and not (np.isfinite(best) and best <= WIDEN_LIMIT * limit):
+            t = np.asarray(measure([c for c in range(len(CLASSES)) if c not in done]), np.float64)
+            return int(np.argmin(np.where(t > 0, t, np.inf))), True, NUM_CANDIDATES, t
+        rest = [int(c) for c in np.argsort(pred, kind="stable")
+                if int(c) not in done and pred[c] < (1.0 + tolerance) * best]
+        if not rest:
+            return pick, len(done) > len(short), count, t
+        t = np.asarray(measure([rest[0]]), np.float64)
+        done.append(rest[0])
+        count += len(members([rest[0]]))
+
+
 def learned_choice(pred_cls, measured_ms, tolerance: float = 0.05):
-    """The learned choice on known measurements (21 configuration times):
-    the fastest configuration of the shortlisted classes; if it exceeds the
-    model's prediction for its class by more than `tolerance` (or nothing on
-    the shortlist can be planned), the measured best of all.  Returns
-    (configuration, fell back, configurations measured)."""
+    """choose() on known measurements (the 21 configuration times): returns
+    (configuration, measured beyond the shortlist, configurations
+    measured)."""
     m = np.asarray(measured_ms, np.float64)
-    ok = np.where(m > 0, m, np.inf)
-    subs = members(shortlist(pred_cls, tolerance))
-    pick = subs[int(np.argmin(ok[subs]))]
-    if not (m[pick] > 0) or m[pick] > (1.0 + tolerance) * pred_cls[CLASS_OF[pick]]:
-        return int(np.argmin(ok)), True, NUM_CANDIDATES
-    return pick, False, len(subs)
+    seen = np.full(m.shape, np.nan)
+
+    def reveal(classes):
+        idx = members(classes)
+        seen[idx] = m[idx]
+        return seen
+
+    pick, widened, count, _ = choose(pred_cls, reveal, tolerance)
+    return pick, widened, count
 
 
 class Tuner:
@@ -312,22 +360,23 @@ class Tuner:
         return int(np.argmin(ok)), t
 
     def learned(self, problem, func, **kw) -> dict:
+        """choose() on the problem: the configurations measured as deployed.
+        The record: pick, predicted class times, shortlist, whether classes
+        beyond the shortlist were measured (fell_back), how many
+        configurations were measured and their times (NaN: not measured)."""
         pred = self.model.predict(features(problem, func))[0]
-        short = shortlist(pred, self.tolerance)
-        subs = members(short)
-        t = measure(problem, func, which=subs, **kw)
-        ok = np.where(t[subs] > 0, t[subs], np.inf)
-        pick = subs[int(np.argmin(ok))]
-        res = {"pick": pick, "predicted": pred, "shortlist": short,
-               "measured_pick": float(t[pick]), "fell_back": False, "measured": len(subs),
-               "times": t}
-        if not (t[pick] > 0) or t[pick] > (1.0 + self.tolerance) * pred[CLASS_OF[pick]]:
-            rest = [i for i in range(NUM_CANDIDATES) if i not in subs]
-            t2 = measure(problem, func, which=rest, **kw)
-            t[rest] = t2[rest]
-            best = int(np.argmin(np.where(t > 0, t, np.inf)))
-            res.update(pick=best, fell_back=True, measured=NUM_CANDIDATES, times=t)
-        return res
+        times = np.full(NUM_CANDIDATES, np.nan)
+
+        def run(classes):
+            idx = members(classes)
+            t = measure(problem, func, which=idx, **kw)
+            times[idx] = t[idx]
+            return times
+
+        pick, widened, count, t = choose(pred, run, self.tolerance)
+        return {"pick": pick, "predicted": pred, "shortlist": shortlist(pred, self.tolerance),
+                "measured_pick": float(t[pick]), "fell_back": bool(widened),
+                "measured": int(count), "times": t}
 
 
 # ---- evaluation ---------------------------------------------------------------

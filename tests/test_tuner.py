@@ -94,11 +94,21 @@ def test_learned_choice_closed_form():
     n34 = n3 + int((tuner.CLASS_OF == 4).sum())
     assert tuner.shortlist([9, 9, 9, 2.05, 2.0, 3]) == [3, 4]
     assert tuner.learned_choice(np.array([9, 9, 9, 2.05, 2.0, 3]), meas) == (best3, False, n34)
-    # class 4 predicted 1.9, measured best 2.1 > 1.05 * 1.9: measured search
-    assert tuner.learned_choice(np.array([9, 9, 9, 3, 1.9, 3]), meas) == \
+    best4 = int(np.nonzero(tuner.CLASS_OF == 4)[0][0])
+    n4 = int((tuner.CLASS_OF == 4).sum())
+    # class 4 predicted 1.9, measured best 2.1 > 1.05 * 1.9 (a miss of 1.105,
+    # below the widening limit): no other class is predicted below
+    # 1.05 * 2.1, so class 4's best is kept after measuring it alone
+    assert tuner.learned_choice(np.array([9, 9, 9, 3, 1.9, 3]), meas) == (best4, False, n4)
+    # the same miss with class 3 predicted 2.15 < 1.05 * 2.1: class 3 is
+    # measured next; its 2.0 is within 5 % of its prediction and is kept
+    assert tuner.learned_choice(np.array([9, 9, 9, 2.15, 1.9, 3]), meas) == \
+        (best3, True, n4 + n3)
+    # a miss beyond the widening limit (2.1 > 1.3 * 1.5): measured search
+    assert tuner.WIDEN_LIMIT == 1.3
+    assert tuner.learned_choice(np.array([9, 9, 9, 3, 1.5, 3]), meas) == \
         (best3, True, tuner.NUM_CANDIDATES)
     # exactly at the tolerance stays
-    best4 = int(np.nonzero(tuner.CLASS_OF == 4)[0][0])
     assert tuner.learned_choice(np.array([9, 9, 9, 9, 2.0, 9]), meas)[:2] == (best4, False)
     # nothing on the shortlist can be planned: measured search
     m2 = meas.copy()
@@ -165,8 +175,14 @@ def test_measure_and_learned_tuner(gpu):
     assert res["shortlist"] == tuner.shortlist(pred, 0.5)
     assert int(np.argmin(pred)) in res["shortlist"]
     assert res["fell_back"] or tuner.CLASS_OF[res["pick"]] in res["shortlist"]
-    assert res["measured"] == (tuner.NUM_CANDIDATES if res["fell_back"]
-                               else len(tuner.members(res["shortlist"])))
+    n_short = len(tuner.members(res["shortlist"]))
+    if res["fell_back"]:
+        assert n_short < res["measured"] <= tuner.NUM_CANDIDATES
+    else:
+        assert res["measured"] == n_short
+    # exactly the measured configurations have times
+    assert int(np.isfinite(res["times"]).sum()) == res["measured"]
+    assert np.isfinite(res["times"][res["pick"]])
 
 
 CORPUS = tuner.CORPUS_PATH
