@@ -107,6 +107,10 @@ def parse():
                          "block colour) at block sizes 128-1024")
     ap.add_argument("--soa", action="store_true",
                     help="res_calc: point data in SOA layout (d * n + i) instead of AOS")
+    ap.add_argument("--share", type=int, default=0,
+                    help="res_calc: time, on this GPU, each share of a SHARE-way partition "
+                         "of the cells (owned cells, the edges touching them, their halo) "
+                         "as one GPU of a sharded run would run it")
     args = ap.parse_args()
     # the 3D element loops, getacc and mine default to the two-phase
     # write-back on recursive-bisection blocks (the autotuner's pick on hex
@@ -521,6 +525,67 @@ def bench_volna_batch(args):
                         "lds_bytes": st["lds_bytes"]})
 
 
+def share_mesh(mesh: dict, part: np.ndarray, s: int):
+    """The submesh one GPU runs for share s of a partition of the cells:
+    the owned cells (numbered first), the edges touching them, and the halo
+    cells and nodes those edges read.  Returns (mesh dict, owned, halo)."""
+    pecell, pedge = mesh["pecell"], mesh["pedge"]
+    own = part == s
+    emask = own[pecell[:, 0]] | own[pecell[:, 1]]
+    e_cells, e_nodes = pecell[emask], pedge[emask]
+    owned = np.nonzero(own)[0]
+    halo = np.setdiff1d(np.unique(e_cells), owned)
+    cells = np.concatenate([owned, halo])
+    cmap = np.full(mesh["num_cells"], -1, np.int64)
+    cmap[cells] = np.arange(cells.size)
+    nodes = np.unique(e_nodes)
+    nmap = np.full(mesh["num_nodes"], -1, np.int64)
+    nmap[nodes] = np.arange(nodes.size)
+    sub = {"pecell": cmap[e_cells].astype(np.uint32), "pedge": nmap[e_nodes].astype(np.uint32),
+           "x": mesh["x"][nodes], "q": mesh["q"][cells], "adt": mesh["adt"][cells],
+           "num_cells": int(cells.size), "num_nodes": int(nodes.size),
+           "num_edges": int(emask.sum())}
+    return sub, int(owned.size), int(halo.size)
+
+
+def bench_share(args) -> dict:
+    """Each share of a G-way partition of the airfoil's cells (recursive
+    bisection of the cell graph into equal parts), run on this GPU as one
+    GPU of a sharded run would run it: its owned cells, every edge touching
+    one of them (edges across the cut are computed on both sides) and the
+    halo cells and nodes they read, with the default plan (owner compute on
+    240-point parts, the gather kernel).  The halo cells are owned by blocks
+    of the share's plan too, so their (discarded) sums are written: O(cut)
+    extra work.  Reports each share's time per step and its halo, the
+    q / adt values a timestep would refresh from the neighbours."""
+    from cuda_fot_amd import capi, meshgen, plan, res_calc_problem
+    g = args.share
+    mesh = meshgen.airfoil(args.nx, args.ny, seed=0, shuffle=7)
+    ind, end = plan.graph_csr(mesh["pecell"], mesh["num_cells"])
+    part, nparts = plan.bisect_partition(ind, end, -(-mesh["num_cells"] // g))
+    shares = []
+    for s in range(nparts):
+        sub, owned, halo = share_mesh(mesh, part, s)
+        p = res_calc_problem(sub)
+        p.partition_points(240)
+        loop, _, _ = make_loop(p, "hierarchical", 512, True, False, "owner", 2, True, False,
+                               True, False, False, args.nontemporal)
+        wall, ev = time_loop(loop, args.steps, args.warmup)
+        loop.close()
+        shares.append({"owned_cells": owned, "halo_cells": halo, "edges": sub["num_edges"],
+                       "us_per_step": round(ev / args.steps * 1e6, 2),
+                       "halo_refresh_bytes": 40 * halo})
+    slowest = max(sh["us_per_step"] for sh in shares)
+    return {"metric": f"res_calc: the {nparts} shares of a {nparts}-way cell partition, each "
+                      "timed alone on one GPU",
+            "workload": f"res_calc airfoil {args.nx}x{args.ny}, shuffled, seed 0",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "shares": shares,
+            "slowest_share_us_per_step": slowest,
+            "edges_total": mesh["num_edges"],
+            "edges_computed": int(sum(sh["edges"] for sh in shares)),
+            "note": "one GPU runs each share in turn; no multi-GPU run is measured"}
+
+
 def tune_res_calc(args) -> dict:
     """The autotuner's choice for the airfoil as generated (shuffled, not
     reordered): the learned cost model's shortlist of strategy classes,
@@ -782,6 +847,9 @@ def main():
         return
     if args.workload in ("mini_aero", "hex", "lulesh", "mine", "getacc"):
         print(json.dumps(bench_single(args, args.workload)), flush=True)
+        return
+    if args.share > 1:
+        print(json.dumps(bench_share(args)), flush=True)
         return
     tune = None
     if (args.tune or "learned") != "none":

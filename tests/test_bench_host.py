@@ -71,3 +71,35 @@ def test_res_calc_traffic_matches_configuration(monkeypatch):
     # another part size has no summary
     c = _parse(["--tune", "none", "--points-per-block", "200"], monkeypatch)
     assert bench.load_traffic(c, {"nontemporal": 0}) is None
+
+
+def test_share_mesh_covers_the_mesh_once_per_owner():
+    # the shares of a 4-way cell partition: owned cells partition the mesh,
+    # every edge is in the share of each of its cells' owners (twice across
+    # the cut, once inside), halos are the other cells those edges touch,
+    # and every submesh id is in range
+    import numpy as np
+    from cuda_fot_amd import meshgen, plan
+    mesh = meshgen.airfoil(60, 40, seed=1, shuffle=7)
+    ind, end = plan.graph_csr(mesh["pecell"], mesh["num_cells"])
+    part, n = plan.bisect_partition(ind, end, -(-mesh["num_cells"] // 4))
+    assert n == 4
+    pe = mesh["pecell"]
+    cut = int((part[pe[:, 0]] != part[pe[:, 1]]).sum())
+    owned_total, edges_total = 0, 0
+    for s in range(n):
+        sub, owned, halo = bench.share_mesh(mesh, part, s)
+        owned_total += owned
+        edges_total += sub["num_edges"]
+        assert sub["pecell"].max() < sub["num_cells"] == owned + halo
+        assert sub["pedge"].max() < sub["num_nodes"]
+        # every edge of the share touches an owned cell (numbered first)
+        assert np.all((sub["pecell"] < owned).any(axis=1))
+        # the halo cells are exactly the non-owned cells its edges read
+        assert np.array_equal(np.unique(sub["pecell"][sub["pecell"] >= owned]),
+                              np.arange(owned, owned + halo))
+        # data follow the renumbering
+        mine = np.nonzero(part == s)[0]
+        assert np.array_equal(sub["q"][:owned], mesh["q"][mine])
+    assert owned_total == mesh["num_cells"]
+    assert edges_total == mesh["num_edges"] + cut
