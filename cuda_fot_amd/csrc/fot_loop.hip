@@ -576,15 +576,19 @@ __global__ __launch_bounds__(1024) void batch_owner_kernel(const BatchProblem *_
                                                            uint32_t nwork,
                                                            uint32_t *__restrict__ queue) {
   extern __shared__ double lds[];
-  __shared__ uint32_t item;
-  for (;;) {
-    __syncthreads(); // the previous item's LDS and `item` are no longer read
-    if (threadIdx.x == 0) item = atomicAdd(&queue[0], 1u);
-    __syncthreads();
-    const uint32_t w = item;
-    if (w >= nwork) break;
+  // the next item is taken while the current one runs, so the queue
+  // atomic's round trip overlaps the work instead of preceding each item
+  __shared__ uint32_t item[2];
+  if (threadIdx.x == 0) item[0] = atomicAdd(&queue[0], 1u);
+  __syncthreads();
+  uint32_t w = item[0], buf = 0;
+  while (w < nwork) {
+    if (threadIdx.x == 0) item[buf ^ 1u] = atomicAdd(&queue[0], 1u);
     const uint2 pb = work[w];
     owner_block<F, SOA>(probs[pb.x].args, probs[pb.x].hier, pb.y, lds);
+    __syncthreads(); // this item's LDS no longer read; the next index visible
+    buf ^= 1u;
+    w = item[buf];
   }
   if (threadIdx.x == 0) {
     __threadfence();
