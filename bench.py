@@ -832,7 +832,8 @@ def main():
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the engine has no multi-GPU path; an "
               "owner-compute plan sharded by point partition would need no result "
-              "exchange, only a halo refresh of the read-only inputs", file=sys.stderr)
+              "exchange, only a halo refresh of the read-only inputs (--share G times "
+              "each share of a G-way partition on this GPU)", file=sys.stderr)
         sys.exit(2)
     from cuda_fot_amd import capi
     if capi.device_count() < 1:
